@@ -1,0 +1,127 @@
+"""Benchmark of the Yee timestep on one MI355X.
+
+Workload: 3D Yee + CPML, FP32, 256^3 nodes, CPML 12 cells on all faces,
+TF/SF plane wave along +x (FP64 aux-1D line, Ez-polarised Gaussian-modulated
+pulse), centred eps_r = 6 cube of side 64, 8 Ez probes recorded every step.
+A step is one full timestep (H half step + E half step) of the whole grid.
+
+Prints ONE JSON line: Gcell-updates/s (whole job), the roofline of the
+dominant kernel (algorithmic bytes per launch / average launch duration,
+timed with HIP events on the simulation's stream) and a CPU baseline (the
+FP64 reference-structured oracle on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(args):
+    """FP64 oracle (reference loop structure, OpenMP) on a bounded sample:
+    the full 256^3 grid of the same workload for a few timesteps."""
+    from luxum_amd import configs
+    from oracle.lxoracle import OracleSim
+    threads = min(os.cpu_count() or 1, 16)
+    w = configs.config1(n=args.n, steps=args.cpu_steps)
+    o = OracleSim(w.p, threads=threads)
+    o.set_materials(w.ids, w.eps_r)
+    o.set_probes(w.probes)
+    o.step(1)
+    t0 = time.perf_counter()
+    o.step(args.cpu_steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    gcells = w.cells * args.cpu_steps / dt / 1e9
+    return {"value": round(gcells, 5), "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 reference-structured oracle, {args.n}^3 grid of the same workload, "
+                      f"{args.cpu_steps} timesteps ({dt:.1f} s), {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--fp16", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-steps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the whole domain is resident on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    from luxum_amd import configs
+    from luxum_amd.params import LX_FP16, LX_FP32
+    from luxum_amd.sim import Context, Simulation
+
+    prec = LX_FP16 if args.fp16 else LX_FP32
+    w = configs.config1(n=args.n, steps=max(args.steps + args.warmup + args.kernel_steps, 2000), precision=prec)
+    ctx = Context(0)
+    sim = Simulation(w.p, ctx)
+    sim.set_materials(w.ids, w.eps_r)
+    sim.set_probes(w.probes)
+    print(f"bench: {w.name} {'fp16' if args.fp16 else 'fp32'} warmup {args.warmup}", file=sys.stderr, flush=True)
+    sim.step(args.warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    elapsed = time.perf_counter() - t0
+    ms_step = elapsed * 1e3 / args.steps
+    gcells = w.cells * args.steps / elapsed / 1e9
+    print(f"bench: timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    # per-kernel durations with HIP events on the simulation's stream
+    ms_h, ms_e, ms_ev_step = sim.time_kernels(args.kernel_steps)
+    info = sim.info()
+    s = 2 if args.fp16 else 4
+    slab = sum(info.slab_cells)
+    bytes_h = info.cells * 9 * s + slab * 16
+    bytes_e = info.cells * (9 * s + 1) + slab * 16
+    if ms_e >= ms_h:
+        dom, dbytes, dms = "e_update_3d", bytes_e, ms_e
+    else:
+        dom, dbytes, dms = "h_update_3d", bytes_h, ms_h
+    achieved = dbytes / (dms * 1e-3) / 1e9
+    step_gbs = info.bytes_per_step / (ms_step * 1e-3) / 1e9
+    mx, _ = sim.monitor()
+
+    out = {
+        "metric": "Gcell-updates/s, %HBM roofline; batched solves/s, p50/p99 @ fixed L2 err",
+        "value": round(gcells, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16-storage/f32" if args.fp16 else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"3D Yee+CPML {args.n}^3, CPML 12, TF/SF aux-1D plane wave +x, eps_r=6 cube, "
+                               f"8 probes, {'FP16 storage' if args.fp16 else 'FP32'}",
+                   "cells": info.cells, "bytes_per_step": info.bytes_per_step,
+                   "step_GBps": round(step_gbs, 1), "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                   "ms_h": round(ms_h, 5), "ms_e": round(ms_e, 5), "max_E2": mx},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": dbytes, "ms_per_launch": round(dms, 5)},
+    }
+    sim.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,224 @@
+/*
+ * luxum.h -- C ABI of the MI355X-native Yee FDTD timestep.
+ *
+ * The reference (Luxum) has no FFI layer: its boundary is the C++ class API
+ * that main() drives.  Each entry point below names the reference interface
+ * it replaces.  All entry points are extern "C", take plain pointers and
+ * sizes, never throw and never exit(); errors are status codes with a
+ * thread-local message from lx_last_error().
+ *
+ * Conventions shared with the reference:
+ *   - nx, ny, nz are NODE counts ("X/Y/Z Length", lib/parallel/grid.cc:83-85).
+ *   - Field component extents follow the Yee stagger of lib/field/vfield.cc:58-87
+ *     (a staggered axis has n-1 entries).
+ *   - Host-side field buffers are FP64, C order (i, j, k) with k fastest,
+ *     exactly the layout lib/io/fileio.cc:143-157 writes.
+ *   - Planar (2D) runs use the reference's x-z plane with (Ey, Hx, Hz),
+ *     ny == 1 (lib/io/reader.cc:151-154).
+ */
+#ifndef LUXUM_H
+#define LUXUM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum lx_status {
+    LX_OK = 0,
+    LX_EINVAL = 1,    /* invalid argument or parameter set              */
+    LX_ENOMEM = 2,    /* device or host allocation failed                */
+    LX_EDEVICE = 3,   /* HIP runtime error                               */
+    LX_EIO = 4,       /* file could not be read or written               */
+    LX_EBUSY = 5,     /* admission refused: not enough device memory     */
+    LX_EUNSTABLE = 6  /* non-finite field value detected by the monitor  */
+} lx_status;
+
+/* Field component ids (order of vfield Vx,Vy,Vz for E then H). */
+enum { LX_EX = 0, LX_EY = 1, LX_EZ = 2, LX_HX = 3, LX_HY = 4, LX_HZ = 5 };
+
+/* Storage precision of E and H on the device (arithmetic is FP32). */
+enum { LX_FP32 = 0, LX_FP16 = 1 };
+
+/* Incident-field provider for TF/SF injection. */
+enum {
+    LX_INC_ANALYTIC = 0, /* reference formula exp(-xi^2/sigma), src/solvers/maxwell.cc:194-812 */
+    LX_INC_AUX1D = 1     /* FP64 1D auxiliary Yee line along k (axis-aligned k only)          */
+};
+
+/* Waveform of the aux-1D line source and of the soft point source. */
+enum {
+    LX_WAVE_GAUSSIAN = 0,      /* exp(-((t-t0)/tau)^2)                        */
+    LX_WAVE_GAUSSIAN_SINE = 1  /* exp(-((t-t0)/tau)^2) * sin(2 pi f0 (t-t0))   */
+};
+
+/* Point-source kind. */
+enum {
+    LX_SRC_HARD_CW = 0,  /* reference: E(src) = int(amp)*sin(int(omega)*t*dt)  (maxwell_three_dim_pml.cc:161-165) */
+    LX_SRC_SOFT = 1      /* E(src) += amp * waveform(t*dt)                                                     */
+};
+
+/*
+ * Simulation parameters.  Fields of documents 1-5 mirror class reader
+ * (lib/io/reader.h:52-167) one to one; the "extension" block is the optional
+ * sixth document of parameters.yaml.  lx_params_default() fills a vacuum,
+ * PEC-walled 3D grid with every extension at its reference-compatible value.
+ */
+typedef struct lx_params {
+    /* doc 1 */
+    int32_t is_planar;
+    /* doc 2 */
+    int32_t nx, ny, nz;
+    double dx, dy, dz;          /* metres */
+    double courant;             /* S: dt = S*dx/c (maxwell.cc:67) */
+    int32_t num_timesteps;
+    /* doc 3 (kept for format parity; one GPU holds the whole domain) */
+    int32_t npx, npy, n_threads;
+    /* doc 4: sources */
+    int32_t use_ps;
+    int32_t src_pos[3];
+    double src_amp, src_omega;
+    int32_t use_pw;
+    double k_pw[3];
+    double e_pw[3];
+    double off_pw;
+    int32_t tf_lo[3];           /* x1_pw, y1_pw, z1_pw */
+    int32_t tf_hi[3];           /* x2_pw, y2_pw, z2_pw */
+    /* doc 5: CPML */
+    int32_t use_pml;
+    double pml_thickness;       /* metres */
+    double sigma_ratio;         /* sigma_max / sigma_optimal */
+    double sigma_m;
+    double kappa_max;
+    double a_max;
+    double a_m;
+    /* doc 6: extensions (absent from the reference) */
+    int32_t precision;          /* LX_FP32 | LX_FP16 */
+    int32_t incident;           /* LX_INC_ANALYTIC | LX_INC_AUX1D */
+    double pw_sigma;            /* analytic Gaussian width, m^2 (reference: 2e-4, maxwell.cc:71) */
+    int32_t pw_wave;            /* aux-1D line waveform */
+    double pw_lambda0;          /* carrier wavelength, metres */
+    double pw_fwhm;             /* envelope FWHM in carrier periods */
+    double pw_delay;            /* envelope centre, in envelope widths tau */
+    int32_t src_kind;           /* LX_SRC_HARD_CW | LX_SRC_SOFT */
+    int32_t src_comp;           /* component of the point source (default Ez in 3D, Ey in 2D) */
+    int32_t src_wave;
+    double src_lambda0, src_fwhm, src_delay;
+    int32_t snapshot_every;     /* 0: none */
+    int32_t monitor_every;      /* 0: none; else FP64 max|E|^2 + NaN guard every n steps */
+} lx_params;
+
+typedef struct lx_ctx lx_ctx;
+typedef struct lx_sim lx_sim;
+typedef struct lx_batch lx_batch;
+
+/* Derived quantities of a created simulation (read-only report). */
+typedef struct lx_sim_info {
+    double dt, c0, eps0, mu0, eta0;
+    int64_t cells;              /* nx*ny*nz */
+    int64_t slab_cells[3];      /* cells inside the CPML slabs of each axis (both sides) */
+    int64_t device_bytes;       /* device memory held by the simulation */
+    int64_t bytes_per_step;     /* algorithmic HBM bytes per full timestep */
+    int32_t step;               /* timesteps taken so far */
+    int32_t slab_lo[3], slab_hi[3];  /* CPML slab: [0,lo) and [hi,n) per axis */
+} lx_sim_info;
+
+/* ---- parameters ------------------------------------------------------- */
+
+/* Fill p with defaults (no reference counterpart: reader has no defaults). */
+lx_status lx_params_default(lx_params *p);
+
+/* Replaces reader::reader / readYAML / checkData (lib/io/reader.cc:68-162):
+ * parse the five-document parameters.yaml at `path` (+ optional doc 6). */
+lx_status lx_params_load_yaml(const char *path, lx_params *p);
+
+/* Replaces reader::checkData (lib/io/reader.cc:141-162) plus the solver's own
+ * preconditions (TF box strictly inside the grid, Courant limit). */
+lx_status lx_params_check(const lx_params *p);
+
+/* ---- context ------------------------------------------------------------ */
+
+/* No reference counterpart (MPI_Init + rank setup, main.cc:55, parallel.cc:57-77,
+ * are replaced by one device). */
+lx_status lx_ctx_create(int32_t device, lx_ctx **out);
+void lx_ctx_destroy(lx_ctx *ctx);
+/* Device memory in bytes: total and currently free. */
+lx_status lx_ctx_mem_info(lx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
+
+/* ---- single simulation ------------------------------------------------- */
+
+/* Replaces maxwell::maxwell (src/solvers/maxwell.cc:60-148) and the field
+ * allocation at the top of each solve* (e.g. maxwell_three_dim_pml.cc:62-132). */
+lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out);
+void lx_sim_destroy(lx_sim *sim);
+
+/* Replaces the Cb set-up of maxwell_three_dim_pml.cc:109-132: ids[n] per node
+ * (nx*ny*nz, C order, k fastest); eps_r[n_ids] relative permittivity per id.
+ * Every E component at storage index (i,j,k) takes eps_r[ids[i,j,k]]. */
+lx_status lx_sim_set_materials(lx_sim *sim, const uint8_t *ids, const double *eps_r,
+                               int32_t n_ids);
+
+/* Probe points (no reference counterpart; the reference only prints max|E|^2):
+ * comp_ijk holds n quadruples (component, i, j, k).  Values are recorded after
+ * every timestep, FP64, SI units. */
+lx_status lx_sim_set_probes(lx_sim *sim, const int32_t *comp_ijk, int32_t n);
+
+/* Replaces one or more iterations of the solve* time loop
+ * (maxwell_three_dim_pml.cc:158-283, maxwell_three_dim.cc:85-195,
+ *  maxwell_two_dim_pml.cc:126-188, maxwell_two_dim.cc:85-144).
+ * Asynchronous on the simulation's compute stream. */
+lx_status lx_sim_step(lx_sim *sim, int32_t nsteps);
+
+/* Block until all work of the simulation is done. */
+lx_status lx_sim_sync(lx_sim *sim);
+
+/* Copy component `comp` (LX_EX..LX_HZ) to host, reference layout and SI units.
+ * n must equal the component's element count.  Synchronous. */
+lx_status lx_sim_read_field(lx_sim *sim, int32_t comp, double *dst, size_t n);
+
+/* Replaces fileio::writeHDF5's data path (lib/io/fileio.cc:184-212): enqueue
+ * a snapshot of `comp` on the side stream into dst (pinned or pageable host
+ * memory, reference layout, FP64).  The copy reflects the state after the
+ * steps enqueued so far; it is complete after lx_sim_sync. */
+lx_status lx_sim_snapshot_async(lx_sim *sim, int32_t comp, double *dst, size_t n);
+
+/* Copy recorded probe values: dst[t*n_probes + p] for t < *n_steps. */
+lx_status lx_sim_read_probes(lx_sim *sim, double *dst, size_t n, int32_t *n_steps);
+
+/* FP64 max over cells of Ex^2+Ey^2+Ez^2 at equal indices (the reference
+ * monitor, maxwell_three_dim_pml.cc:266-276; 2D: max Ey^2,
+ * maxwell_two_dim_pml.cc:181) and total discrete field energy. Synchronous. */
+lx_status lx_sim_monitor(lx_sim *sim, double *max_e2, double *energy);
+
+lx_status lx_sim_get_info(lx_sim *sim, lx_sim_info *info);
+
+/* The simulation's compute stream (hipStream_t) for event timing. */
+void *lx_sim_stream(lx_sim *sim);
+
+/* Average device time per launch, in ms, of the H-update and E-update
+ * kernels over `nsteps` extra timesteps, timed with events on the compute
+ * stream around every launch (the steps are real steps: state advances). */
+lx_status lx_sim_time_kernels(lx_sim *sim, int32_t nsteps, double *ms_h, double *ms_e,
+                              double *ms_step);
+
+/* ---- batches of independent simulations ------------------------------- */
+
+/* No reference counterpart: advance many simulations of uneven sizes in one
+ * persistent kernel per half-step over a flattened tile list. */
+lx_status lx_batch_create(lx_ctx *ctx, lx_batch **out);
+void lx_batch_destroy(lx_batch *b);
+lx_status lx_batch_add(lx_batch *b, lx_sim *sim);
+lx_status lx_batch_remove(lx_batch *b, lx_sim *sim);
+lx_status lx_batch_step(lx_batch *b, int32_t nsteps);
+lx_status lx_batch_sync(lx_batch *b);
+
+/* ---- errors ------------------------------------------------------------- */
+const char *lx_last_error(void);
+const char *lx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LUXUM_H */

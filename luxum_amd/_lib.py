@@ -1,0 +1,111 @@
+"""ctypes binding of libluxum.so (the C ABI in include/luxum.h).
+
+The library is built in-tree by ``make -C luxum_amd`` (or
+``__graft_entry__.build()``).  There is no fallback: if the shared library is
+missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from .params import LxParams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libluxum.so")
+
+LX_OK, LX_EINVAL, LX_ENOMEM, LX_EDEVICE, LX_EIO, LX_EBUSY, LX_EUNSTABLE = range(7)
+STATUS_NAMES = {0: "LX_OK", 1: "LX_EINVAL", 2: "LX_ENOMEM", 3: "LX_EDEVICE", 4: "LX_EIO", 5: "LX_EBUSY",
+                6: "LX_EUNSTABLE"}
+
+# Every symbol the header declares (checked by tests/test_abi.py).
+EXPORTS = (
+    "lx_params_default", "lx_params_load_yaml", "lx_params_check",
+    "lx_ctx_create", "lx_ctx_destroy", "lx_ctx_mem_info",
+    "lx_sim_create", "lx_sim_destroy", "lx_sim_set_materials", "lx_sim_set_probes",
+    "lx_sim_step", "lx_sim_sync", "lx_sim_read_field", "lx_sim_snapshot_async",
+    "lx_sim_read_probes", "lx_sim_monitor", "lx_sim_get_info", "lx_sim_stream",
+    "lx_sim_time_kernels",
+    "lx_batch_create", "lx_batch_destroy", "lx_batch_add", "lx_batch_remove",
+    "lx_batch_step", "lx_batch_sync",
+    "lx_last_error", "lx_version",
+)
+
+
+class LuxumError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class SimInfo(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("c0", C.c_double), ("eps0", C.c_double), ("mu0", C.c_double),
+        ("eta0", C.c_double),
+        ("cells", C.c_int64),
+        ("slab_cells", C.c_int64 * 3),
+        ("device_bytes", C.c_int64),
+        ("bytes_per_step", C.c_int64),
+        ("step", C.c_int32),
+        ("slab_lo", C.c_int32 * 3),
+        ("slab_hi", C.c_int32 * 3),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libluxum.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C luxum_amd` "
+                           "or __graft_entry__.build(); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    P = C.POINTER(LxParams)
+    st = C.c_int
+    sig = {
+        "lx_params_default": ([P], st),
+        "lx_params_load_yaml": ([C.c_char_p, P], st),
+        "lx_params_check": ([P], st),
+        "lx_ctx_create": ([C.c_int32, C.POINTER(vp)], st),
+        "lx_ctx_destroy": ([vp], None),
+        "lx_ctx_mem_info": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], st),
+        "lx_sim_create": ([vp, P, C.POINTER(vp)], st),
+        "lx_sim_destroy": ([vp], None),
+        "lx_sim_set_materials": ([vp, vp, vp, C.c_int32], st),
+        "lx_sim_set_probes": ([vp, vp, C.c_int32], st),
+        "lx_sim_step": ([vp, C.c_int32], st),
+        "lx_sim_sync": ([vp], st),
+        "lx_sim_read_field": ([vp, C.c_int32, vp, C.c_size_t], st),
+        "lx_sim_snapshot_async": ([vp, C.c_int32, vp, C.c_size_t], st),
+        "lx_sim_read_probes": ([vp, vp, C.c_size_t, C.POINTER(C.c_int32)], st),
+        "lx_sim_monitor": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], st),
+        "lx_sim_get_info": ([vp, C.POINTER(SimInfo)], st),
+        "lx_sim_stream": ([vp], vp),
+        "lx_sim_time_kernels": ([vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)], st),
+        "lx_batch_create": ([vp, C.POINTER(vp)], st),
+        "lx_batch_destroy": ([vp], None),
+        "lx_batch_add": ([vp, vp], st),
+        "lx_batch_remove": ([vp, vp], st),
+        "lx_batch_step": ([vp, C.c_int32], st),
+        "lx_batch_sync": ([vp], st),
+        "lx_last_error": ([], C.c_char_p),
+        "lx_version": ([], C.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != LX_OK:
+        msg = lib().lx_last_error()
+        raise LuxumError(status, msg.decode() if msg else "")

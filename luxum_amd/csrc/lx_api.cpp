@@ -1,0 +1,912 @@
+// lx_api.cpp -- C ABI of the MI355X Yee timestep (include/luxum.h).
+//
+// Device units.  The reference updates (maxwell_three_dim_pml.cc:182-255)
+//     H -= Db*(c1/k1 - c2/k2) + Db*(psi1 - psi2),   Db = dt/mu0
+//     E += Cb*(c1/k1 - c2/k2) + Cb*(psi1 - psi2),   Cb = dt/(eps0*eps_r)
+// with c1 = dF/d_a (split curl) are stored here with Hs = eta0*H and the
+// curl written as a plain difference dF (no 1/d_a):
+//     Hs -= (dE1*kh_a1 - dE2*kh_a2) + (psiH1 - psiH2)
+//     E  += cb[id]*((dHs1*ke_a1 - dHs2*ke_a2) + (psiE1 - psiE2))
+// where ch = c*dt/dx (= eta0*Db/dx), kh_a = ch*(dx/d_a)/kappa_h,
+// ke_a = (dx/d_a)/kappa_e, cb = c*dt/(dx*eps_r) (= Cb/(eta0*dx)), and the psi
+// recursions psi = b*psi + c'*dF use c'_h = ch*c_h*(dx/d_a), c'_e = c_e*(dx/d_a).
+// All of these are exact rewrites of the reference's algebra; they are
+// evaluated in FP64 on the host and rounded once to FP32, so every device
+// quantity is O(1) (FP16 storage needs that).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/luxum.h"
+#include "lx_internal.hpp"
+#include "lx_util.hpp"
+
+namespace lx {
+char *error_buffer() {
+    static thread_local char buf[512];
+    return buf;
+}
+}  // namespace lx
+
+using namespace lx;
+
+#define HIP_TRY(x)                                                                                         \
+    do {                                                                                                   \
+        hipError_t _e = (x);                                                                               \
+        if (_e != hipSuccess) return fail(LX_EDEVICE, "%s failed: %s", #x, hipGetErrorString(_e));          \
+    } while (0)
+
+struct lx_ctx {
+    int device;
+    int64_t total_mem;
+};
+
+struct lx_sim {
+    lx_ctx *ctx = nullptr;
+    lx_params p{};
+    int planar = 0, fp16 = 0, pml = 0;
+    int N[3] = {0, 0, 0};
+    int px = 0;
+    long long plane = 0;
+    size_t esize = 4;
+    // constants (maxwell.cc:63-84)
+    double eps0 = 0, mu0 = 0, c0 = 0, dt = 0, eta0 = 0, sigma_max = 0;
+    hipStream_t cs = nullptr, ss = nullptr;
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    void *E[3] = {nullptr, nullptr, nullptr}, *H[3] = {nullptr, nullptr, nullptr};
+    uint8_t *mat = nullptr;
+    float *cb = nullptr;
+    float *tab = nullptr;
+    float *psi = nullptr;
+    int *counter = nullptr;
+    Tables tables{};
+    Slabs slabs{};
+    int64_t slab_cells[3] = {0, 0, 0};
+    TfSet tfH{}, tfE{};
+    IncParams inc{};
+    AuxLine aux{};
+    int use_aux = 0;
+    PointSrc ps{};
+    // probes
+    int np = 0, pcap = 0;
+    long long *pidx = nullptr;
+    int *pcomp = nullptr;
+    double *prec = nullptr;
+    std::vector<int32_t> probe_host;
+    // snapshots
+    double *stage[2] = {nullptr, nullptr};
+    size_t stage_n[2] = {0, 0};
+    hipEvent_t ready[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
+    int snap_next = 0;
+    double *rd_stage = nullptr;
+    size_t rd_n = 0;
+    double *mon = nullptr;
+    // graphs
+    hipGraphExec_t gexec = nullptr;
+    int gK = 0;
+    int graphs = 1;
+    int steps = 0;
+    float cb_host[256];
+};
+
+namespace {
+
+template <typename T>
+Grid<T> make_grid(const lx_sim *s) {
+    Grid<T> g{};
+    g.nx = s->N[0];
+    g.ny = s->N[1];
+    g.nz = s->N[2];
+    g.px = s->px;
+    g.plane = s->plane;
+    g.kz = 16;
+    for (int c = 0; c < 3; ++c) {
+        g.f.E[c] = (T *)s->E[c];
+        g.f.H[c] = (T *)s->H[c];
+    }
+    g.mat = s->mat;
+    g.cb = s->cb;
+    g.tab = s->tables;
+    g.slab = s->slabs;
+    g.pml = s->pml;
+    g.counter = s->counter;
+    return g;
+}
+
+lx_status dalloc(lx_sim *s, void **out, size_t bytes) {
+    void *p = nullptr;
+    size_t b = bytes ? (bytes + 255) / 256 * 256 : 256;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(LX_ENOMEM, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+    }
+    s->allocs.push_back(p);
+    s->bytes += (int64_t)b;
+    *out = p;
+    return LX_OK;
+}
+
+// ---- CPML profiles (maxwell.cc:997-1280), FP64 ---------------------------
+// Same expressions as the reference; the piecewise form equals the
+// reference's masked sums wherever those are finite.
+struct Prof {
+    std::vector<double> sig, kap, alp, b, c;
+};
+
+void build_profile(const lx_sim *s, int dim, int is_E, int n, Prof &pr) {
+    const lx_params &p = s->p;
+    const double delta = dim == 0 ? p.dx : dim == 1 ? p.dy : p.dz;
+    const int N = s->N[dim];
+    const double d = p.pml_thickness / delta, m = p.sigma_m, ma = p.a_m;
+    pr.sig.assign(n, 0.0);
+    pr.kap.assign(n, 1.0);
+    pr.alp.assign(n, 0.0);
+    pr.b.assign(n, 1.0);
+    pr.c.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        const double x = is_E ? (double)i : i + 0.5;
+        double sg = 0.0, ka = 1.0, al = 0.0;
+        if (x <= (d - 1)) {
+            sg = std::pow(((d - 1) - x) / (d - 1), m) * s->sigma_max;
+            ka = std::pow(((d - 1) - x) / (d - 1), m) * (p.kappa_max - 1) + 1;
+            al = std::pow((x / (d - 1)), ma) * p.a_max;
+        }
+        if (x >= (N - d)) {
+            sg = std::pow((x - N + d) / (d - 1), m) * s->sigma_max;
+            ka = std::pow((x - N + d) / (d - 1), m) * (p.kappa_max - 1) + 1;
+            al = std::pow((N - 1 - x) / (d - 1), ma) * p.a_max;
+        }
+        pr.sig[i] = sg;
+        pr.kap[i] = ka;
+        pr.alp[i] = al;
+        pr.b[i] = std::exp(-((sg / ka / s->eps0) - (al / s->eps0)) * s->dt);   // maxwell.cc:1215
+        pr.c[i] = sg == 0 ? 0.0 : sg / (sg * ka + std::pow(ka, 2) * al) * (pr.b[i] - 1);   // :1271-1278
+    }
+}
+
+lx_status build_tables(lx_sim *s) {
+    const int axes = 3;
+    int Nmax = std::max(s->N[0], std::max(s->N[1], s->N[2]));
+    std::vector<float> host((size_t)axes * 6 * Nmax, 0.f);
+    const double d[3] = {s->p.dx, s->p.dy, s->p.dz};
+    const double ch = s->c0 * s->dt / s->p.dx;
+    for (int a = 0; a < 3; ++a) {
+        const int N = s->N[a];
+        Prof pe, ph;
+        const bool on = s->pml && !(s->planar && a == 1);
+        if (on) {
+            build_profile(s, a, 1, N, pe);
+            build_profile(s, a, 0, N - 1, ph);
+        } else {
+            pe.kap.assign(N, 1.0); pe.b.assign(N, 1.0); pe.c.assign(N, 0.0);
+            ph.kap.assign(std::max(N - 1, 1), 1.0); ph.b.assign(std::max(N - 1, 1), 1.0); ph.c.assign(std::max(N - 1, 1), 0.0);
+        }
+        const double r = (s->planar && a == 1) ? 1.0 : s->p.dx / d[a];   // planar runs have no y derivative
+        float *kh = &host[((size_t)a * 6 + 0) * Nmax], *bh = &host[((size_t)a * 6 + 1) * Nmax],
+              *chh = &host[((size_t)a * 6 + 2) * Nmax], *ke = &host[((size_t)a * 6 + 3) * Nmax],
+              *be = &host[((size_t)a * 6 + 4) * Nmax], *ce = &host[((size_t)a * 6 + 5) * Nmax];
+        for (int i = 0; i < N; ++i) {
+            const int ih = std::min(i, (int)ph.kap.size() - 1);
+            kh[i] = (float)(ch * r / ph.kap[ih]);
+            bh[i] = (float)ph.b[ih];
+            chh[i] = (float)(ch * ph.c[ih] * r);
+            ke[i] = (float)(r / pe.kap[i]);
+            be[i] = (float)pe.b[i];
+            ce[i] = (float)(pe.c[i] * r);
+        }
+        // slab extents: every index with c != 0 (E or H node) lies in [0,lo) U [hi,N)
+        int lo = 0, hi = N;
+        if (on) {
+            auto nz = [&](int i) { return pe.c[i] != 0.0 || (i < N - 1 && ph.c[i] != 0.0); };
+            while (lo < N && nz(lo)) ++lo;
+            while (hi > lo && nz(hi - 1)) --hi;
+            for (int i = lo; i < hi; ++i)
+                if (nz(i)) return fail(LX_EINVAL, "CPML profile along axis %d is not confined to the boundary slabs", a);
+        }
+        s->slabs.lo[a] = lo;
+        s->slabs.hi[a] = hi;
+        s->slabs.w[a] = lo + (N - hi);
+    }
+    void *p = nullptr;
+    lx_status st = dalloc(s, &p, host.size() * sizeof(float));
+    if (st) return st;
+    s->tab = (float *)p;
+    HIP_TRY(hipMemcpy(s->tab, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice));
+    for (int a = 0; a < 3; ++a) {
+        s->tables.kh[a] = s->tab + ((size_t)a * 6 + 0) * Nmax;
+        s->tables.bh[a] = s->tab + ((size_t)a * 6 + 1) * Nmax;
+        s->tables.chh[a] = s->tab + ((size_t)a * 6 + 2) * Nmax;
+        s->tables.ke[a] = s->tab + ((size_t)a * 6 + 3) * Nmax;
+        s->tables.be[a] = s->tab + ((size_t)a * 6 + 4) * Nmax;
+        s->tables.ce[a] = s->tab + ((size_t)a * 6 + 5) * Nmax;
+    }
+    // psi slabs: per axis 2 H + 2 E arrays of w_a * (product of the other extents)
+    size_t total = 0;
+    for (int a = 0; a < 3; ++a) {
+        long long other = 1;
+        for (int b = 0; b < 3; ++b)
+            if (b != a) other *= s->N[b];
+        s->slab_cells[a] = (int64_t)s->slabs.w[a] * other;
+        total += 4 * (size_t)s->slab_cells[a];
+    }
+    st = dalloc(s, &p, std::max<size_t>(total, 1) * sizeof(float));
+    if (st) return st;
+    s->psi = (float *)p;
+    launch_zero_bytes(s->psi, std::max<size_t>(total, 1) * sizeof(float), s->cs);
+    size_t off = 0;
+    for (int a = 0; a < 3; ++a) {
+        for (int w = 0; w < 2; ++w) {
+            s->slabs.psiH[a][w] = s->psi + off; off += s->slab_cells[a];
+            s->slabs.psiE[a][w] = s->psi + off; off += s->slab_cells[a];
+        }
+    }
+    return LX_OK;
+}
+
+// ---- TF/SF boxes (maxwell.cc:512-929) -----------------------------------
+// For a face normal to axis a, the two split-curl entries it corrects:
+// {target component, split curl (1|2), differentiated component}.
+const int kHFace[3][2][3] = {
+    {{LX_HZ, 1, LX_EY}, {LX_HY, 2, LX_EZ}},
+    {{LX_HZ, 2, LX_EX}, {LX_HX, 1, LX_EZ}},
+    {{LX_HY, 1, LX_EX}, {LX_HX, 2, LX_EY}},
+};
+const int kEFace[3][2][3] = {
+    {{LX_EY, 2, LX_HZ}, {LX_EZ, 1, LX_HY}},
+    {{LX_EX, 1, LX_HZ}, {LX_EZ, 2, LX_HX}},
+    {{LX_EX, 2, LX_HY}, {LX_EY, 1, LX_HX}},
+};
+
+// plane_wave_initialise (maxwell.cc:825-929) with one rank: clip to [0, N-1].
+void pw_clip(int lo, int hi, int N, int *st, int *en) {
+    const int s0 = 0, e0 = N - 1;
+    if (lo > e0) { *st = -1; *en = -1; }
+    else if (lo >= s0) { *st = lo; *en = hi > e0 ? e0 : hi; }
+    else if (hi > e0) { *st = s0; *en = e0; }
+    else if (hi >= s0) { *st = s0; *en = hi; }
+    else { *st = -1; *en = -1; }
+}
+
+void empty_box(TfBox &b) {
+    b.lo[0] = b.lo[1] = b.lo[2] = 1;
+    b.hi[0] = b.hi[1] = b.hi[2] = 0;
+    b.off[0] = b.off[1] = b.off[2] = 0.f;
+    b.dir = 0.0;
+}
+
+void build_tfsf(lx_sim *s) {
+    for (int c = 0; c < 3; ++c)
+        for (int w = 0; w < 2; ++w)
+            for (int f = 0; f < 2; ++f) { empty_box(s->tfH.box[c][w][f]); empty_box(s->tfE.box[c][w][f]); }
+    s->tfH.any = s->tfE.any = 0;
+    if (!s->p.use_pw) return;
+    const lx_params &p = s->p;
+    const bool three = !s->planar;
+    double Edir[3], Hdir[3];
+    for (int a = 0; a < 3; ++a) Edir[a] = p.e_pw[a];
+    const double *k = p.k_pw;
+    double hx = k[1] * Edir[2] - k[2] * Edir[1], hy = k[2] * Edir[0] - k[0] * Edir[2], hz = k[0] * Edir[1] - k[1] * Edir[0];
+    double ha = std::pow(std::pow(hx, 2) + std::pow(hy, 2) + std::pow(hz, 2), 0.5);   // maxwell.cc:95-102
+    Hdir[0] = hx / ha; Hdir[1] = hy / ha; Hdir[2] = hz / ha;
+    // E-update corrections in device units: (g*Hdir/(d_a*c*mu0)) * d_a * eta0
+    const double hfac = s->eta0 / (s->c0 * s->mu0);
+    int st[3], en[3];
+    for (int a = 0; a < 3; ++a) {
+        if (!three && a == 1) { st[a] = en[a] = -1; continue; }
+        pw_clip(p.tf_lo[a], p.tf_hi[a], s->N[a], &st[a], &en[a]);
+    }
+    for (int half = 0; half < 2; ++half) {   // 0: H update (curl E), 1: E update (curl H)
+        TfSet &set = half == 0 ? s->tfH : s->tfE;
+        for (int a = 0; a < 3; ++a) {
+            if (!three && a == 1) continue;
+            const int b0 = (a + 1) % 3, b1 = (a + 2) % 3;
+            bool trans_ok = true;
+            for (int b : {b0, b1})
+                if ((three || b != 1) && st[b] == -1) trans_ok = false;
+            if (!trans_ok) continue;
+            for (int side = 0; side < 2; ++side) {
+                int idx;
+                double coord;
+                if (half == 0) {
+                    idx = side == 0 ? p.tf_lo[a] - 1 : p.tf_hi[a];
+                    coord = side == 0 ? (double)p.tf_lo[a] : (double)p.tf_hi[a];
+                } else {
+                    idx = side == 0 ? p.tf_lo[a] : p.tf_hi[a];
+                    coord = side == 0 ? p.tf_lo[a] - 0.5 : p.tf_hi[a] + 0.5;
+                }
+                if (idx < 0 || idx > s->N[a] - 1) continue;
+                for (int e = 0; e < 2; ++e) {
+                    const int *fd = half == 0 ? kHFace[a][e] : kEFace[a][e];
+                    const int tgt = fd[0], which = fd[1], src = fd[2];
+                    if (!three && !(tgt == LX_EY || tgt == LX_HX || tgt == LX_HZ)) continue;
+                    if (!three && !(src == LX_EY || src == LX_HX || src == LX_HZ)) continue;
+                    double sgn = side == 0 ? -1.0 : 1.0;   // low face subtracts, high face adds
+                    double dir;
+                    if (half == 0) {
+                        if (!three) sgn = -sgn;            // 2D injects the negated wave (maxwell.cc:523)
+                        dir = sgn * Edir[src];
+                    } else {
+                        dir = sgn * Hdir[src - 3] * hfac;
+                        if (!three) dir = -dir;            // explicit (-1) of maxwell.cc:563-588
+                    }
+                    if (dir == 0.0) continue;
+                    TfBox &bx = set.box[tgt % 3][which - 1][side];
+                    bx.lo[a] = bx.hi[a] = idx;
+                    bx.off[a] = (float)(coord - idx);
+                    for (int b : {b0, b1}) {
+                        if (!three && b == 1) { bx.lo[b] = bx.hi[b] = 0; bx.off[b] = 0.f; continue; }
+                        const int sg = kStag[src][b];
+                        bx.lo[b] = st[b];
+                        bx.hi[b] = en[b] - sg;
+                        bx.off[b] = sg ? 0.5f : 0.f;
+                    }
+                    bx.dir = dir;
+                    set.any = 1;
+                }
+            }
+        }
+    }
+}
+
+double waveform_host(int kind, double t, double c, double lambda0, double fwhm, double delay) {
+    const double f0 = c / lambda0, T0 = 1.0 / f0;
+    const double tau = fwhm * T0 / (2.0 * std::sqrt(std::log(2.0)));
+    const double t0 = delay * tau;
+    const double x = (t - t0) / tau;
+    const double env = std::exp(-x * x);
+    if (kind == LX_WAVE_GAUSSIAN) return env;
+    return env * std::sin(2.0 * M_PI * f0 * (t - t0));
+}
+
+lx_status build_incident(lx_sim *s) {
+    const lx_params &p = s->p;
+    IncParams &in = s->inc;
+    std::memset(&in, 0, sizeof(in));
+    in.mode = p.incident;
+    in.planar = s->planar;
+    in.dx = p.dx; in.dy = p.dy; in.dz = p.dz;
+    in.kx = p.k_pw[0]; in.ky = p.k_pw[1]; in.kz = p.k_pw[2];
+    in.init_dx = p.off_pw * p.dx;
+    in.cdt = s->c0 * s->dt;
+    in.inv_sigma = 1.0 / p.pw_sigma;
+    if (!(p.use_pw && p.incident == LX_INC_AUX1D)) return LX_OK;
+    int ax = 0;
+    for (int a = 0; a < 3; ++a)
+        if (p.k_pw[a] != 0.0) ax = a;
+    const double d[3] = {p.dx, p.dy, p.dz};
+    AuxLine &A = s->aux;
+    A.S = s->c0 * s->dt / d[ax];
+    const int x0 = p.tf_lo[ax] - 2;
+    const int T = p.num_timesteps > 0 ? p.num_timesteps : 1;
+    A.L = (p.tf_hi[ax] + 2 - x0) + (int)std::ceil(A.S * T / 2.0) + 8;
+    A.wave = p.pw_wave;
+    A.c = s->c0;
+    A.dt = s->dt;
+    A.lambda0 = p.pw_lambda0;
+    A.fwhm = p.pw_fwhm;
+    A.delay = p.pw_delay;
+    void *q = nullptr;
+    lx_status st = dalloc(s, &q, sizeof(double) * 2 * (size_t)A.L);
+    if (st) return st;
+    A.e = (double *)q;
+    A.h = A.e + A.L;
+    std::vector<double> init(2 * (size_t)A.L, 0.0);
+    init[0] = waveform_host(A.wave, 0.0, A.c, A.lambda0, A.fwhm, A.delay);
+    HIP_TRY(hipMemcpy(A.e, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice));
+    in.axis = ax;
+    in.x0 = x0;
+    in.L = A.L;
+    in.e = A.e;
+    in.h = A.h;
+    s->use_aux = 1;
+    return LX_OK;
+}
+
+void fill_cb_vacuum(lx_sim *s) {
+    const double v = s->c0 * s->dt / s->p.dx;
+    for (int q = 0; q < 256; ++q) s->cb_host[q] = (float)v;
+}
+
+template <typename T>
+void enqueue_one_t(lx_sim *s) {
+    Grid<T> g = make_grid<T>(s);
+    if (s->p.use_ps) launch_point_source(g, s->ps, s->cs);
+    if (s->planar) launch_h2(g, s->tfH, s->inc, s->cs);
+    else launch_h3(g, s->tfH, s->inc, s->cs);
+    if (s->use_aux) launch_aux_h(s->aux, s->cs);
+    if (s->planar) launch_e2(g, s->tfE, s->inc, s->cs);
+    else launch_e3(g, s->tfE, s->inc, s->cs);
+    if (s->use_aux) launch_aux_e(s->aux, s->counter, s->cs);
+    if (s->np) {
+        ProbeSet ps{s->np, s->pidx, s->pcomp, s->prec, s->pcap, 1.0 / s->eta0};
+        launch_probes(g, ps, s->cs);
+    }
+}
+
+void enqueue_steps(lx_sim *s, int n) {
+    for (int q = 0; q < n; ++q) {
+        if (s->fp16) enqueue_one_t<__half>(s);
+        else enqueue_one_t<float>(s);
+    }
+}
+
+void drop_graph(lx_sim *s) {
+    if (s->gexec) {
+        (void)hipGraphExecDestroy(s->gexec);
+        s->gexec = nullptr;
+    }
+}
+
+lx_status ensure_graph(lx_sim *s, int K) {
+    if (s->gexec && s->gK == K) return LX_OK;
+    drop_graph(s);
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s->cs, hipStreamCaptureModeThreadLocal));
+    enqueue_steps(s, K);
+    hipError_t e = hipStreamEndCapture(s->cs, &graph);
+    if (e != hipSuccess || !graph) {
+        (void)hipGetLastError();
+        s->graphs = 0;
+        return LX_OK;
+    }
+    e = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        s->gexec = nullptr;
+        s->graphs = 0;
+        return LX_OK;
+    }
+    s->gK = K;
+    return LX_OK;
+}
+
+lx_status ensure_probe_cap(lx_sim *s, int need) {
+    if (!s->np || need <= s->pcap) return LX_OK;
+    int cap = std::max(need, std::max(2 * s->pcap, s->p.num_timesteps));
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, sizeof(double) * (size_t)cap * s->np);
+    if (e != hipSuccess) return fail(LX_ENOMEM, "probe buffer allocation failed");
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    if (s->prec) {
+        HIP_TRY(hipMemcpy(q, s->prec, sizeof(double) * (size_t)s->pcap * s->np, hipMemcpyDeviceToDevice));
+        (void)hipFree(s->prec);
+        s->bytes -= (int64_t)sizeof(double) * s->pcap * s->np;
+    }
+    s->prec = (double *)q;
+    s->pcap = cap;
+    s->bytes += (int64_t)sizeof(double) * cap * s->np;
+    drop_graph(s);
+    return LX_OK;
+}
+
+lx_status check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(LX_EDEVICE, "kernel launch failed: %s", hipGetErrorString(e));
+    return LX_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *lx_last_error(void) { return lx::error_buffer(); }
+const char *lx_version(void) { return "luxum-mi355x 0.1 (gfx950)"; }
+
+lx_status lx_ctx_create(int32_t device, lx_ctx **out) {
+    if (!out) return fail(LX_EINVAL, "lx_ctx_create: null pointer");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(LX_EINVAL, "device %d out of range (%d devices)", device, n);
+    HIP_TRY(hipSetDevice(device));
+    lx_ctx *c = new (std::nothrow) lx_ctx;
+    if (!c) return fail(LX_ENOMEM, "out of host memory");
+    c->device = device;
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    c->total_mem = (int64_t)tot;
+    *out = c;
+    return LX_OK;
+}
+
+void lx_ctx_destroy(lx_ctx *ctx) { delete ctx; }
+
+lx_status lx_ctx_mem_info(lx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes) {
+    if (!ctx) return fail(LX_EINVAL, "null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    if (free_bytes) *free_bytes = (int64_t)fr;
+    if (total_bytes) *total_bytes = (int64_t)tot;
+    return LX_OK;
+}
+
+void lx_sim_destroy(lx_sim *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    if (s->cs) (void)hipStreamSynchronize(s->cs);
+    if (s->ss) (void)hipStreamSynchronize(s->ss);
+    drop_graph(s);
+    for (void *p : s->allocs) (void)hipFree(p);
+    if (s->prec) (void)hipFree(s->prec);
+    for (int q = 0; q < 2; ++q) {
+        if (s->stage[q]) (void)hipFree(s->stage[q]);
+        if (s->ready[q]) (void)hipEventDestroy(s->ready[q]);
+        if (s->done[q]) (void)hipEventDestroy(s->done[q]);
+    }
+    if (s->rd_stage) (void)hipFree(s->rd_stage);
+    if (s->cs) (void)hipStreamDestroy(s->cs);
+    if (s->ss) (void)hipStreamDestroy(s->ss);
+    delete s;
+}
+
+lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
+    if (!ctx || !p || !out) return fail(LX_EINVAL, "lx_sim_create: null pointer");
+    lx_status st = lx_params_check(p);
+    if (st) return st;
+    HIP_TRY(hipSetDevice(ctx->device));
+    lx_sim *s = new (std::nothrow) lx_sim;
+    if (!s) return fail(LX_ENOMEM, "out of host memory");
+    s->ctx = ctx;
+    s->p = *p;
+    s->planar = p->is_planar;
+    s->fp16 = p->precision == LX_FP16;
+    s->pml = p->use_pml;
+    s->N[0] = p->nx; s->N[1] = p->is_planar ? 1 : p->ny; s->N[2] = p->nz;
+    s->px = (s->N[0] + 63) / 64 * 64;
+    s->plane = (long long)s->px * s->N[1];
+    s->esize = s->fp16 ? 2 : 4;
+    const char *ng = std::getenv("LX_NO_GRAPH");
+    s->graphs = !(ng && *ng && *ng != '0');
+    // constants exactly as maxwell.cc:63-84
+    s->eps0 = 8.85 * std::pow(10, -12);
+    s->mu0 = M_PI * 4 * std::pow(10, -7);
+    s->c0 = 1 / std::pow(s->eps0 * s->mu0, 0.5);
+    s->dt = p->courant * p->dx / s->c0;
+    s->eta0 = std::sqrt(s->mu0 / s->eps0);
+    if (p->use_pml) {
+        double sopt = 0.8 * (p->sigma_m + 1) / p->dx / std::pow(s->mu0 / s->eps0, 0.5);
+        s->sigma_max = sopt * p->sigma_ratio;
+    }
+    auto bail = [&](lx_status e) { lx_sim_destroy(s); return e; };
+    hipError_t he = hipStreamCreateWithFlags(&s->cs, hipStreamNonBlocking);
+    if (he != hipSuccess) return bail(fail(LX_EDEVICE, "stream: %s", hipGetErrorString(he)));
+    he = hipStreamCreateWithFlags(&s->ss, hipStreamNonBlocking);
+    if (he != hipSuccess) return bail(fail(LX_EDEVICE, "stream: %s", hipGetErrorString(he)));
+    for (int q = 0; q < 2; ++q) {
+        if (hipEventCreateWithFlags(&s->ready[q], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->done[q], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(LX_EDEVICE, "event creation failed"));
+    }
+    const size_t ncell = (size_t)s->plane * s->N[2];
+    const size_t fbytes = ncell * s->esize;
+    for (int c = 0; c < 3; ++c) {
+        int ext[3];
+        if (comp_extent(s->N, s->planar, c, ext)) {
+            if ((st = dalloc(s, &s->E[c], fbytes))) return bail(st);
+            launch_zero_bytes(s->E[c], (fbytes + 3) / 4 * 4, s->cs);
+        }
+        if (comp_extent(s->N, s->planar, c + 3, ext)) {
+            if ((st = dalloc(s, &s->H[c], fbytes))) return bail(st);
+            launch_zero_bytes(s->H[c], (fbytes + 3) / 4 * 4, s->cs);
+        }
+    }
+    void *q = nullptr;
+    if ((st = dalloc(s, &q, ncell))) return bail(st);
+    s->mat = (uint8_t *)q;
+    launch_fill_u8(s->mat, 0, ncell, s->cs);
+    if ((st = dalloc(s, &q, 256 * sizeof(float)))) return bail(st);
+    s->cb = (float *)q;
+    fill_cb_vacuum(s);
+    he = hipMemcpy(s->cb, s->cb_host, sizeof(s->cb_host), hipMemcpyHostToDevice);
+    if (he != hipSuccess) return bail(fail(LX_EDEVICE, "copy: %s", hipGetErrorString(he)));
+    if ((st = dalloc(s, &q, 64))) return bail(st);
+    s->counter = (int *)q;
+    launch_zero_bytes(s->counter, 64, s->cs);
+    if ((st = dalloc(s, &q, 64))) return bail(st);
+    s->mon = (double *)q;
+    if ((st = build_tables(s))) return bail(st);
+    build_tfsf(s);
+    if ((st = build_incident(s))) return bail(st);
+    if (p->use_ps) {
+        PointSrc &ps = s->ps;
+        ps.kind = p->src_kind;
+        ps.comp = p->src_comp;
+        ps.wave = p->src_wave;
+        const int j = s->planar ? 0 : p->src_pos[1];
+        ps.idx = (long long)p->src_pos[2] * s->plane + (long long)j * s->px + p->src_pos[0];
+        if (p->src_kind == LX_SRC_HARD_CW) {
+            ps.amp = (double)(int)p->src_amp;   // int amp_source (maxwell.h:203)
+            ps.omega = (int)p->src_omega;       // int omega_source (maxwell.h:206)
+        } else {
+            ps.amp = p->src_amp;
+            ps.omega = 0;
+        }
+        ps.dt = s->dt;
+        ps.c = s->c0;
+        ps.lambda0 = p->src_lambda0;
+        ps.fwhm = p->src_fwhm;
+        ps.delay = p->src_delay;
+    }
+    if ((st = check_launch())) return bail(st);
+    he = hipStreamSynchronize(s->cs);
+    if (he != hipSuccess) return bail(fail(LX_EDEVICE, "set-up failed: %s", hipGetErrorString(he)));
+    *out = s;
+    return LX_OK;
+}
+
+lx_status lx_sim_set_materials(lx_sim *s, const uint8_t *ids, const double *eps_r, int32_t n_ids) {
+    if (!s || !ids || !eps_r) return fail(LX_EINVAL, "lx_sim_set_materials: null pointer");
+    if (n_ids < 1 || n_ids > 256) return fail(LX_EINVAL, "n_ids must be in [1, 256]");
+    for (int q = 0; q < n_ids; ++q)
+        if (!(eps_r[q] > 0)) return fail(LX_EINVAL, "eps_r must be positive");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    const int Nx = s->N[0], Ny = s->N[1], Nz = s->N[2];
+    // transpose (i,j,k) C order -> x-fastest device layout
+    std::vector<uint8_t> dev((size_t)s->plane * Nz, 0);
+    for (int i = 0; i < Nx; ++i)
+        for (int j = 0; j < Ny; ++j) {
+            const uint8_t *src = ids + ((size_t)i * Ny + j) * Nz;
+            for (int k = 0; k < Nz; ++k) {
+                if (src[k] >= n_ids) return fail(LX_EINVAL, "material id %d >= n_ids", (int)src[k]);
+                dev[(size_t)k * s->plane + (size_t)j * s->px + i] = src[k];
+            }
+        }
+    for (int q = 0; q < 256; ++q) {
+        const double er = q < n_ids ? eps_r[q] : 1.0;
+        const double Cb = s->dt / (s->eps0 * er);             // maxwell_three_dim_pml.cc:122-128
+        s->cb_host[q] = (float)(Cb / (s->eta0 * s->p.dx));
+    }
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    HIP_TRY(hipMemcpy(s->mat, dev.data(), dev.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->cb, s->cb_host, sizeof(s->cb_host), hipMemcpyHostToDevice));
+    return LX_OK;
+}
+
+lx_status lx_sim_set_probes(lx_sim *s, const int32_t *cijk, int32_t n) {
+    if (!s || (n > 0 && !cijk) || n < 0) return fail(LX_EINVAL, "lx_sim_set_probes: bad arguments");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    std::vector<long long> idx(n);
+    std::vector<int> comp(n);
+    for (int q = 0; q < n; ++q) {
+        const int c = cijk[4 * q];
+        int ext[3];
+        if (c < 0 || c > 5 || !comp_extent(s->N, s->planar, c, ext)) return fail(LX_EINVAL, "probe %d: bad component", q);
+        for (int a = 0; a < 3; ++a)
+            if (cijk[4 * q + 1 + a] < 0 || cijk[4 * q + 1 + a] >= ext[a]) return fail(LX_EINVAL, "probe %d outside the grid", q);
+        idx[q] = (long long)cijk[4 * q + 3] * s->plane + (long long)cijk[4 * q + 2] * s->px + cijk[4 * q + 1];
+        comp[q] = c;
+    }
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    drop_graph(s);
+    if (s->pidx) { (void)hipFree(s->pidx); s->pidx = nullptr; }
+    if (s->pcomp) { (void)hipFree(s->pcomp); s->pcomp = nullptr; }
+    if (s->prec) { (void)hipFree(s->prec); s->prec = nullptr; s->bytes -= (int64_t)sizeof(double) * s->pcap * s->np; }
+    s->np = 0;
+    s->pcap = 0;
+    s->probe_host.assign(cijk, cijk + 4 * (size_t)n);
+    if (n == 0) return LX_OK;
+    HIP_TRY(hipMalloc(&s->pidx, sizeof(long long) * n));
+    HIP_TRY(hipMalloc(&s->pcomp, sizeof(int) * n));
+    HIP_TRY(hipMemcpy(s->pidx, idx.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->pcomp, comp.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    s->np = n;
+    return ensure_probe_cap(s, std::max(s->p.num_timesteps, 1) + s->steps);
+}
+
+lx_status lx_sim_step(lx_sim *s, int32_t nsteps) {
+    if (!s || nsteps < 0) return fail(LX_EINVAL, "lx_sim_step: bad arguments");
+    if (nsteps == 0) return LX_OK;
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    lx_status st = ensure_probe_cap(s, s->steps + nsteps);
+    if (st) return st;
+    int left = nsteps;
+    const int chunk = s->p.monitor_every > 0 ? s->p.monitor_every : nsteps;
+    while (left > 0) {
+        const int n = std::min(left, chunk);
+        const int K = 32;
+        int done = 0;
+        if (s->graphs && n >= K) {
+            if ((st = ensure_graph(s, K))) return st;
+            if (s->gexec) {
+                for (; done + K <= n; done += K) HIP_TRY(hipGraphLaunch(s->gexec, s->cs));
+            }
+        }
+        enqueue_steps(s, n - done);
+        if ((st = check_launch())) return st;
+        s->steps += n;
+        left -= n;
+        if (s->p.monitor_every > 0) {
+            double mx, en;
+            if ((st = lx_sim_monitor(s, &mx, &en))) return st;
+        }
+    }
+    return LX_OK;
+}
+
+lx_status lx_sim_sync(lx_sim *s) {
+    if (!s) return fail(LX_EINVAL, "null simulation");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    HIP_TRY(hipStreamSynchronize(s->ss));
+    return LX_OK;
+}
+
+static lx_status gather(lx_sim *s, int comp, double *dst_dev, int ext[3]) {
+    const double scale = comp >= 3 ? 1.0 / s->eta0 : 1.0;
+    if (s->fp16) launch_gather(make_grid<__half>(s), comp, ext[0], ext[1], ext[2], scale, dst_dev, s->planar, s->cs);
+    else launch_gather(make_grid<float>(s), comp, ext[0], ext[1], ext[2], scale, dst_dev, s->planar, s->cs);
+    return check_launch();
+}
+
+lx_status lx_sim_read_field(lx_sim *s, int32_t comp, double *dst, size_t n) {
+    if (!s || !dst) return fail(LX_EINVAL, "lx_sim_read_field: null pointer");
+    int ext[3];
+    if (comp < 0 || comp > 5 || !comp_extent(s->N, s->planar, comp, ext)) return fail(LX_EINVAL, "bad component %d", comp);
+    const size_t m = (size_t)ext[0] * ext[1] * ext[2];
+    if (n != m) return fail(LX_EINVAL, "buffer holds %zu values, component has %zu", n, m);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    if (s->rd_n < m) {
+        if (s->rd_stage) (void)hipFree(s->rd_stage);
+        s->rd_stage = nullptr;
+        HIP_TRY(hipMalloc(&s->rd_stage, sizeof(double) * m));
+        s->rd_n = m;
+    }
+    lx_status st = gather(s, comp, s->rd_stage, ext);
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(dst, s->rd_stage, sizeof(double) * m, hipMemcpyDeviceToHost, s->cs));
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    return LX_OK;
+}
+
+lx_status lx_sim_snapshot_async(lx_sim *s, int32_t comp, double *dst, size_t n) {
+    if (!s || !dst) return fail(LX_EINVAL, "lx_sim_snapshot_async: null pointer");
+    int ext[3];
+    if (comp < 0 || comp > 5 || !comp_extent(s->N, s->planar, comp, ext)) return fail(LX_EINVAL, "bad component %d", comp);
+    const size_t m = (size_t)ext[0] * ext[1] * ext[2];
+    if (n != m) return fail(LX_EINVAL, "buffer holds %zu values, component has %zu", n, m);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    const int slot = s->snap_next;
+    s->snap_next ^= 1;
+    if (s->stage_n[slot] < m) {
+        HIP_TRY(hipEventSynchronize(s->done[slot]));
+        if (s->stage[slot]) (void)hipFree(s->stage[slot]);
+        s->stage[slot] = nullptr;
+        HIP_TRY(hipMalloc(&s->stage[slot], sizeof(double) * m));
+        s->stage_n[slot] = m;
+    }
+    // the previous copy out of this staging slot must be finished
+    HIP_TRY(hipStreamWaitEvent(s->cs, s->done[slot], 0));
+    lx_status st = gather(s, comp, s->stage[slot], ext);
+    if (st) return st;
+    HIP_TRY(hipEventRecord(s->ready[slot], s->cs));
+    HIP_TRY(hipStreamWaitEvent(s->ss, s->ready[slot], 0));
+    HIP_TRY(hipMemcpyAsync(dst, s->stage[slot], sizeof(double) * m, hipMemcpyDeviceToHost, s->ss));
+    HIP_TRY(hipEventRecord(s->done[slot], s->ss));
+    return LX_OK;
+}
+
+lx_status lx_sim_read_probes(lx_sim *s, double *dst, size_t n, int32_t *n_steps) {
+    if (!s || !n_steps) return fail(LX_EINVAL, "lx_sim_read_probes: null pointer");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    const int rec = std::min(s->steps, s->pcap);
+    const size_t m = (size_t)rec * s->np;
+    *n_steps = s->np ? rec : 0;
+    if (!s->np) return LX_OK;
+    if (n < m || !dst) return fail(LX_EINVAL, "probe buffer too small: need %zu", m);
+    HIP_TRY(hipMemcpy(dst, s->prec, sizeof(double) * m, hipMemcpyDeviceToHost));
+    return LX_OK;
+}
+
+lx_status lx_sim_monitor(lx_sim *s, double *max_e2, double *energy) {
+    if (!s) return fail(LX_EINVAL, "null simulation");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    launch_zero_bytes(s->mon, 32, s->cs);
+    if (s->fp16) launch_monitor(make_grid<__half>(s), s->planar, s->mon, s->cs);
+    else launch_monitor(make_grid<float>(s), s->planar, s->mon, s->cs);
+    lx_status st = check_launch();
+    if (st) return st;
+    double out[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(out, s->mon, sizeof(double) * 3, hipMemcpyDeviceToHost, s->cs));
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    const double vol = s->p.dx * s->p.dz * (s->planar ? 1.0 : s->p.dy);
+    if (max_e2) *max_e2 = out[0];
+    if (energy) *energy = 0.5 * s->eps0 * out[1] * vol;
+    if (out[2] > 0) return fail(LX_EUNSTABLE, "non-finite field values after %d steps", s->steps);
+    return LX_OK;
+}
+
+lx_status lx_sim_get_info(lx_sim *s, lx_sim_info *info) {
+    if (!s || !info) return fail(LX_EINVAL, "null pointer");
+    std::memset(info, 0, sizeof(*info));
+    info->dt = s->dt;
+    info->c0 = s->c0;
+    info->eps0 = s->eps0;
+    info->mu0 = s->mu0;
+    info->eta0 = s->eta0;
+    info->cells = (int64_t)s->N[0] * s->N[1] * s->N[2];
+    int64_t slab_bytes = 0;
+    for (int a = 0; a < 3; ++a) {
+        info->slab_cells[a] = s->slab_cells[a];
+        info->slab_lo[a] = s->slabs.lo[a];
+        info->slab_hi[a] = s->slabs.hi[a];
+        slab_bytes += s->slab_cells[a] * (s->planar ? 16 : 32);
+    }
+    const int64_t per = s->planar ? (9 * (int64_t)s->esize + 1) : (18 * (int64_t)s->esize + 1);
+    info->bytes_per_step = info->cells * per + slab_bytes;
+    info->device_bytes = s->bytes;
+    info->step = s->steps;
+    return LX_OK;
+}
+
+void *lx_sim_stream(lx_sim *s) { return s ? (void *)s->cs : nullptr; }
+
+lx_status lx_sim_time_kernels(lx_sim *s, int32_t nsteps, double *ms_h, double *ms_e, double *ms_step) {
+    if (!s || nsteps < 1) return fail(LX_EINVAL, "lx_sim_time_kernels: bad arguments");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    lx_status st = ensure_probe_cap(s, s->steps + nsteps);
+    if (st) return st;
+    std::vector<hipEvent_t> ev(4 * (size_t)nsteps + 2);
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventRecord(ev[4 * nsteps], s->cs));
+    for (int q = 0; q < nsteps; ++q) {
+        if (s->fp16) {
+            Grid<__half> g = make_grid<__half>(s);
+            if (s->p.use_ps) launch_point_source(g, s->ps, s->cs);
+            HIP_TRY(hipEventRecord(ev[4 * q + 0], s->cs));
+            if (s->planar) launch_h2(g, s->tfH, s->inc, s->cs); else launch_h3(g, s->tfH, s->inc, s->cs);
+            HIP_TRY(hipEventRecord(ev[4 * q + 1], s->cs));
+            if (s->use_aux) launch_aux_h(s->aux, s->cs);
+            HIP_TRY(hipEventRecord(ev[4 * q + 2], s->cs));
+            if (s->planar) launch_e2(g, s->tfE, s->inc, s->cs); else launch_e3(g, s->tfE, s->inc, s->cs);
+            HIP_TRY(hipEventRecord(ev[4 * q + 3], s->cs));
+            if (s->use_aux) launch_aux_e(s->aux, s->counter, s->cs);
+            if (s->np) { ProbeSet ps{s->np, s->pidx, s->pcomp, s->prec, s->pcap, 1.0 / s->eta0}; launch_probes(g, ps, s->cs); }
+        } else {
+            Grid<float> g = make_grid<float>(s);
+            if (s->p.use_ps) launch_point_source(g, s->ps, s->cs);
+            HIP_TRY(hipEventRecord(ev[4 * q + 0], s->cs));
+            if (s->planar) launch_h2(g, s->tfH, s->inc, s->cs); else launch_h3(g, s->tfH, s->inc, s->cs);
+            HIP_TRY(hipEventRecord(ev[4 * q + 1], s->cs));
+            if (s->use_aux) launch_aux_h(s->aux, s->cs);
+            HIP_TRY(hipEventRecord(ev[4 * q + 2], s->cs));
+            if (s->planar) launch_e2(g, s->tfE, s->inc, s->cs); else launch_e3(g, s->tfE, s->inc, s->cs);
+            HIP_TRY(hipEventRecord(ev[4 * q + 3], s->cs));
+            if (s->use_aux) launch_aux_e(s->aux, s->counter, s->cs);
+            if (s->np) { ProbeSet ps{s->np, s->pidx, s->pcomp, s->prec, s->pcap, 1.0 / s->eta0}; launch_probes(g, ps, s->cs); }
+        }
+    }
+    HIP_TRY(hipEventRecord(ev[4 * nsteps + 1], s->cs));
+    if ((st = check_launch())) return st;
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    double th = 0, te = 0;
+    for (int q = 0; q < nsteps; ++q) {
+        float a = 0, b = 0;
+        HIP_TRY(hipEventElapsedTime(&a, ev[4 * q + 0], ev[4 * q + 1]));
+        HIP_TRY(hipEventElapsedTime(&b, ev[4 * q + 2], ev[4 * q + 3]));
+        th += a;
+        te += b;
+    }
+    float tot = 0;
+    HIP_TRY(hipEventElapsedTime(&tot, ev[4 * nsteps], ev[4 * nsteps + 1]));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    s->steps += nsteps;
+    if (ms_h) *ms_h = th / nsteps;
+    if (ms_e) *ms_e = te / nsteps;
+    if (ms_step) *ms_step = tot / nsteps;
+    return LX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerance: relative L2 <= 1e-5 for FP32 storage on every field component
+and on the probe time series (the FP64 oracle is the reference).  FP16
+storage has its own measured bound (test_fp16_bound).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from luxum_amd import configs
+from luxum_amd.params import (LX_EX, LX_EY, LX_EZ, LX_FP16, LX_HX, LX_HY, LX_HZ, LX_INC_ANALYTIC,
+                              LX_INC_AUX1D, LX_SRC_HARD_CW, LX_SRC_SOFT, default_params)
+from oracle.lxoracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+TOL32 = 1e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_pair(ctx, p, steps, ids=None, eps=None, probes=None):
+    from luxum_amd.sim import Simulation
+    say(f"case nx={p.nx} ny={p.ny} nz={p.nz} planar={p.is_planar} pml={p.use_pml} pw={p.use_pw} "
+        f"inc={p.incident} ps={p.use_ps} prec={p.precision} steps={steps}")
+    g = Simulation(p, ctx)
+    o = OracleSim(p)
+    if ids is not None:
+        g.set_materials(ids, eps)
+        o.set_materials(ids, eps)
+    if probes is not None:
+        g.set_probes(probes)
+        o.set_probes(probes)
+    g.step(steps)
+    o.step(steps)
+    g.sync()
+    return g, o
+
+
+def field_errors(g, o):
+    """Relative L2 of the E and H vector fields (all components jointly)."""
+    out = {}
+    comps = g.p.components()
+    for name, group in (("E", [c for c in comps if c < 3]), ("H", [c for c in comps if c >= 3])):
+        num = den = 0.0
+        for c in group:
+            a, b = g.field(c), o.field(c)
+            assert a.shape == b.shape
+            assert np.all(np.isfinite(a))
+            num += float(np.sum((a - b) ** 2))
+            den += float(np.sum(b ** 2))
+        out[name] = (num / den) ** 0.5 if den > 0 else num ** 0.5
+    return out
+
+
+def check_fields(g, o, tol, label=""):
+    err = field_errors(g, o)
+    say(f"{label}: rel L2 E {err['E']:.3e} H {err['H']:.3e}")
+    for k, e in err.items():
+        assert e <= tol, f"{label} {k}: rel L2 {e:.3e} > {tol:.1e}"
+    return max(err.values())
+
+
+def test_3d_cpml_aux_cube(gpu_ctx):
+    w = configs.config1(n=48, steps=300, pml=8, margin=6)
+    g, o = run_pair(gpu_ctx, w.p, 300, w.ids, w.eps_r, w.probes)
+    worst = check_fields(g, o, TOL32, "3d cpml aux")
+    pg, po = g.probes(), o.probes()
+    assert pg.shape == po.shape == (300, 8)
+    e = rel_l2(pg, po)
+    say(f"3d cpml aux: fields {worst:.2e} probes {e:.2e}")
+    assert e <= TOL32
+
+
+def test_3d_plain_analytic_oblique(gpu_ctx):
+    p = default_params(nx=40, ny=36, nz=34, use_pw=1, incident=LX_INC_ANALYTIC, off_pw=90.0,
+                       pw_sigma=(20 * 5e-8) ** 2, tf_lo=(8, 7, 6), tf_hi=(30, 27, 26))
+    p.k_pw[:] = (0.5773502691896258,) * 3
+    p.e_pw[:] = (0.70710678118, 0.0, -0.70710678118)
+    g, o = run_pair(gpu_ctx, p, 160)
+    check_fields(g, o, TOL32, "3d plain analytic")
+
+
+def test_3d_cpml_analytic_anisotropic(gpu_ctx):
+    p = default_params(nx=36, ny=30, nz=40, dx=5e-8, dy=4e-8, dz=6e-8, courant=0.45, use_pml=1,
+                       pml_thickness=6 * 5e-8, use_pw=1, incident=LX_INC_ANALYTIC, off_pw=80.0,
+                       pw_sigma=(20 * 5e-8) ** 2, tf_lo=(12, 10, 12), tf_hi=(24, 20, 26))
+    p.k_pw[:] = (0.0, 0.6, 0.8)
+    p.e_pw[:] = (1.0, 0.0, 0.0)
+    g, o = run_pair(gpu_ctx, p, 150)
+    check_fields(g, o, TOL32, "3d cpml analytic aniso")
+
+
+def test_2d_cpml_aux(gpu_ctx):
+    w = configs.config0(n=96, steps=400)
+    g, o = run_pair(gpu_ctx, w.p, 400, w.ids, w.eps_r, w.probes)
+    worst = check_fields(g, o, TOL32, "2d cpml aux")
+    e = rel_l2(g.probes(), o.probes())
+    say(f"2d cpml aux: fields {worst:.2e} probes {e:.2e}")
+    assert e <= TOL32
+
+
+def test_2d_plain_analytic(gpu_ctx):
+    p = default_params(is_planar=1, nx=80, ny=1, nz=70, dy=0.0, use_pw=1, incident=LX_INC_ANALYTIC,
+                       off_pw=90.0, pw_sigma=(20 * 5e-8) ** 2, tf_lo=(10, 0, 10), tf_hi=(60, 0, 55),
+                       courant=0.5, src_comp=LX_EY)
+    p.k_pw[:] = (0.70710678118, 0.0, 0.70710678118)
+    p.e_pw[:] = (0.0, 1.0, 0.0)
+    g, o = run_pair(gpu_ctx, p, 200)
+    check_fields(g, o, TOL32, "2d plain analytic")
+
+
+def test_point_sources(gpu_ctx):
+    # reference hard CW source (int amplitude and omega), 3D CPML
+    p = default_params(nx=32, ny=30, nz=28, use_pml=1, pml_thickness=6 * 5e-8, use_ps=1,
+                       src_kind=LX_SRC_HARD_CW, src_amp=2.7, src_omega=3.0e15, src_pos=(16, 15, 14))
+    g, o = run_pair(gpu_ctx, p, 120)
+    check_fields(g, o, TOL32, "hard cw")
+    # soft Gaussian-modulated dipole
+    p2 = default_params(nx=32, ny=30, nz=28, use_pml=1, pml_thickness=6 * 5e-8, use_ps=1,
+                        src_kind=LX_SRC_SOFT, src_amp=1.0, src_lambda0=20 * 5e-8, src_fwhm=3.0,
+                        src_delay=4.0, src_pos=(15, 14, 13))
+    g, o = run_pair(gpu_ctx, p2, 150)
+    check_fields(g, o, TOL32, "soft dipole")
+
+
+def test_2d_soft_source(gpu_ctx):
+    p = default_params(is_planar=1, nx=64, ny=1, nz=60, dy=0.0, use_pml=1, pml_thickness=8 * 5e-8, use_ps=1,
+                       src_kind=LX_SRC_SOFT, src_comp=LX_EY, src_amp=1.0, src_lambda0=20 * 5e-8,
+                       src_fwhm=3.0, src_delay=4.0, src_pos=(30, 0, 28))
+    g, o = run_pair(gpu_ctx, p, 200)
+    check_fields(g, o, TOL32, "2d soft")
+
+
+def test_monitor_and_snapshot(gpu_ctx):
+    w = configs.config1(n=32, steps=120, pml=6, margin=4)
+    g, o = run_pair(gpu_ctx, w.p, 120, w.ids, w.eps_r)
+    mx, en = g.monitor()
+    ref = o.max_e2()
+    assert abs(mx - ref) <= 1e-4 * ref
+    snap = np.zeros(w.p.field_shape(LX_EZ))
+    g.snapshot_async(LX_EZ, snap)
+    g.sync()
+    assert np.array_equal(snap, g.field(LX_EZ))
+
+
+def test_fp16_bound(gpu_ctx):
+    w = configs.config1(n=48, steps=300, pml=8, margin=6, precision=LX_FP16)
+    g, o = run_pair(gpu_ctx, w.p, 300, w.ids, w.eps_r, w.probes)
+    worst = max(field_errors(g, o).values())
+    e = rel_l2(g.probes(), o.probes())
+    say(f"fp16 storage: fields {worst:.2e} probes {e:.2e}")
+    assert worst <= 2e-2 and e <= 2e-2
+
+
+def test_chunked_stepping_matches(gpu_ctx):
+    """Graph replay (chunks of 32 steps) and eager steps give the same bits."""
+    from luxum_amd.sim import Simulation
+    w = configs.config1(n=32, steps=100, pml=6, margin=4)
+    a = Simulation(w.p, gpu_ctx)
+    b = Simulation(w.p, gpu_ctx)
+    for s in (a, b):
+        s.set_materials(w.ids, w.eps_r)
+    a.step(100)
+    for _ in range(100):
+        b.step(1)
+    for c in w.p.components():
+        assert np.array_equal(a.field(c), b.field(c))
