@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=64)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--fp16", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-steps", type=int, default=50)
     args = ap.parse_args()
@@ -69,6 +69,8 @@ def main():
     sim = Simulation(w.p, ctx)
     sim.set_materials(w.ids, w.eps_r)
     sim.set_probes(w.probes)
+    kz = sim.autotune()
+    print(f"bench: autotuned z-block depth {kz}", file=sys.stderr, flush=True)
     print(f"bench: {w.name} {'fp16' if args.fp16 else 'fp32'} warmup {args.warmup}", file=sys.stderr, flush=True)
     sim.step(args.warmup)
     sim.sync()
@@ -112,7 +114,7 @@ def main():
                                f"8 probes, {'FP16 storage' if args.fp16 else 'FP32'}",
                    "cells": info.cells, "bytes_per_step": info.bytes_per_step,
                    "step_GBps": round(step_gbs, 1), "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
-                   "ms_h": round(ms_h, 5), "ms_e": round(ms_e, 5), "max_E2": mx},
+                   "ms_h": round(ms_h, 5), "ms_e": round(ms_e, 5), "max_E2": mx, "kz": kz},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": dbytes, "ms_per_launch": round(dms, 5)},

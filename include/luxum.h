@@ -203,6 +203,12 @@ void *lx_sim_stream(lx_sim *sim);
 lx_status lx_sim_time_kernels(lx_sim *sim, int32_t nsteps, double *ms_h, double *ms_e,
                               double *ms_step);
 
+/* No reference counterpart: choose the z-block depth of the bulk kernels by
+ * timing the candidates on this grid (before the first step; the state is
+ * reset afterwards).  Results are cached per (shape, precision, CPML) for the
+ * process.  *kz_out receives the chosen depth (may be NULL). */
+lx_status lx_sim_autotune(lx_sim *sim, int32_t *kz_out);
+
 /* ---- batches of independent simulations ------------------------------- */
 
 /* No reference counterpart: advance many simulations of uneven sizes in one

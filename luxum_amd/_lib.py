@@ -25,7 +25,7 @@ EXPORTS = (
     "lx_sim_create", "lx_sim_destroy", "lx_sim_set_materials", "lx_sim_set_probes",
     "lx_sim_step", "lx_sim_sync", "lx_sim_read_field", "lx_sim_snapshot_async",
     "lx_sim_read_probes", "lx_sim_monitor", "lx_sim_get_info", "lx_sim_stream",
-    "lx_sim_time_kernels",
+    "lx_sim_time_kernels", "lx_sim_autotune",
     "lx_batch_create", "lx_batch_destroy", "lx_batch_add", "lx_batch_remove",
     "lx_batch_step", "lx_batch_sync",
     "lx_last_error", "lx_version",
@@ -88,6 +88,7 @@ def lib():
         "lx_sim_stream": ([vp], vp),
         "lx_sim_time_kernels": ([vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)], st),
+        "lx_sim_autotune": ([vp, C.POINTER(C.c_int32)], st),
         "lx_batch_create": ([vp, C.POINTER(vp)], st),
         "lx_batch_destroy": ([vp], None),
         "lx_batch_add": ([vp, vp], st),

@@ -19,7 +19,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <new>
 #include <vector>
 
@@ -60,16 +62,18 @@ struct lx_sim {
     hipStream_t cs = nullptr, ss = nullptr;
     std::vector<void *> allocs;
     int64_t bytes = 0;
-    void *E[3] = {nullptr, nullptr, nullptr}, *H[3] = {nullptr, nullptr, nullptr};
+    void *F = nullptr;          // six components, component c at F + c*fstride elements
+    long long fstride = 0;
     uint8_t *mat = nullptr;
     float *cb = nullptr;
     float *tab = nullptr;
     float *psi = nullptr;
     int *counter = nullptr;
-    Tables tables{};
-    Slabs slabs{};
+    int tn = 0;
+    int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0}, sw[3] = {0, 0, 0};
     int64_t slab_cells[3] = {0, 0, 0};
-    TfSet tfH{}, tfE{};
+    TfList tfH{}, tfE{};
+    int kz = 32;
     IncParams inc{};
     AuxLine aux{};
     int use_aux = 0;
@@ -106,16 +110,30 @@ Grid<T> make_grid(const lx_sim *s) {
     g.nz = s->N[2];
     g.px = s->px;
     g.plane = s->plane;
-    g.kz = 16;
-    for (int c = 0; c < 3; ++c) {
-        g.f.E[c] = (T *)s->E[c];
-        g.f.H[c] = (T *)s->H[c];
-    }
+    g.fstride = s->fstride;
+    g.F = (T *)s->F;
     g.mat = s->mat;
     g.cb = s->cb;
-    g.tab = s->tables;
-    g.slab = s->slabs;
+    g.tab = s->tab;
+    g.tn = s->tn;
+    g.psi = s->psi;
+    for (int a = 0; a < 3; ++a) {
+        g.scells[a] = s->slab_cells[a];
+        g.slo[a] = s->slo[a];
+        g.shi[a] = s->shi[a];
+        g.sw[a] = s->sw[a];
+    }
     g.pml = s->pml;
+    g.kz = s->kz;
+    g.ntx = (g.nx + 255) / 256;
+    if (s->planar) {
+        g.nty = (g.nz + kBY - 1) / kBY;
+        g.ntz = 1;
+    } else {
+        g.nty = (g.ny + kBY - 1) / kBY;
+        g.ntz = (g.nz + g.kz - 1) / g.kz;
+    }
+    g.ntiles = g.ntx * g.nty * g.ntz;
     g.counter = s->counter;
     return g;
 }
@@ -174,7 +192,9 @@ void build_profile(const lx_sim *s, int dim, int is_E, int n, Prof &pr) {
 
 lx_status build_tables(lx_sim *s) {
     const int axes = 3;
-    int Nmax = std::max(s->N[0], std::max(s->N[1], s->N[2]));
+    // rows cover the padded x pitch (kernels read 4 x cells at a time) and are 16-byte aligned
+    int Nmax = std::max(s->px, std::max(s->N[1], s->N[2]));
+    Nmax = (Nmax + 3) / 4 * 4;
     std::vector<float> host((size_t)axes * 6 * Nmax, 0.f);
     const double d[3] = {s->p.dx, s->p.dy, s->p.dz};
     const double ch = s->c0 * s->dt / s->p.dx;
@@ -194,7 +214,7 @@ lx_status build_tables(lx_sim *s) {
               *chh = &host[((size_t)a * 6 + 2) * Nmax], *ke = &host[((size_t)a * 6 + 3) * Nmax],
               *be = &host[((size_t)a * 6 + 4) * Nmax], *ce = &host[((size_t)a * 6 + 5) * Nmax];
         for (int i = 0; i < N; ++i) {
-            const int ih = std::min(i, (int)ph.kap.size() - 1);
+            const int ih = std::min(i, (int)ph.kap.size() - 1);   // the H table has N-1 entries; pad the last
             kh[i] = (float)(ch * r / ph.kap[ih]);
             bh[i] = (float)ph.b[ih];
             chh[i] = (float)(ch * ph.c[ih] * r);
@@ -211,43 +231,27 @@ lx_status build_tables(lx_sim *s) {
             for (int i = lo; i < hi; ++i)
                 if (nz(i)) return fail(LX_EINVAL, "CPML profile along axis %d is not confined to the boundary slabs", a);
         }
-        s->slabs.lo[a] = lo;
-        s->slabs.hi[a] = hi;
-        s->slabs.w[a] = lo + (N - hi);
+        s->slo[a] = lo;
+        s->shi[a] = hi;
+        s->sw[a] = lo + (N - hi);
     }
     void *p = nullptr;
     lx_status st = dalloc(s, &p, host.size() * sizeof(float));
     if (st) return st;
     s->tab = (float *)p;
+    s->tn = Nmax;
     HIP_TRY(hipMemcpy(s->tab, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice));
-    for (int a = 0; a < 3; ++a) {
-        s->tables.kh[a] = s->tab + ((size_t)a * 6 + 0) * Nmax;
-        s->tables.bh[a] = s->tab + ((size_t)a * 6 + 1) * Nmax;
-        s->tables.chh[a] = s->tab + ((size_t)a * 6 + 2) * Nmax;
-        s->tables.ke[a] = s->tab + ((size_t)a * 6 + 3) * Nmax;
-        s->tables.be[a] = s->tab + ((size_t)a * 6 + 4) * Nmax;
-        s->tables.ce[a] = s->tab + ((size_t)a * 6 + 5) * Nmax;
-    }
-    // psi slabs: per axis 2 H + 2 E arrays of w_a * (product of the other extents)
+    // psi slabs: per axis [H slot0, H slot1, E slot0, E slot1], each w_a * (product of the other extents)
+    // x slab: (k, j, slab x) ; y slab: (k, slab y, px) ; z slab: (slab z, j, px)
     size_t total = 0;
-    for (int a = 0; a < 3; ++a) {
-        long long other = 1;
-        for (int b = 0; b < 3; ++b)
-            if (b != a) other *= s->N[b];
-        s->slab_cells[a] = (int64_t)s->slabs.w[a] * other;
-        total += 4 * (size_t)s->slab_cells[a];
-    }
+    s->slab_cells[0] = (int64_t)s->sw[0] * s->N[1] * s->N[2];
+    s->slab_cells[1] = (int64_t)s->sw[1] * s->N[2] * s->px;
+    s->slab_cells[2] = (int64_t)s->sw[2] * s->N[1] * s->px;
+    for (int a = 0; a < 3; ++a) total += 4 * (size_t)s->slab_cells[a];
     st = dalloc(s, &p, std::max<size_t>(total, 1) * sizeof(float));
     if (st) return st;
     s->psi = (float *)p;
     launch_zero_bytes(s->psi, std::max<size_t>(total, 1) * sizeof(float), s->cs);
-    size_t off = 0;
-    for (int a = 0; a < 3; ++a) {
-        for (int w = 0; w < 2; ++w) {
-            s->slabs.psiH[a][w] = s->psi + off; off += s->slab_cells[a];
-            s->slabs.psiE[a][w] = s->psi + off; off += s->slab_cells[a];
-        }
-    }
     return LX_OK;
 }
 
@@ -275,19 +279,11 @@ void pw_clip(int lo, int hi, int N, int *st, int *en) {
     else { *st = -1; *en = -1; }
 }
 
-void empty_box(TfBox &b) {
-    b.lo[0] = b.lo[1] = b.lo[2] = 1;
-    b.hi[0] = b.hi[1] = b.hi[2] = 0;
-    b.off[0] = b.off[1] = b.off[2] = 0.f;
-    b.dir = 0.0;
-}
-
-void build_tfsf(lx_sim *s) {
-    for (int c = 0; c < 3; ++c)
-        for (int w = 0; w < 2; ++w)
-            for (int f = 0; f < 2; ++f) { empty_box(s->tfH.box[c][w][f]); empty_box(s->tfE.box[c][w][f]); }
-    s->tfH.any = s->tfE.any = 0;
-    if (!s->p.use_pw) return;
+lx_status build_tfsf(lx_sim *s) {
+    s->tfH.n = s->tfE.n = 0;
+    s->tfH.total = s->tfE.total = 0;
+    s->tfH.dev = s->tfE.dev = nullptr;
+    if (!s->p.use_pw) return LX_OK;
     const lx_params &p = s->p;
     const bool three = !s->planar;
     double Edir[3], Hdir[3];
@@ -304,7 +300,7 @@ void build_tfsf(lx_sim *s) {
         pw_clip(p.tf_lo[a], p.tf_hi[a], s->N[a], &st[a], &en[a]);
     }
     for (int half = 0; half < 2; ++half) {   // 0: H update (curl E), 1: E update (curl H)
-        TfSet &set = half == 0 ? s->tfH : s->tfE;
+        TfList &set = half == 0 ? s->tfH : s->tfE;
         for (int a = 0; a < 3; ++a) {
             if (!three && a == 1) continue;
             const int b0 = (a + 1) % 3, b1 = (a + 2) % 3;
@@ -338,7 +334,12 @@ void build_tfsf(lx_sim *s) {
                         if (!three) dir = -dir;            // explicit (-1) of maxwell.cc:563-588
                     }
                     if (dir == 0.0) continue;
-                    TfBox &bx = set.box[tgt % 3][which - 1][side];
+                    if (set.n >= kMaxTf) return fail(LX_EINVAL, "too many TF/SF boxes");
+                    TfBox &bx = set.b[set.n++];
+                    bx.comp = tgt % 3;
+                    bx.curl = which - 1;
+                    bx.fa = a;
+                    bx.dir = dir;
                     bx.lo[a] = bx.hi[a] = idx;
                     bx.off[a] = (float)(coord - idx);
                     for (int b : {b0, b1}) {
@@ -348,12 +349,27 @@ void build_tfsf(lx_sim *s) {
                         bx.hi[b] = en[b] - sg;
                         bx.off[b] = sg ? 0.5f : 0.f;
                     }
-                    bx.dir = dir;
-                    set.any = 1;
                 }
             }
         }
     }
+    for (TfList *L : {&s->tfH, &s->tfE}) {
+        int start = 0;
+        for (int q = 0; q < L->n; ++q) {
+            TfBox &bx = L->b[q];
+            bx.start = start;
+            int cnt = 1;
+            for (int a = 0; a < 3; ++a) cnt *= std::max(0, bx.hi[a] - bx.lo[a] + 1);
+            start += cnt;
+        }
+        L->total = start;
+        void *q = nullptr;
+        lx_status e = dalloc(s, &q, sizeof(TfBox) * kMaxTf);
+        if (e) return e;
+        HIP_TRY(hipMemcpy(q, L->b, sizeof(TfBox) * kMaxTf, hipMemcpyHostToDevice));
+        L->dev = (const TfBox *)q;
+    }
+    return LX_OK;
 }
 
 double waveform_host(int kind, double t, double c, double lambda0, double fwhm, double delay) {
@@ -415,27 +431,44 @@ void fill_cb_vacuum(lx_sim *s) {
     for (int q = 0; q < 256; ++q) s->cb_host[q] = (float)v;
 }
 
-template <typename T>
-void enqueue_one_t(lx_sim *s) {
-    Grid<T> g = make_grid<T>(s);
-    if (s->p.use_ps) launch_point_source(g, s->ps, s->cs);
-    if (s->planar) launch_h2(g, s->tfH, s->inc, s->cs);
-    else launch_h3(g, s->tfH, s->inc, s->cs);
-    if (s->use_aux) launch_aux_h(s->aux, s->cs);
-    if (s->planar) launch_e2(g, s->tfE, s->inc, s->cs);
-    else launch_e3(g, s->tfE, s->inc, s->cs);
-    if (s->use_aux) launch_aux_e(s->aux, s->counter, s->cs);
-    if (s->np) {
-        ProbeSet ps{s->np, s->pidx, s->pcomp, s->prec, s->pcap, 1.0 / s->eta0};
-        launch_probes(g, ps, s->cs);
-    }
+ProbeArgs probe_args(const lx_sim *s) {
+    ProbeArgs pr{};
+    pr.n = s->np;
+    pr.comp = s->pcomp;
+    pr.idx = s->pidx;
+    pr.rec = s->prec;
+    pr.cap = s->pcap;
+    pr.hscale = 1.0 / s->eta0;
+    return pr;
 }
 
-void enqueue_steps(lx_sim *s, int n) {
+// One launch sequence of n timesteps: per step [point source] H launch,
+// E launch; then the sequence-end kernel.  Kernels read the step index as
+// counter[0] + toff, so the same sequence can be captured once and replayed.
+template <typename T>
+void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
+    Grid<T> g = make_grid<T>(s);
+    const ProbeArgs pr = probe_args(s);
+    const bool corr = s->tfH.n > 0 || s->use_aux || s->np > 0;
     for (int q = 0; q < n; ++q) {
-        if (s->fp16) enqueue_one_t<__half>(s);
-        else enqueue_one_t<float>(s);
+        Launch l{q, s->use_aux, s->np > 0};
+        if (s->p.use_ps) launch_point_source(g, s->ps, q, s->cs);
+        if (ev) (void)hipEventRecord((*ev)[4 * q + 0], s->cs);
+        launch_bulk(g, s->planar, 0, s->cs);
+        if (ev) (void)hipEventRecord((*ev)[4 * q + 1], s->cs);
+        if (corr) launch_tfsf(g, s->tfH, s->inc, s->aux, pr, l, 0, s->cs);
+        if (ev) (void)hipEventRecord((*ev)[4 * q + 2], s->cs);
+        launch_bulk(g, s->planar, 1, s->cs);
+        if (ev) (void)hipEventRecord((*ev)[4 * q + 3], s->cs);
+        if (corr) launch_tfsf(g, s->tfE, s->inc, s->aux, pr, l, 1, s->cs);
     }
+    launch_finish(g, pr, n, s->cs);
+}
+
+void enqueue_seq(lx_sim *s, int n, std::vector<hipEvent_t> *ev = nullptr) {
+    if (n <= 0) return;
+    if (s->fp16) enqueue_seq_t<__half>(s, n, ev);
+    else enqueue_seq_t<float>(s, n, ev);
 }
 
 void drop_graph(lx_sim *s) {
@@ -450,7 +483,7 @@ lx_status ensure_graph(lx_sim *s, int K) {
     drop_graph(s);
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(s->cs, hipStreamCaptureModeThreadLocal));
-    enqueue_steps(s, K);
+    enqueue_seq(s, K);
     hipError_t e = hipStreamEndCapture(s->cs, &graph);
     if (e != hipSuccess || !graph) {
         (void)hipGetLastError();
@@ -590,18 +623,10 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
             return bail(fail(LX_EDEVICE, "event creation failed"));
     }
     const size_t ncell = (size_t)s->plane * s->N[2];
-    const size_t fbytes = ncell * s->esize;
-    for (int c = 0; c < 3; ++c) {
-        int ext[3];
-        if (comp_extent(s->N, s->planar, c, ext)) {
-            if ((st = dalloc(s, &s->E[c], fbytes))) return bail(st);
-            launch_zero_bytes(s->E[c], (fbytes + 3) / 4 * 4, s->cs);
-        }
-        if (comp_extent(s->N, s->planar, c + 3, ext)) {
-            if ((st = dalloc(s, &s->H[c], fbytes))) return bail(st);
-            launch_zero_bytes(s->H[c], (fbytes + 3) / 4 * 4, s->cs);
-        }
-    }
+    s->fstride = (long long)((ncell + 127) / 128 * 128);   // 256-byte aligned components
+    const size_t fbytes = (size_t)s->fstride * 6 * s->esize;
+    if ((st = dalloc(s, &s->F, fbytes))) return bail(st);
+    launch_zero_bytes(s->F, fbytes, s->cs);
     void *q = nullptr;
     if ((st = dalloc(s, &q, ncell))) return bail(st);
     s->mat = (uint8_t *)q;
@@ -617,7 +642,7 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
     if ((st = dalloc(s, &q, 64))) return bail(st);
     s->mon = (double *)q;
     if ((st = build_tables(s))) return bail(st);
-    build_tfsf(s);
+    if ((st = build_tfsf(s))) return bail(st);
     if ((st = build_incident(s))) return bail(st);
     if (p->use_ps) {
         PointSrc &ps = s->ps;
@@ -723,7 +748,7 @@ lx_status lx_sim_step(lx_sim *s, int32_t nsteps) {
                 for (; done + K <= n; done += K) HIP_TRY(hipGraphLaunch(s->gexec, s->cs));
             }
         }
-        enqueue_steps(s, n - done);
+        enqueue_seq(s, n - done);
         if ((st = check_launch())) return st;
         s->steps += n;
         left -= n;
@@ -745,8 +770,8 @@ lx_status lx_sim_sync(lx_sim *s) {
 
 static lx_status gather(lx_sim *s, int comp, double *dst_dev, int ext[3]) {
     const double scale = comp >= 3 ? 1.0 / s->eta0 : 1.0;
-    if (s->fp16) launch_gather(make_grid<__half>(s), comp, ext[0], ext[1], ext[2], scale, dst_dev, s->planar, s->cs);
-    else launch_gather(make_grid<float>(s), comp, ext[0], ext[1], ext[2], scale, dst_dev, s->planar, s->cs);
+    if (s->fp16) launch_gather(make_grid<__half>(s), comp, ext[0], ext[1], ext[2], scale, dst_dev, s->cs);
+    else launch_gather(make_grid<float>(s), comp, ext[0], ext[1], ext[2], scale, dst_dev, s->cs);
     return check_launch();
 }
 
@@ -840,8 +865,8 @@ lx_status lx_sim_get_info(lx_sim *s, lx_sim_info *info) {
     int64_t slab_bytes = 0;
     for (int a = 0; a < 3; ++a) {
         info->slab_cells[a] = s->slab_cells[a];
-        info->slab_lo[a] = s->slabs.lo[a];
-        info->slab_hi[a] = s->slabs.hi[a];
+        info->slab_lo[a] = s->slo[a];
+        info->slab_hi[a] = s->shi[a];
         slab_bytes += s->slab_cells[a] * (s->planar ? 16 : 32);
     }
     const int64_t per = s->planar ? (9 * (int64_t)s->esize + 1) : (18 * (int64_t)s->esize + 1);
@@ -853,6 +878,85 @@ lx_status lx_sim_get_info(lx_sim *s, lx_sim_info *info) {
 
 void *lx_sim_stream(lx_sim *s) { return s ? (void *)s->cs : nullptr; }
 
+static lx_status reset_state(lx_sim *s) {
+    launch_zero_bytes(s->F, (size_t)s->fstride * 6 * s->esize, s->cs);
+    size_t total = 0;
+    for (int a = 0; a < 3; ++a) total += 4 * (size_t)s->slab_cells[a];
+    launch_zero_bytes(s->psi, std::max<size_t>(total, 1) * sizeof(float), s->cs);
+    launch_zero_bytes(s->counter, 64, s->cs);
+    if (s->prec) launch_zero_bytes(s->prec, sizeof(double) * (size_t)s->pcap * s->np, s->cs);
+    if (s->use_aux) {
+        std::vector<double> init(2 * (size_t)s->aux.L, 0.0);
+        init[0] = waveform_host(s->aux.wave, 0.0, s->aux.c, s->aux.lambda0, s->aux.fwhm, s->aux.delay);
+        HIP_TRY(hipMemcpyAsync(s->aux.e, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice, s->cs));
+    }
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    s->steps = 0;
+    return check_launch();
+}
+
+lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
+    if (!s) return fail(LX_EINVAL, "null simulation");
+    if (s->steps != 0) return fail(LX_EINVAL, "lx_sim_autotune must run before the first step");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    using Key = std::tuple<int, int, int, int, int, int>;
+    static std::mutex mu;
+    static std::map<Key, int> cache;
+    const Key key{s->N[0], s->N[1], s->N[2], s->fp16, s->pml, s->planar};
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            s->kz = it->second;
+            drop_graph(s);
+            if (kz_out) *kz_out = s->kz;
+            return LX_OK;
+        }
+    }
+    if (!s->planar) {
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        float best = 1e30f;
+        int best_kz = s->kz;
+        for (int kz : {4, 8, 16, 32, 64}) {
+            if (kz > s->N[2] && kz != 4) continue;
+            s->kz = kz;
+            for (int rep = 0; rep < 2; ++rep) {
+                HIP_TRY(hipEventRecord(e0, s->cs));
+                for (int q = 0; q < 4; ++q) {
+                    if (s->fp16) {
+                        const Grid<__half> g = make_grid<__half>(s);
+                        launch_bulk(g, 0, 0, s->cs);
+                        launch_bulk(g, 0, 1, s->cs);
+                    } else {
+                        const Grid<float> g = make_grid<float>(s);
+                        launch_bulk(g, 0, 0, s->cs);
+                        launch_bulk(g, 0, 1, s->cs);
+                    }
+                }
+                HIP_TRY(hipEventRecord(e1, s->cs));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                if (rep == 1 && ms < best) { best = ms; best_kz = kz; }
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        s->kz = best_kz;
+    }
+    drop_graph(s);
+    lx_status st = reset_state(s);
+    if (st) return st;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        cache[key] = s->kz;
+    }
+    if (kz_out) *kz_out = s->kz;
+    return LX_OK;
+}
+
 lx_status lx_sim_time_kernels(lx_sim *s, int32_t nsteps, double *ms_h, double *ms_e, double *ms_step) {
     if (!s || nsteps < 1) return fail(LX_EINVAL, "lx_sim_time_kernels: bad arguments");
     HIP_TRY(hipSetDevice(s->ctx->device));
@@ -861,33 +965,7 @@ lx_status lx_sim_time_kernels(lx_sim *s, int32_t nsteps, double *ms_h, double *m
     std::vector<hipEvent_t> ev(4 * (size_t)nsteps + 2);
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(ev[4 * nsteps], s->cs));
-    for (int q = 0; q < nsteps; ++q) {
-        if (s->fp16) {
-            Grid<__half> g = make_grid<__half>(s);
-            if (s->p.use_ps) launch_point_source(g, s->ps, s->cs);
-            HIP_TRY(hipEventRecord(ev[4 * q + 0], s->cs));
-            if (s->planar) launch_h2(g, s->tfH, s->inc, s->cs); else launch_h3(g, s->tfH, s->inc, s->cs);
-            HIP_TRY(hipEventRecord(ev[4 * q + 1], s->cs));
-            if (s->use_aux) launch_aux_h(s->aux, s->cs);
-            HIP_TRY(hipEventRecord(ev[4 * q + 2], s->cs));
-            if (s->planar) launch_e2(g, s->tfE, s->inc, s->cs); else launch_e3(g, s->tfE, s->inc, s->cs);
-            HIP_TRY(hipEventRecord(ev[4 * q + 3], s->cs));
-            if (s->use_aux) launch_aux_e(s->aux, s->counter, s->cs);
-            if (s->np) { ProbeSet ps{s->np, s->pidx, s->pcomp, s->prec, s->pcap, 1.0 / s->eta0}; launch_probes(g, ps, s->cs); }
-        } else {
-            Grid<float> g = make_grid<float>(s);
-            if (s->p.use_ps) launch_point_source(g, s->ps, s->cs);
-            HIP_TRY(hipEventRecord(ev[4 * q + 0], s->cs));
-            if (s->planar) launch_h2(g, s->tfH, s->inc, s->cs); else launch_h3(g, s->tfH, s->inc, s->cs);
-            HIP_TRY(hipEventRecord(ev[4 * q + 1], s->cs));
-            if (s->use_aux) launch_aux_h(s->aux, s->cs);
-            HIP_TRY(hipEventRecord(ev[4 * q + 2], s->cs));
-            if (s->planar) launch_e2(g, s->tfE, s->inc, s->cs); else launch_e3(g, s->tfE, s->inc, s->cs);
-            HIP_TRY(hipEventRecord(ev[4 * q + 3], s->cs));
-            if (s->use_aux) launch_aux_e(s->aux, s->counter, s->cs);
-            if (s->np) { ProbeSet ps{s->np, s->pidx, s->pcomp, s->prec, s->pcap, 1.0 / s->eta0}; launch_probes(g, ps, s->cs); }
-        }
-    }
+    enqueue_seq(s, nsteps, &ev);
     HIP_TRY(hipEventRecord(ev[4 * nsteps + 1], s->cs));
     if ((st = check_launch())) return st;
     HIP_TRY(hipStreamSynchronize(s->cs));

@@ -1,19 +1,21 @@
 // lx_internal.hpp -- device data layout and kernel parameter blocks.
 //
 // Device layout (one simulation):
-//   * Each field component is one array of nz planes x ny rows x px columns,
-//     x fastest (px = nx rounded up to 64 elements).  The reference's C order
-//     has z fastest (lib/field/field.h:54); x-fastest lets every wavefront
-//     read 64 consecutive x cells while the kernels march along z.
-//   * All six components share that shape; a component's staggered axis uses
-//     n-1 entries (lib/field/vfield.cc:75-84) and the unused tail stays zero.
+//   * The six field components live in one allocation, component c at
+//     F + c*fstride (Ex,Ey,Ez,Hx,Hy,Hz).  Each is nz planes x ny rows x px
+//     columns, x fastest (px = nx rounded up to 64 elements).  The
+//     reference's C order has z fastest (lib/field/field.h:54); x-fastest
+//     lets a wavefront read 64 consecutive x cells while the kernels march
+//     along z.
+//   * A component's staggered axis uses n-1 entries (lib/field/vfield.cc:75-84);
+//     the unused tail stays zero.
 //   * H is stored scaled, Hs = eta0*H, and lengths are measured in dx, so every
-//     update coefficient is O(1) (see lx_api.cpp for the derivation).
-//   * Materials: one uint8 id per node, all E components at a storage index
-//     share it; cb[id] = c*dt/(dx*eps_r) lives in a 256-entry table.
+//     update coefficient is O(1) (lx_api.cpp has the derivation).
+//   * Materials: one uint8 id per node shared by the E components at that
+//     storage index; cb[id] = c*dt/(dx*eps_r) lives in a 256-entry table.
 //   * CPML psi lives only in the boundary slabs: per axis a, the cells with
-//     index < slo[a] or >= shi[a], with four FP32 arrays per axis
-//     (two H and two E components that carry a derivative along a).
+//     index < slo[a] or >= shi[a]; four FP32 arrays per axis (psi of the two H
+//     and the two E components that carry a derivative along a).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
@@ -21,26 +23,36 @@
 
 namespace lx {
 
-constexpr int kBX = 64;   // threads along x per block (one wavefront)
-constexpr int kBY = 4;    // rows per block
+constexpr int kBX = 64;      // threads along x per block (one wavefront)
+constexpr int kBY = 4;       // rows per block
+constexpr int kMaxTf = 12;   // TF/SF correction boxes per half step
+
+// Coefficient tables: tab[(a*6 + kind)*tn + i], kind:
+//   0 kh = ch*(dx/d_a)/kappa_h   1 bh = b_h   2 chh = ch*c_h*(dx/d_a)   (H update)
+//   3 ke = (dx/d_a)/kappa_e      4 be = b_e   5 ce = c_e*(dx/d_a)       (E update)
+enum { TK_KH = 0, TK_BH = 1, TK_CH = 2, TK_KE = 3, TK_BE = 4, TK_CE = 5 };
 
 // One TF/SF correction box: the cells (of one component, one split curl, one
 // face of the TF box) whose curl difference gets  dir * g(node)  added.
 struct TfBox {
-    int lo[3];       // inclusive index box; empty when lo[0] > hi[0]
+    int lo[3];       // inclusive index box
     int hi[3];
     float off[3];    // node coordinate = index + off (cells)
-    double dir;      // signed direction factor in device units (0: inactive)
+    int comp;        // target component 0..2 (of E or H)
+    int curl;        // 0: first split curl, 1: second
+    int fa;          // axis normal to the face
+    int start;       // first flattened cell index of this box
+    double dir;      // signed direction factor in device units
 };
 
-// Boxes of one half step: [component][curl 0/1][face lo/hi].
-struct TfSet {
-    TfBox box[3][2][2];
-    int any;
+struct TfList {
+    int n;
+    int total;             // face cells over all boxes
+    TfBox b[kMaxTf];       // host copy
+    const TfBox *dev;      // device copy read by the kernels
 };
 
-// Incident-field provider (see lx_api.cpp: analytic = reference Gaussian,
-// aux = FP64 1D line updated on the device).
+// Incident-field provider.
 struct IncParams {
     int mode;            // LX_INC_ANALYTIC / LX_INC_AUX1D
     int planar;
@@ -53,59 +65,65 @@ struct IncParams {
     const double *h;     // line eta0*H at half nodes [L-1]
 };
 
-template <typename T>
-struct Fields {
-    T *E[3];
-    T *H[3];
+// FP64 aux-1D incident line, advanced by an extra block of the bulk kernels.
+struct AuxLine {
+    double *e, *h;
+    int L;
+    double S;            // c*dt/d_axis
+    int wave;
+    double c, dt, lambda0, fwhm, delay;
 };
 
-// Coefficient tables per axis (length n_a, FP32, computed in FP64 on host):
-//   kh = ch*(dx/d_a)/kappa_h, bh = b_h, chh = ch*c_h*(dx/d_a)   (H update)
-//   ke = (dx/d_a)/kappa_e,    be = b_e, ce = c_e*(dx/d_a)       (E update)
-struct Tables {
-    const float *kh[3], *bh[3], *chh[3];
-    const float *ke[3], *be[3], *ce[3];
-};
-
-struct Slabs {
-    int lo[3], hi[3], w[3];   // slab [0,lo) U [hi,n); thickness w = lo + n - hi
-    float *psiH[3][2];        // [axis][0: curl1 of comp (a+2)%3, 1: curl2 of comp (a+1)%3]
-    float *psiE[3][2];
+// Probes: E probes are recorded by the H launch of the next step (E is final
+// and read-only then), H probes by the E launch of the same step; the last
+// step's E probes by the sequence-end kernel.
+struct ProbeArgs {
+    int n;
+    const int *comp;         // device [n]
+    const long long *idx;    // device [n], linear index into the component
+    double *rec;             // device [cap * n]
+    int cap;
+    double hscale;           // 1/eta0
 };
 
 template <typename T>
 struct Grid {
     int nx, ny, nz, px;
     long long plane;          // px * ny
-    int kz;                   // z cells per block
-    Fields<T> f;
+    long long fstride;        // elements between components
+    T *F;
     const uint8_t *mat;
     const float *cb;          // [256]
-    Tables tab;
-    Slabs slab;
+    const float *tab;
+    int tn;                   // table row length
+    float *psi;
+    long long scells[3];      // psi cells per array on each axis
+    int slo[3], shi[3], sw[3];
     int pml;
-    int *counter;             // ping-pong step counter [2]
+    int kz;                   // z planes per block (3D) / rows per block (2D)
+    int ntx, nty, ntz, ntiles;
+    int *counter;             // counter[0] = steps completed before this launch sequence
 };
 
-// Launchers (lx_kernels.hip).
-template <typename T>
-void launch_h3(const Grid<T> &g, const TfSet &tf, const IncParams &inc, hipStream_t s);
-template <typename T>
-void launch_e3(const Grid<T> &g, const TfSet &tf, const IncParams &inc, hipStream_t s);
-template <typename T>
-void launch_h2(const Grid<T> &g, const TfSet &tf, const IncParams &inc, hipStream_t s);
-template <typename T>
-void launch_e2(const Grid<T> &g, const TfSet &tf, const IncParams &inc, hipStream_t s);
-
-struct AuxLine {
-    double *e, *h;
-    int L;
-    double S;             // c*dt/d_axis
-    int wave;
-    double c, dt, lambda0, fwhm, delay;
+struct Launch {
+    int toff;                 // step offset of this launch from counter[0]
+    int aux;                  // advance the aux line in the extra block
+    int probes;               // record probes in the extra block
 };
-void launch_aux_h(const AuxLine &a, hipStream_t s);
-void launch_aux_e(const AuxLine &a, const int *counter, hipStream_t s);
+
+// Bulk half-step kernel: half 0 = H update, 1 = E update.
+template <typename T>
+void launch_bulk(const Grid<T> &g, int planar, int half, hipStream_t s);
+
+// TF/SF face corrections of one half step (+ aux line + probes in an extra block).
+template <typename T>
+void launch_tfsf(const Grid<T> &g, const TfList &tf, const IncParams &inc, const AuxLine &aux, const ProbeArgs &pr,
+                 Launch l, int half, hipStream_t s);
+
+// End of a launch sequence of n steps: records the E probes of the last step
+// and advances counter[0] by n.
+template <typename T>
+void launch_finish(const Grid<T> &g, const ProbeArgs &pr, int n, hipStream_t s);
 
 struct PointSrc {
     int kind, comp, wave;
@@ -115,26 +133,14 @@ struct PointSrc {
     double dt, c, lambda0, fwhm, delay;
 };
 template <typename T>
-void launch_point_source(const Grid<T> &g, const PointSrc &ps, hipStream_t s);
-
-struct ProbeSet {
-    int n;
-    const long long *idx;   // device linear index per probe
-    const int *comp;        // component per probe
-    double *rec;            // [cap * n]
-    int cap;
-    double hscale;          // 1/eta0
-};
-template <typename T>
-void launch_probes(const Grid<T> &g, const ProbeSet &p, hipStream_t s);
+void launch_point_source(const Grid<T> &g, const PointSrc &ps, int toff, hipStream_t s);
 
 // Gather one component into FP64 reference layout (C order, z fastest).
 template <typename T>
-void launch_gather(const Grid<T> &g, int comp, int ex, int ey, int ez, double scale, double *dst,
-                   int planar, hipStream_t s);
+void launch_gather(const Grid<T> &g, int comp, int ex, int ey, int ez, double scale, double *dst, hipStream_t s);
 
-// FP64 monitor: out[0] = max |E|^2 (reference index range), out[1] = energy sum,
-// out[2] = non-finite count.
+// FP64 monitor: out[0] = max |E|^2 (reference index range), out[1] = sum of
+// E^2 + Hs^2, out[2] = non-finite count.
 template <typename T>
 void launch_monitor(const Grid<T> &g, int planar, double *out, hipStream_t s);
 

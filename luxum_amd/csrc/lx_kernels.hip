@@ -1,344 +1,118 @@
 // lx_kernels.hip -- gfx950 kernels of the Yee timestep.
 //
-// One launch per half step.  Each thread owns one (i, j) column of the
-// x-fastest device layout and marches along z over kz planes, keeping the
-// plane above (H update) or below (E update) in registers, so the z
-// neighbour is loaded once per cell.  A thread updates all three components
-// at its node: the split curl differences (lib/field/vfield.cc:196-275), the
-// TF/SF corrections of that half step (src/solvers/maxwell.cc:512-812), the
-// CPML psi recursion on the slab cells (maxwell_three_dim_pml.cc:182-255) and
-// the field update, with no curl temporaries in memory.
+// Bulk half-step kernels (one launch per half step).  A block of 256 threads
+// (4 wavefronts) owns a 256 (x) x 4 (y) tile: each lane holds 4 consecutive x
+// cells (one 16-byte load per component), each wavefront one y row.  The
+// block marches along z over kz planes; every lane keeps the plane above (H
+// update) or below (E update) in registers, so the z neighbour is read once.
+// x neighbours come from the adjacent lane by a cross-lane shuffle, y
+// neighbours from the adjacent row through the cache.  A lane updates the
+// three components of its cells: split-curl differences
+// (lib/field/vfield.cc:196-275), the CPML psi recursion on slab cells
+// (maxwell_three_dim_pml.cc:182-255) and the field update, with no curl
+// temporaries in memory.
+//
+// TF/SF (src/solvers/maxwell.cc:512-812) is applied by a second, small launch
+// over the face cells only: the corrections are linear in the curl, so adding
+// dir*g to a curl difference after the bulk update equals adding it before.
+// One extra block of that launch advances the FP64 aux-1D incident line and
+// records probes.
+//
+// Block ids are remapped so that each of the 8 XCDs (which receive block ids
+// round robin) works on a contiguous range of tiles: tiles adjacent in y and z
+// share their halo rows and planes in that XCD's L2.
 #include "lx_internal.hpp"
 #include "../../include/luxum.h"
 
 namespace lx {
 
+// ---------------------------------------------------------------------------
+// loads and stores
+// ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ float ld(const T *p, long long i);
+__device__ __forceinline__ float ld1(const T *p);
 template <>
-__device__ __forceinline__ float ld<float>(const float *p, long long i) { return p[i]; }
+__device__ __forceinline__ float ld1<float>(const float *p) { return *p; }
 template <>
-__device__ __forceinline__ float ld<__half>(const __half *p, long long i) { return __half2float(p[i]); }
+__device__ __forceinline__ float ld1<__half>(const __half *p) { return __half2float(*p); }
 
 template <typename T>
-__device__ __forceinline__ void st(T *p, long long i, float v);
+__device__ __forceinline__ void st1(T *p, float v);
 template <>
-__device__ __forceinline__ void st<float>(float *p, long long i, float v) { p[i] = v; }
+__device__ __forceinline__ void st1<float>(float *p, float v) { *p = v; }
 template <>
-__device__ __forceinline__ void st<__half>(__half *p, long long i, float v) { p[i] = __float2half(v); }
+__device__ __forceinline__ void st1<__half>(__half *p, float v) { *p = __float2half(v); }
 
-__device__ __forceinline__ bool in_box(const TfBox &b, int i, int j, int k) {
-    return i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] && k >= b.lo[2] && k <= b.hi[2];
+template <typename T>
+__device__ __forceinline__ float4 ld4(const T *p);
+template <>
+__device__ __forceinline__ float4 ld4<float>(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+template <>
+__device__ __forceinline__ float4 ld4<__half>(const __half *p) {
+    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+    const __half2 a = *reinterpret_cast<const __half2 *>(&u.x), b = *reinterpret_cast<const __half2 *>(&u.y);
+    const float2 fa = __half22float2(a), fb = __half22float2(b);
+    return make_float4(fa.x, fa.y, fb.x, fb.y);
 }
 
-// Incident scalar at node coordinates (cells) and time level tp (steps).
-__device__ double inc_value(const IncParams &in, double cx, double cy, double cz, double tp, bool e_node) {
-    if (in.mode == LX_INC_AUX1D) {
-        double cc = in.axis == 0 ? cx : (in.axis == 1 ? cy : cz);
-        int m = (int)floor(cc) - in.x0;
-        if (m < 0 || m >= in.L) return 0.0;
-        if (e_node) return in.e[m];
-        return m < in.L - 1 ? in.h[m] : 0.0;
-    }
-    double pos = in.planar ? (cx * in.dx * in.kx + cz * in.dz * in.kz)
-                           : (cx * in.dx * in.kx + cy * in.dy * in.ky + cz * in.dz * in.kz);
-    double arg = pos + in.init_dx - tp * in.cdt;
-    return exp(-(arg * arg) * in.inv_sigma);
+template <typename T>
+__device__ __forceinline__ void st4(T *p, float4 v);
+template <>
+__device__ __forceinline__ void st4<float>(float *p, float4 v) { *reinterpret_cast<float4 *>(p) = v; }
+template <>
+__device__ __forceinline__ void st4<__half>(__half *p, float4 v) {
+    const __half2 a = __floats2half2_rn(v.x, v.y), b = __floats2half2_rn(v.z, v.w);
+    uint2 u;
+    u.x = *reinterpret_cast<const unsigned *>(&a);
+    u.y = *reinterpret_cast<const unsigned *>(&b);
+    *reinterpret_cast<uint2 *>(p) = u;
 }
 
-// Sum of the corrections of one split curl at (i,j,k).
-__device__ __forceinline__ float tf_delta(const TfBox *b2, const IncParams &inc, int i, int j, int k, double tp,
-                                          bool e_node) {
-    float d = 0.f;
+__device__ __forceinline__ float4 f4(float v) { return make_float4(v, v, v, v); }
+__device__ __forceinline__ float4 operator-(float4 a, float4 b) { return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
+__device__ __forceinline__ float4 operator+(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+__device__ __forceinline__ float4 operator*(float4 a, float4 b) { return make_float4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w); }
+__device__ __forceinline__ float4 operator*(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
+__device__ __forceinline__ float get(const float4 &v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
+__device__ __forceinline__ void set(float4 &v, int e, float s) {
+    if (e == 0) v.x = s; else if (e == 1) v.y = s; else if (e == 2) v.z = s; else v.w = s;
+}
+// keep the old value where the cell is outside the component's update range
+__device__ __forceinline__ float4 sel(unsigned m, float4 nw, float4 od) {
+    return make_float4((m & 1) ? nw.x : od.x, (m & 2) ? nw.y : od.y, (m & 4) ? nw.z : od.z, (m & 8) ? nw.w : od.w);
+}
+// 4-bit mask of elements e (x = i0+e) with lo <= x <= hi
+__device__ __forceinline__ unsigned xmask(int i0, int lo, int hi) {
+    unsigned m = 0;
 #pragma unroll
-    for (int f = 0; f < 2; ++f) {
-        const TfBox &b = b2[f];
-        if (in_box(b, i, j, k)) {
-            double g = inc_value(inc, i + (double)b.off[0], j + (double)b.off[1], k + (double)b.off[2], tp, e_node);
-            d += (float)(b.dir * g);
-        }
-    }
-    return d;
+    for (int e = 0; e < 4; ++e) m |= (i0 + e >= lo && i0 + e <= hi) ? (1u << e) : 0u;
+    return m;
+}
+
+// Hardware sends block b to XCD b % 8; give XCD x a contiguous tile range.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    const int q = n >> 3, r = n & 7;
+    const int x = b & 7, l = b >> 3;
+    return x * q + min(x, r) + l;
 }
 
 __device__ __forceinline__ int slab_index(int x, int lo, int hi) { return x < lo ? x : lo + (x - hi); }
 
-// ---------------------------------------------------------------------------
-// 3D H update: Hs -= (d1*kh_a1 - d2*kh_a2) + (psi1 - psi2)
-// comp c: curl1 along (c+1)%3, curl2 along (c+2)%3.
-// ---------------------------------------------------------------------------
+// psi arrays: axis a holds [H slot0, H slot1, E slot0, E slot1]; slot w of
+// axis a belongs to split curl w of component (a + 2 - w) % 3.
 template <typename T>
-__global__ __launch_bounds__(kBX *kBY) void k_h3(Grid<T> g, TfSet tf, IncParams inc) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int t = g.counter[0];
-    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) g.counter[1] = t;
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    if (i >= nx || j >= ny) return;
-    const int k0 = blockIdx.z * g.kz;
-    const int k1 = min(k0 + g.kz, nz);
-    const int ip = min(i + 1, nx - 1), jp = min(j + 1, ny - 1);
-    const bool vhx = j < ny - 1;
-    const bool vhy = i < nx - 1;
-    const bool vhz = (i < nx - 1) && (j < ny - 1);
-    const long long px = g.px, plane = g.plane;
-    const T *Ex = g.f.E[0], *Ey = g.f.E[1], *Ez = g.f.E[2];
-    T *Hx = g.f.H[0], *Hy = g.f.H[1], *Hz = g.f.H[2];
-    const double tp = t - 0.5;
-
-    const int ix = min(i, nx - 2 < 0 ? 0 : nx - 2), jy = min(j, ny - 2 < 0 ? 0 : ny - 2);
-    const float khx = g.tab.kh[0][ix], khy = g.tab.kh[1][jy];
-    const bool sx = g.pml && (i < g.slab.lo[0] || i >= g.slab.hi[0]);
-    const bool sy = g.pml && (j < g.slab.lo[1] || j >= g.slab.hi[1]);
-    const int six = slab_index(i, g.slab.lo[0], g.slab.hi[0]);
-    const int siy = slab_index(j, g.slab.lo[1], g.slab.hi[1]);
-    const float bhx = sx ? g.tab.bh[0][ix] : 0.f, chx = sx ? g.tab.chh[0][ix] : 0.f;
-    const float bhy = sy ? g.tab.bh[1][jy] : 0.f, chy = sy ? g.tab.chh[1][jy] : 0.f;
-
-    long long b = (long long)k0 * plane + (long long)j * px;
-    float ex0 = ld(Ex, b + i), ey0 = ld(Ey, b + i), ez0 = ld(Ez, b + i);
-    for (int k = k0; k < k1; ++k, b += plane) {
-        const long long bn = (k + 1 < nz) ? b + plane : b;
-        const float ex1 = ld(Ex, bn + i), ey1 = ld(Ey, bn + i), ez1 = ld(Ez, bn + i);
-        const long long bj = (long long)k * plane + (long long)jp * px;
-        const float ezj = ld(Ez, bj + i), exj = ld(Ex, bj + i);
-        const float ezi = ld(Ez, b + ip), eyi = ld(Ey, b + ip);
-        const int kk = min(k, nz - 2 < 0 ? 0 : nz - 2);
-        const float khz = g.tab.kh[2][kk];
-        const bool sz = g.pml && (k < g.slab.lo[2] || k >= g.slab.hi[2]);
-        const int siz = slab_index(k, g.slab.lo[2], g.slab.hi[2]);
-        const float bhz = sz ? g.tab.bh[2][kk] : 0.f, chz = sz ? g.tab.chh[2][kk] : 0.f;
-        const long long ixs = ((long long)k * ny + j) * g.slab.w[0] + six;   // x slab
-        const long long iys = ((long long)k * g.slab.w[1] + siy) * nx + i;   // y slab
-        const long long izs = ((long long)siz * ny + j) * nx + i;            // z slab
-
-        // Hx: d1 = dEz/dy (axis y), d2 = dEy/dz (axis z)
-        if (vhx && k < nz - 1) {
-            float d1 = ezj - ez0, d2 = ey1 - ey0;
-            if (tf.any) {
-                d1 += tf_delta(tf.box[0][0], inc, i, j, k, tp, true);
-                d2 += tf_delta(tf.box[0][1], inc, i, j, k, tp, true);
-            }
-            float acc = d1 * khy - d2 * khz, pp = 0.f;
-            if (sy) { float *q = g.slab.psiH[1][0] + iys; float v = bhy * *q + chy * d1; *q = v; pp += v; }
-            if (sz) { float *q = g.slab.psiH[2][1] + izs; float v = bhz * *q + chz * d2; *q = v; pp -= v; }
-            const long long o = b + i;
-            st(Hx, o, ld(Hx, o) - acc - pp);
-        }
-        // Hy: d1 = dEx/dz (axis z), d2 = dEz/dx (axis x)
-        if (vhy && k < nz - 1) {
-            float d1 = ex1 - ex0, d2 = ezi - ez0;
-            if (tf.any) {
-                d1 += tf_delta(tf.box[1][0], inc, i, j, k, tp, true);
-                d2 += tf_delta(tf.box[1][1], inc, i, j, k, tp, true);
-            }
-            float acc = d1 * khz - d2 * khx, pp = 0.f;
-            if (sz) { float *q = g.slab.psiH[2][0] + izs; float v = bhz * *q + chz * d1; *q = v; pp += v; }
-            if (sx) { float *q = g.slab.psiH[0][1] + ixs; float v = bhx * *q + chx * d2; *q = v; pp -= v; }
-            const long long o = b + i;
-            st(Hy, o, ld(Hy, o) - acc - pp);
-        }
-        // Hz: d1 = dEy/dx (axis x), d2 = dEx/dy (axis y)
-        if (vhz) {
-            float d1 = eyi - ey0, d2 = exj - ex0;
-            if (tf.any) {
-                d1 += tf_delta(tf.box[2][0], inc, i, j, k, tp, true);
-                d2 += tf_delta(tf.box[2][1], inc, i, j, k, tp, true);
-            }
-            float acc = d1 * khx - d2 * khy, pp = 0.f;
-            if (sx) { float *q = g.slab.psiH[0][0] + ixs; float v = bhx * *q + chx * d1; *q = v; pp += v; }
-            if (sy) { float *q = g.slab.psiH[1][1] + iys; float v = bhy * *q + chy * d2; *q = v; pp -= v; }
-            const long long o = b + i;
-            st(Hz, o, ld(Hz, o) - acc - pp);
-        }
-        ex0 = ex1; ey0 = ey1; ez0 = ez1;
-    }
+__device__ __forceinline__ float *psi_base(const Grid<T> &g, int a, int type_slot) {
+    long long base = a == 0 ? 0 : (a == 1 ? 4 * g.scells[0] : 4 * (g.scells[0] + g.scells[1]));
+    return g.psi + base + (long long)type_slot * g.scells[a];
+}
+// psi cell offset inside one array of axis a
+template <typename T>
+__device__ __forceinline__ long long psi_off(const Grid<T> &g, int a, int i, int j, int k) {
+    if (a == 0) return ((long long)k * g.ny + j) * g.sw[0] + slab_index(i, g.slo[0], g.shi[0]);
+    if (a == 1) return ((long long)k * g.sw[1] + slab_index(j, g.slo[1], g.shi[1])) * g.px + i;
+    return ((long long)slab_index(k, g.slo[2], g.shi[2]) * g.ny + j) * g.px + i;
 }
 
-// ---------------------------------------------------------------------------
-// 3D E update: E += cb[id] * ((d1*ke_a1 - d2*ke_a2) + (psi1 - psi2)),
-// interior only (PEC walls come from the curl ranges, vfield.cc:242-268).
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kBX *kBY) void k_e3(Grid<T> g, TfSet tf, IncParams inc) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
-    const int t = g.counter[1];
-    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) g.counter[0] = t + 1;
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    if (i >= nx || j >= ny) return;
-    const int k0 = blockIdx.z * g.kz;
-    const int k1 = min(k0 + g.kz, nz);
-    const int im = max(i - 1, 0), jm = max(j - 1, 0);
-    const bool vex = (i < nx - 1) && (j >= 1) && (j <= ny - 2);
-    const bool vey = (i >= 1) && (i <= nx - 2) && (j < ny - 1);
-    const bool vez = (i >= 1) && (i <= nx - 2) && (j >= 1) && (j <= ny - 2);
-    const long long px = g.px, plane = g.plane;
-    const T *Hx = g.f.H[0], *Hy = g.f.H[1], *Hz = g.f.H[2];
-    T *Ex = g.f.E[0], *Ey = g.f.E[1], *Ez = g.f.E[2];
-    const double tp = (double)t;
-
-    const float kex = g.tab.ke[0][i], key = g.tab.ke[1][j];
-    const bool sx = g.pml && (i < g.slab.lo[0] || i >= g.slab.hi[0]);
-    const bool sy = g.pml && (j < g.slab.lo[1] || j >= g.slab.hi[1]);
-    const int six = slab_index(i, g.slab.lo[0], g.slab.hi[0]);
-    const int siy = slab_index(j, g.slab.lo[1], g.slab.hi[1]);
-    const float bex = sx ? g.tab.be[0][i] : 0.f, cex = sx ? g.tab.ce[0][i] : 0.f;
-    const float bey = sy ? g.tab.be[1][j] : 0.f, cey = sy ? g.tab.ce[1][j] : 0.f;
-
-    long long b = (long long)k0 * plane + (long long)j * px;
-    const long long bprev = k0 > 0 ? b - plane : b;
-    float hxm = ld(Hx, bprev + i), hym = ld(Hy, bprev + i);
-    for (int k = k0; k < k1; ++k, b += plane) {
-        const float hx0 = ld(Hx, b + i), hy0 = ld(Hy, b + i), hz0 = ld(Hz, b + i);
-        const long long bj = (long long)k * plane + (long long)jm * px;
-        const float hzj = ld(Hz, bj + i), hxj = ld(Hx, bj + i);
-        const float hzi = ld(Hz, b + im), hyi = ld(Hy, b + im);
-        const float kez = g.tab.ke[2][k];
-        const bool sz = g.pml && (k < g.slab.lo[2] || k >= g.slab.hi[2]);
-        const int siz = slab_index(k, g.slab.lo[2], g.slab.hi[2]);
-        const float bez = sz ? g.tab.be[2][k] : 0.f, cez = sz ? g.tab.ce[2][k] : 0.f;
-        const long long ixs = ((long long)k * ny + j) * g.slab.w[0] + six;
-        const long long iys = ((long long)k * g.slab.w[1] + siy) * nx + i;
-        const long long izs = ((long long)siz * ny + j) * nx + i;
-        const long long o = b + i;
-        const bool kin = (k >= 1) && (k <= nz - 2);
-        const bool any = (vex && kin) || (vey && kin) || (vez && k < nz - 1);
-        const float cb = any ? g.cb[g.mat[o]] : 0.f;
-
-        // Ex: d1 = dHz/dy (axis y), d2 = dHy/dz (axis z)
-        if (vex && kin) {
-            float d1 = hz0 - hzj, d2 = hy0 - hym;
-            if (tf.any) {
-                d1 += tf_delta(tf.box[0][0], inc, i, j, k, tp, false);
-                d2 += tf_delta(tf.box[0][1], inc, i, j, k, tp, false);
-            }
-            float acc = d1 * key - d2 * kez, pp = 0.f;
-            if (sy) { float *q = g.slab.psiE[1][0] + iys; float v = bey * *q + cey * d1; *q = v; pp += v; }
-            if (sz) { float *q = g.slab.psiE[2][1] + izs; float v = bez * *q + cez * d2; *q = v; pp -= v; }
-            st(Ex, o, ld(Ex, o) + cb * acc + cb * pp);
-        }
-        // Ey: d1 = dHx/dz (axis z), d2 = dHz/dx (axis x)
-        if (vey && kin) {
-            float d1 = hx0 - hxm, d2 = hz0 - hzi;
-            if (tf.any) {
-                d1 += tf_delta(tf.box[1][0], inc, i, j, k, tp, false);
-                d2 += tf_delta(tf.box[1][1], inc, i, j, k, tp, false);
-            }
-            float acc = d1 * kez - d2 * kex, pp = 0.f;
-            if (sz) { float *q = g.slab.psiE[2][0] + izs; float v = bez * *q + cez * d1; *q = v; pp += v; }
-            if (sx) { float *q = g.slab.psiE[0][1] + ixs; float v = bex * *q + cex * d2; *q = v; pp -= v; }
-            st(Ey, o, ld(Ey, o) + cb * acc + cb * pp);
-        }
-        // Ez: d1 = dHy/dx (axis x), d2 = dHx/dy (axis y)
-        if (vez && k < nz - 1) {
-            float d1 = hy0 - hyi, d2 = hx0 - hxj;
-            if (tf.any) {
-                d1 += tf_delta(tf.box[2][0], inc, i, j, k, tp, false);
-                d2 += tf_delta(tf.box[2][1], inc, i, j, k, tp, false);
-            }
-            float acc = d1 * kex - d2 * key, pp = 0.f;
-            if (sx) { float *q = g.slab.psiE[0][0] + ixs; float v = bex * *q + cex * d1; *q = v; pp += v; }
-            if (sy) { float *q = g.slab.psiE[1][1] + iys; float v = bey * *q + cey * d2; *q = v; pp -= v; }
-            st(Ez, o, ld(Ez, o) + cb * acc + cb * pp);
-        }
-        hxm = hx0; hym = hy0;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// 2D (x-z plane): Ey, Hx, Hz (maxwell_two_dim_pml.cc:144-176)
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kBX *kBY) void k_h2(Grid<T> g, TfSet tf, IncParams inc) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int k = blockIdx.y * kBY + threadIdx.y;
-    const int t = g.counter[0];
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) g.counter[1] = t;
-    const int nx = g.nx, nz = g.nz;
-    if (i >= nx || k >= nz) return;
-    const long long px = g.px;
-    const T *Ey = g.f.E[1];
-    T *Hx = g.f.H[0], *Hz = g.f.H[2];
-    const double tp = t - 0.5;
-    const long long o = (long long)k * px + i;
-    const float e0 = ld(Ey, o);
-    // Hx: d2 = dEy/dz (curl2, axis z); Hs_x = Hs_x - (-d2*kh_z) - (0 - psi2)
-    if (k < nz - 1) {
-        float d2 = ld(Ey, o + px) - e0;
-        if (tf.any) d2 += tf_delta(tf.box[0][1], inc, i, 0, k, tp, true);
-        const float khz = g.tab.kh[2][k];
-        float pp = 0.f;
-        if (g.pml && (k < g.slab.lo[2] || k >= g.slab.hi[2])) {
-            const int siz = slab_index(k, g.slab.lo[2], g.slab.hi[2]);
-            float *q = g.slab.psiH[2][1] + (long long)siz * nx + i;
-            float v = g.tab.bh[2][k] * *q + g.tab.chh[2][k] * d2;
-            *q = v;
-            pp = -v;
-        }
-        st(Hx, o, ld(Hx, o) + d2 * khz - pp);
-    }
-    // Hz: d1 = dEy/dx (curl1, axis x)
-    if (i < nx - 1) {
-        float d1 = ld(Ey, o + 1) - e0;
-        if (tf.any) d1 += tf_delta(tf.box[2][0], inc, i, 0, k, tp, true);
-        const float khx = g.tab.kh[0][i];
-        float pp = 0.f;
-        if (g.pml && (i < g.slab.lo[0] || i >= g.slab.hi[0])) {
-            const int six = slab_index(i, g.slab.lo[0], g.slab.hi[0]);
-            float *q = g.slab.psiH[0][0] + (long long)k * g.slab.w[0] + six;
-            float v = g.tab.bh[0][i] * *q + g.tab.chh[0][i] * d1;
-            *q = v;
-            pp = v;
-        }
-        st(Hz, o, ld(Hz, o) - d1 * khx - pp);
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBX *kBY) void k_e2(Grid<T> g, TfSet tf, IncParams inc) {
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int k = blockIdx.y * kBY + threadIdx.y;
-    const int t = g.counter[1];
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) g.counter[0] = t + 1;
-    const int nx = g.nx, nz = g.nz;
-    if (i < 1 || i > nx - 2 || k < 1 || k > nz - 2) return;
-    const long long px = g.px;
-    const T *Hx = g.f.H[0], *Hz = g.f.H[2];
-    T *Ey = g.f.E[1];
-    const double tp = (double)t;
-    const long long o = (long long)k * px + i;
-    // Ey: d1 = dHx/dz (curl1, axis z), d2 = dHz/dx (curl2, axis x)
-    float d1 = ld(Hx, o) - ld(Hx, o - px);
-    float d2 = ld(Hz, o) - ld(Hz, o - 1);
-    if (tf.any) {
-        d1 += tf_delta(tf.box[1][0], inc, i, 0, k, tp, false);
-        d2 += tf_delta(tf.box[1][1], inc, i, 0, k, tp, false);
-    }
-    float acc = d1 * g.tab.ke[2][k] - d2 * g.tab.ke[0][i], pp = 0.f;
-    if (g.pml) {
-        if (k < g.slab.lo[2] || k >= g.slab.hi[2]) {
-            const int siz = slab_index(k, g.slab.lo[2], g.slab.hi[2]);
-            float *q = g.slab.psiE[2][0] + (long long)siz * nx + i;
-            float v = g.tab.be[2][k] * *q + g.tab.ce[2][k] * d1;
-            *q = v;
-            pp += v;
-        }
-        if (i < g.slab.lo[0] || i >= g.slab.hi[0]) {
-            const int six = slab_index(i, g.slab.lo[0], g.slab.hi[0]);
-            float *q = g.slab.psiE[0][1] + (long long)k * g.slab.w[0] + six;
-            float v = g.tab.be[0][i] * *q + g.tab.ce[0][i] * d2;
-            *q = v;
-            pp -= v;
-        }
-    }
-    const float cb = g.cb[g.mat[o]];
-    st(Ey, o, ld(Ey, o) + cb * acc + cb * pp);
-}
-
-// ---------------------------------------------------------------------------
-// aux-1D incident line (FP64), point source, probes
-// ---------------------------------------------------------------------------
 __device__ double waveform(int kind, double t, double c, double lambda0, double fwhm, double delay) {
     const double f0 = c / lambda0, T0 = 1.0 / f0;
     const double tau = fwhm * T0 / (2.0 * sqrt(log(2.0)));
@@ -349,55 +123,552 @@ __device__ double waveform(int kind, double t, double c, double lambda0, double 
     return env * sin(2.0 * M_PI * f0 * (t - t0));
 }
 
-__global__ void k_aux_h(AuxLine a) {
-    for (int m = threadIdx.x; m < a.L - 1; m += blockDim.x) a.h[m] = a.h[m] - a.S * (a.e[m + 1] - a.e[m]);
+// Incident scalar at node coordinates (cells) and time level tp (steps).
+__device__ double inc_value(const IncParams &in, double cx, double cy, double cz, double tp, bool e_node) {
+    if (in.mode == LX_INC_AUX1D) {
+        const double cc = in.axis == 0 ? cx : (in.axis == 1 ? cy : cz);
+        const int m = (int)floor(cc) - in.x0;
+        if (m < 0 || m >= in.L) return 0.0;
+        if (e_node) return in.e[m];
+        return m < in.L - 1 ? in.h[m] : 0.0;
+    }
+    const double pos = in.planar ? (cx * in.dx * in.kx + cz * in.dz * in.kz)
+                                 : (cx * in.dx * in.kx + cy * in.dy * in.ky + cz * in.dz * in.kz);
+    const double arg = pos + in.init_dx - tp * in.cdt;
+    return exp(-(arg * arg) * in.inv_sigma);
 }
 
-__global__ void k_aux_e(AuxLine a, const int *counter) {
-    const int tn = counter[0];   // = t + 1 after the E update of step t
-    for (int m = 1 + threadIdx.x; m < a.L - 1; m += blockDim.x) a.e[m] = a.e[m] - a.S * (a.h[m] - a.h[m - 1]);
-    if (threadIdx.x == 0) {
-        a.e[0] = waveform(a.wave, tn * a.dt, a.c, a.lambda0, a.fwhm, a.delay);
-        a.e[a.L - 1] = 0.0;
+// ---------------------------------------------------------------------------
+// 3D H update:  Hs -= (d1*kh_a1 - d2*kh_a2) + (psi1 - psi2)
+// component c: first split curl along (c+1)%3, second along (c+2)%3.
+// ---------------------------------------------------------------------------
+template <typename T, bool PML>
+__global__ __launch_bounds__(256) void k_h3(Grid<T> g) {
+    const int tile = xcd_remap(blockIdx.x, g.ntiles);
+    const int tx = tile % g.ntx, rr = tile / g.ntx, ty = rr % g.nty, tz = rr / g.nty;
+    const int lane = threadIdx.x & 63;
+    const int i0 = tx * 256 + lane * 4;
+    const int j = ty * kBY + (threadIdx.x >> 6);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (j >= ny) return;                       // whole wavefront
+    const bool on = i0 < nx;
+    const int k0 = tz * g.kz;
+    const int k1 = min(k0 + g.kz, nz);
+    const long long px = g.px, plane = g.plane, fs = g.fstride;
+    const T *Ex = g.F, *Ey = g.F + fs, *Ez = g.F + 2 * fs;
+    T *Hx = g.F + 3 * fs, *Hy = g.F + 4 * fs, *Hz = g.F + 5 * fs;
+    const int tn = g.tn;
+    const float *tab = g.tab;
+    const int jp = min(j + 1, ny - 1);
+    const bool last_lane = (lane == 63) || (i0 + 4 >= nx);
+    const bool nxt = i0 + 4 < nx;
+    // element masks
+    const unsigned mx_all = on ? xmask(i0, 0, nx - 1) : 0u;      // Hx: i < nx
+    const unsigned mx_in = on ? xmask(i0, 0, nx - 2) : 0u;       // Hy, Hz: i < nx-1
+    const bool rowx = j < ny - 1;
+    const float4 khx = on ? *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_KH) * tn + i0) : f4(0.f);
+    const float khy = tab[(1 * 6 + TK_KH) * tn + min(j, ny - 2)];
+    // CPML (slabs)
+    const bool sy = PML && (j < g.slo[1] || j >= g.shi[1]);
+    const float bhy = sy ? tab[(1 * 6 + TK_BH) * tn + j] : 0.f, chy = sy ? tab[(1 * 6 + TK_CH) * tn + j] : 0.f;
+    const bool anyx = PML && on && (i0 < g.slo[0] || i0 + 3 >= g.shi[0]);
+    const unsigned msx = anyx ? (xmask(i0, 0, g.slo[0] - 1) | xmask(i0, g.shi[0], nx - 1)) & mx_all : 0u;
+    float *psy0 = PML ? psi_base(g, 1, 0) : nullptr, *psy1 = PML ? psi_base(g, 1, 1) : nullptr;
+    float *psz0 = PML ? psi_base(g, 2, 0) : nullptr, *psz1 = PML ? psi_base(g, 2, 1) : nullptr;
+    float *psx0 = PML ? psi_base(g, 0, 0) : nullptr, *psx1 = PML ? psi_base(g, 0, 1) : nullptr;
+
+    long long o = (long long)k0 * plane + (long long)j * px + i0;
+    const long long dj = (long long)(jp - j) * px;
+    float4 ex0 = f4(0.f), ey0 = f4(0.f), ez0 = f4(0.f);
+    if (on) { ex0 = ld4(Ex + o); ey0 = ld4(Ey + o); ez0 = ld4(Ez + o); }
+    for (int k = k0; k < k1; ++k, o += plane) {
+        const bool kin = k < nz - 1;
+        float4 ex1 = f4(0.f), ey1 = f4(0.f), ez1 = f4(0.f);
+        if (on && kin) { ex1 = ld4(Ex + o + plane); ey1 = ld4(Ey + o + plane); ez1 = ld4(Ez + o + plane); }
+        // x+1 neighbours of element 3 from the next lane (or memory at the tile edge)
+        float ezn = __shfl_down(ez0.x, 1), eyn = __shfl_down(ey0.x, 1);
+        if (last_lane) {
+            ezn = nxt ? ld1(Ez + o + 4) : 0.f;
+            eyn = nxt ? ld1(Ey + o + 4) : 0.f;
+        }
+        const int kk = min(k, nz - 2);
+        const float khz = tab[(2 * 6 + TK_KH) * tn + kk];
+        const bool sz = PML && (k < g.slo[2] || k >= g.shi[2]);
+        const long long zoff = PML && sz ? ((long long)slab_index(k, g.slo[2], g.shi[2]) * ny + j) * px + i0 : 0;
+        const long long yoff = PML && sy ? ((long long)k * g.sw[1] + slab_index(j, g.slo[1], g.shi[1])) * px + i0 : 0;
+        const float bhz = sz ? tab[(2 * 6 + TK_BH) * tn + k] : 0.f, chz = sz ? tab[(2 * 6 + TK_CH) * tn + k] : 0.f;
+        if (on) {
+            // Hx: d1 = dEz/dy (y, psi y slot0), d2 = dEy/dz (z, psi z slot1)
+            if (rowx && kin) {
+                const float4 ezj = ld4(Ez + o + dj);
+                const float4 d1 = ezj - ez0, d2 = ey1 - ey0;
+                float4 acc = d1 * khy - d2 * khz;
+                if (sy) {
+                    float4 p = ld4(psy0 + yoff);
+                    p = p * bhy + d1 * chy;
+                    st4(psy0 + yoff, p);
+                    acc = acc + p;
+                }
+                if (sz) {
+                    float4 p = ld4(psz1 + zoff);
+                    p = p * bhz + d2 * chz;
+                    st4(psz1 + zoff, p);
+                    acc = acc - p;
+                }
+                const float4 h = ld4(Hx + o);
+                st4(Hx + o, sel(mx_all, h - acc, h));
+            }
+            // Hy: d1 = dEx/dz (z, psi z slot0), d2 = dEz/dx (x, psi x slot1)
+            if (kin) {
+                const float4 ezi = make_float4(ez0.y, ez0.z, ez0.w, ezn);
+                const float4 d1 = ex1 - ex0, d2 = ezi - ez0;
+                float4 acc = d1 * khz - d2 * khx;
+                if (sz) {
+                    float4 p = ld4(psz0 + zoff);
+                    p = p * bhz + d1 * chz;
+                    st4(psz0 + zoff, p);
+                    acc = acc + p;
+                }
+                if (msx) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (msx & (1u << e)) {
+                            float *q = psx1 + psi_off(g, 0, i0 + e, j, k);
+                            const float v = tab[(0 * 6 + TK_BH) * tn + i0 + e] * *q + tab[(0 * 6 + TK_CH) * tn + i0 + e] * get(d2, e);
+                            *q = v;
+                            set(acc, e, get(acc, e) - v);
+                        }
+                }
+                const float4 h = ld4(Hy + o);
+                st4(Hy + o, sel(mx_in, h - acc, h));
+            }
+            // Hz: d1 = dEy/dx (x, psi x slot0), d2 = dEx/dy (y, psi y slot1)
+            if (rowx) {
+                const float4 eyi = make_float4(ey0.y, ey0.z, ey0.w, eyn);
+                const float4 exj = ld4(Ex + o + dj);
+                const float4 d1 = eyi - ey0, d2 = exj - ex0;
+                float4 acc = d1 * khx - d2 * khy;
+                if (msx) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (msx & (1u << e)) {
+                            float *q = psx0 + psi_off(g, 0, i0 + e, j, k);
+                            const float v = tab[(0 * 6 + TK_BH) * tn + i0 + e] * *q + tab[(0 * 6 + TK_CH) * tn + i0 + e] * get(d1, e);
+                            *q = v;
+                            set(acc, e, get(acc, e) + v);
+                        }
+                }
+                if (sy) {
+                    float4 p = ld4(psy1 + yoff);
+                    p = p * bhy + d2 * chy;
+                    st4(psy1 + yoff, p);
+                    acc = acc - p;
+                }
+                const float4 h = ld4(Hz + o);
+                st4(Hz + o, sel(mx_in, h - acc, h));
+            }
+        }
+        ex0 = ex1; ey0 = ey1; ez0 = ez1;
     }
 }
 
+// ---------------------------------------------------------------------------
+// 3D E update:  E += cb[id] * ((d1*ke_a1 - d2*ke_a2) + (psi1 - psi2)),
+// interior only (PEC walls come from the curl ranges, vfield.cc:242-268).
+// ---------------------------------------------------------------------------
+template <typename T, bool PML>
+__global__ __launch_bounds__(256) void k_e3(Grid<T> g) {
+    __shared__ float cbs[256];
+    cbs[threadIdx.x] = g.cb[threadIdx.x];
+    __syncthreads();
+    const int tile = xcd_remap(blockIdx.x, g.ntiles);
+    const int tx = tile % g.ntx, rr = tile / g.ntx, ty = rr % g.nty, tz = rr / g.nty;
+    const int lane = threadIdx.x & 63;
+    const int i0 = tx * 256 + lane * 4;
+    const int j = ty * kBY + (threadIdx.x >> 6);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (j >= ny) return;
+    const bool on = i0 < nx;
+    const int k0 = tz * g.kz;
+    const int k1 = min(k0 + g.kz, nz);
+    const long long px = g.px, plane = g.plane, fs = g.fstride;
+    T *Ex = g.F, *Ey = g.F + fs, *Ez = g.F + 2 * fs;
+    const T *Hx = g.F + 3 * fs, *Hy = g.F + 4 * fs, *Hz = g.F + 5 * fs;
+    const int tn = g.tn;
+    const float *tab = g.tab;
+    const int jm = max(j - 1, 0);
+    const bool first_lane = (lane == 0);
+    const bool prv = i0 >= 1;
+    const bool jin = (j >= 1) && (j <= ny - 2);
+    const unsigned m_ex = on && jin ? xmask(i0, 0, nx - 2) : 0u;            // Ex: i < nx-1, 1 <= j <= ny-2
+    const unsigned m_ey = on && (j < ny - 1) ? xmask(i0, 1, nx - 2) : 0u;   // Ey: 1 <= i <= nx-2, j < ny-1
+    const unsigned m_ez = on && jin ? xmask(i0, 1, nx - 2) : 0u;            // Ez: 1 <= i <= nx-2, 1 <= j <= ny-2
+    if (!__any(m_ex | m_ey | m_ez)) return;
+    const float4 kex = on ? *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_KE) * tn + i0) : f4(0.f);
+    const float key = tab[(1 * 6 + TK_KE) * tn + j];
+    const bool sy = PML && (j < g.slo[1] || j >= g.shi[1]);
+    const float bey = sy ? tab[(1 * 6 + TK_BE) * tn + j] : 0.f, cey = sy ? tab[(1 * 6 + TK_CE) * tn + j] : 0.f;
+    const bool anyx = PML && on && (i0 < g.slo[0] || i0 + 3 >= g.shi[0]);
+    const unsigned msx = anyx ? xmask(i0, 0, g.slo[0] - 1) | xmask(i0, g.shi[0], nx - 1) : 0u;
+    float *psy0 = PML ? psi_base(g, 1, 2) : nullptr, *psy1 = PML ? psi_base(g, 1, 3) : nullptr;
+    float *psz0 = PML ? psi_base(g, 2, 2) : nullptr, *psz1 = PML ? psi_base(g, 2, 3) : nullptr;
+    float *psx0 = PML ? psi_base(g, 0, 2) : nullptr, *psx1 = PML ? psi_base(g, 0, 3) : nullptr;
+
+    long long o = (long long)k0 * plane + (long long)j * px + i0;
+    const long long dj = (long long)(j - jm) * px;
+    float4 hxm = f4(0.f), hym = f4(0.f);
+    if (on && k0 > 0) { hxm = ld4(Hx + o - plane); hym = ld4(Hy + o - plane); }
+    for (int k = k0; k < k1; ++k, o += plane) {
+        float4 hx0 = f4(0.f), hy0 = f4(0.f), hz0 = f4(0.f);
+        if (on) { hx0 = ld4(Hx + o); hy0 = ld4(Hy + o); hz0 = ld4(Hz + o); }
+        // x-1 neighbours of element 0 from the previous lane (or memory at the tile edge)
+        float hzp = __shfl_up(hz0.w, 1), hyp = __shfl_up(hy0.w, 1);
+        if (first_lane) {
+            hzp = (on && prv) ? ld1(Hz + o - 1) : 0.f;
+            hyp = (on && prv) ? ld1(Hy + o - 1) : 0.f;
+        }
+        const bool kin = (k >= 1) && (k <= nz - 2);
+        const float kez = tab[(2 * 6 + TK_KE) * tn + k];
+        const bool sz = PML && (k < g.slo[2] || k >= g.shi[2]);
+        const long long zoff = PML && sz ? ((long long)slab_index(k, g.slo[2], g.shi[2]) * ny + j) * px + i0 : 0;
+        const long long yoff = PML && sy ? ((long long)k * g.sw[1] + slab_index(j, g.slo[1], g.shi[1])) * px + i0 : 0;
+        const float bez = sz ? tab[(2 * 6 + TK_BE) * tn + k] : 0.f, cez = sz ? tab[(2 * 6 + TK_CE) * tn + k] : 0.f;
+        if (on) {
+            const uchar4 id = *reinterpret_cast<const uchar4 *>(g.mat + o);
+            const float4 cb = make_float4(cbs[id.x], cbs[id.y], cbs[id.z], cbs[id.w]);
+            // Ex: d1 = dHz/dy (y, psi y slot0), d2 = dHy/dz (z, psi z slot1)
+            if (kin && m_ex) {
+                const float4 hzj = ld4(Hz + o - dj);
+                const float4 d1 = hz0 - hzj, d2 = hy0 - hym;
+                float4 acc = d1 * key - d2 * kez;
+                if (sy) {
+                    float4 p = ld4(psy0 + yoff);
+                    p = p * bey + d1 * cey;
+                    st4(psy0 + yoff, p);
+                    acc = acc + p;
+                }
+                if (sz) {
+                    float4 p = ld4(psz1 + zoff);
+                    p = p * bez + d2 * cez;
+                    st4(psz1 + zoff, p);
+                    acc = acc - p;
+                }
+                const float4 e = ld4(Ex + o);
+                st4(Ex + o, sel(m_ex, e + cb * acc, e));
+            }
+            // Ey: d1 = dHx/dz (z, psi z slot0), d2 = dHz/dx (x, psi x slot1)
+            if (kin && m_ey) {
+                const float4 hzi = make_float4(hzp, hz0.x, hz0.y, hz0.z);
+                const float4 d1 = hx0 - hxm, d2 = hz0 - hzi;
+                float4 acc = d1 * kez - d2 * kex;
+                if (sz) {
+                    float4 p = ld4(psz0 + zoff);
+                    p = p * bez + d1 * cez;
+                    st4(psz0 + zoff, p);
+                    acc = acc + p;
+                }
+                if (msx & m_ey) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (msx & m_ey & (1u << e)) {
+                            float *q = psx1 + psi_off(g, 0, i0 + e, j, k);
+                            const float v = tab[(0 * 6 + TK_BE) * tn + i0 + e] * *q + tab[(0 * 6 + TK_CE) * tn + i0 + e] * get(d2, e);
+                            *q = v;
+                            set(acc, e, get(acc, e) - v);
+                        }
+                }
+                const float4 e = ld4(Ey + o);
+                st4(Ey + o, sel(m_ey, e + cb * acc, e));
+            }
+            // Ez: d1 = dHy/dx (x, psi x slot0), d2 = dHx/dy (y, psi y slot1)
+            if (k < nz - 1 && m_ez) {
+                const float4 hyi = make_float4(hyp, hy0.x, hy0.y, hy0.z);
+                const float4 hxj = ld4(Hx + o - dj);
+                const float4 d1 = hy0 - hyi, d2 = hx0 - hxj;
+                float4 acc = d1 * kex - d2 * key;
+                if (msx & m_ez) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (msx & m_ez & (1u << e)) {
+                            float *q = psx0 + psi_off(g, 0, i0 + e, j, k);
+                            const float v = tab[(0 * 6 + TK_BE) * tn + i0 + e] * *q + tab[(0 * 6 + TK_CE) * tn + i0 + e] * get(d1, e);
+                            *q = v;
+                            set(acc, e, get(acc, e) + v);
+                        }
+                }
+                if (sy) {
+                    float4 p = ld4(psy1 + yoff);
+                    p = p * bey + d2 * cey;
+                    st4(psy1 + yoff, p);
+                    acc = acc - p;
+                }
+                const float4 e = ld4(Ez + o);
+                st4(Ez + o, sel(m_ez, e + cb * acc, e));
+            }
+        }
+        hxm = hx0; hym = hy0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2D (x-z plane): Ey, Hx, Hz (maxwell_two_dim_pml.cc:144-176).  Tiles of
+// 256 (x) x 4 (z) cells, 4 cells per lane, one z row per wavefront.
+// ---------------------------------------------------------------------------
+template <typename T, bool PML>
+__global__ __launch_bounds__(256) void k_h2(Grid<T> g) {
+    const int tile = xcd_remap(blockIdx.x, g.ntiles);
+    const int tx = tile % g.ntx, tz = tile / g.ntx;
+    const int lane = threadIdx.x & 63;
+    const int i0 = tx * 256 + lane * 4;
+    const int k = tz * kBY + (threadIdx.x >> 6);
+    const int nx = g.nx, nz = g.nz;
+    if (k >= nz) return;
+    const bool on = i0 < nx;
+    const long long px = g.px, fs = g.fstride;
+    const T *Ey = g.F + fs;
+    T *Hx = g.F + 3 * fs, *Hz = g.F + 5 * fs;
+    const int tn = g.tn;
+    const float *tab = g.tab;
+    const long long o = (long long)k * px + i0;
+    float4 e0 = f4(0.f);
+    if (on) e0 = ld4(Ey + o);
+    float en = __shfl_down(e0.x, 1);
+    if (lane == 63 || i0 + 4 >= nx) en = (i0 + 4 < nx) ? ld1(Ey + o + 4) : 0.f;
+    if (!on) return;
+    // Hx: second split curl d2 = dEy/dz (z, psi z slot1):  Hs_x -= -d2*kh_z - psi
+    if (k < nz - 1) {
+        const float4 d2 = ld4(Ey + o + px) - e0;
+        float4 acc = d2 * (-tab[(2 * 6 + TK_KH) * tn + k]);
+        if (PML && (k < g.slo[2] || k >= g.shi[2])) {
+            float *q = psi_base(g, 2, 1) + psi_off(g, 2, i0, 0, k);
+            float4 p = ld4(q);
+            p = p * tab[(2 * 6 + TK_BH) * tn + k] + d2 * tab[(2 * 6 + TK_CH) * tn + k];
+            st4(q, p);
+            acc = acc - p;
+        }
+        const float4 h = ld4(Hx + o);
+        st4(Hx + o, sel(xmask(i0, 0, nx - 1), h - acc, h));
+    }
+    // Hz: first split curl d1 = dEy/dx (x, psi x slot0)
+    {
+        const float4 ei = make_float4(e0.y, e0.z, e0.w, en);
+        const float4 d1 = ei - e0;
+        float4 acc = d1 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_KH) * tn + i0);
+        const unsigned mi = xmask(i0, 0, nx - 2);
+        if (PML && (i0 < g.slo[0] || i0 + 3 >= g.shi[0])) {
+            const unsigned ms = (xmask(i0, 0, g.slo[0] - 1) | xmask(i0, g.shi[0], nx - 1)) & mi;
+            float *pb = psi_base(g, 0, 0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (ms & (1u << e)) {
+                    float *q = pb + psi_off(g, 0, i0 + e, 0, k);
+                    const float v = tab[(0 * 6 + TK_BH) * tn + i0 + e] * *q + tab[(0 * 6 + TK_CH) * tn + i0 + e] * get(d1, e);
+                    *q = v;
+                    set(acc, e, get(acc, e) + v);
+                }
+        }
+        const float4 h = ld4(Hz + o);
+        st4(Hz + o, sel(mi, h - acc, h));
+    }
+}
+
+template <typename T, bool PML>
+__global__ __launch_bounds__(256) void k_e2(Grid<T> g) {
+    __shared__ float cbs[256];
+    cbs[threadIdx.x] = g.cb[threadIdx.x];
+    __syncthreads();
+    const int tile = xcd_remap(blockIdx.x, g.ntiles);
+    const int tx = tile % g.ntx, tz = tile / g.ntx;
+    const int lane = threadIdx.x & 63;
+    const int i0 = tx * 256 + lane * 4;
+    const int k = tz * kBY + (threadIdx.x >> 6);
+    const int nx = g.nx, nz = g.nz;
+    if (k < 1 || k > nz - 2) return;
+    const bool on = i0 < nx;
+    const long long px = g.px, fs = g.fstride;
+    const T *Hx = g.F + 3 * fs, *Hz = g.F + 5 * fs;
+    T *Ey = g.F + fs;
+    const int tn = g.tn;
+    const float *tab = g.tab;
+    const long long o = (long long)k * px + i0;
+    float4 hz0 = f4(0.f);
+    if (on) hz0 = ld4(Hz + o);
+    float hp = __shfl_up(hz0.w, 1);
+    if (lane == 0) hp = (on && i0 >= 1) ? ld1(Hz + o - 1) : 0.f;
+    if (!on) return;
+    const unsigned m = xmask(i0, 1, nx - 2);
+    if (!m) return;
+    // Ey: d1 = dHx/dz (z, psi z slot0), d2 = dHz/dx (x, psi x slot1)
+    const float4 d1 = ld4(Hx + o) - ld4(Hx + o - px);
+    const float4 d2 = hz0 - make_float4(hp, hz0.x, hz0.y, hz0.z);
+    float4 acc = d1 * tab[(2 * 6 + TK_KE) * tn + k] - d2 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_KE) * tn + i0);
+    if (PML) {
+        if (k < g.slo[2] || k >= g.shi[2]) {
+            float *q = psi_base(g, 2, 2) + psi_off(g, 2, i0, 0, k);
+            float4 p = ld4(q);
+            p = p * tab[(2 * 6 + TK_BE) * tn + k] + d1 * tab[(2 * 6 + TK_CE) * tn + k];
+            st4(q, p);
+            acc = acc + p;
+        }
+        if (i0 < g.slo[0] || i0 + 3 >= g.shi[0]) {
+            const unsigned ms = (xmask(i0, 0, g.slo[0] - 1) | xmask(i0, g.shi[0], nx - 1)) & m;
+            float *pb = psi_base(g, 0, 3);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (ms & (1u << e)) {
+                    float *q = pb + psi_off(g, 0, i0 + e, 0, k);
+                    const float v = tab[(0 * 6 + TK_BE) * tn + i0 + e] * *q + tab[(0 * 6 + TK_CE) * tn + i0 + e] * get(d2, e);
+                    *q = v;
+                    set(acc, e, get(acc, e) - v);
+                }
+        }
+    }
+    const uchar4 id = *reinterpret_cast<const uchar4 *>(g.mat + o);
+    const float4 cb = make_float4(cbs[id.x], cbs[id.y], cbs[id.z], cbs[id.w]);
+    const float4 e = ld4(Ey + o);
+    st4(Ey + o, sel(m, e + cb * acc, e));
+}
+
+// ---------------------------------------------------------------------------
+// TF/SF face corrections + aux line + probes (one launch per half step)
+// ---------------------------------------------------------------------------
+// H half (IS_H): aux line H phase, E probes of step t-1.
+// E half:        aux line E phase (source sample t+1), H probes of step t.
+template <typename T, bool IS_H>
+__device__ void extra_block(const Grid<T> &g, const AuxLine &aux, const ProbeArgs &pr, Launch l, int t) {
+    if (IS_H) {
+        if (l.aux)
+            for (int m = threadIdx.x; m < aux.L - 1; m += blockDim.x) aux.h[m] = aux.h[m] - aux.S * (aux.e[m + 1] - aux.e[m]);
+        if (l.probes && t >= 1 && t - 1 < pr.cap)
+            for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
+                if (pr.comp[q] < 3)
+                    pr.rec[(long long)(t - 1) * pr.n + q] = (double)ld1(g.F + pr.comp[q] * g.fstride + pr.idx[q]);
+    } else {
+        if (l.aux) {
+            for (int m = 1 + threadIdx.x; m < aux.L - 1; m += blockDim.x)
+                aux.e[m] = aux.e[m] - aux.S * (aux.h[m] - aux.h[m - 1]);
+            if (threadIdx.x == 0) {
+                aux.e[0] = waveform(aux.wave, (t + 1) * aux.dt, aux.c, aux.lambda0, aux.fwhm, aux.delay);
+                aux.e[aux.L - 1] = 0.0;
+            }
+        }
+        if (l.probes && t < pr.cap)
+            for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
+                if (pr.comp[q] >= 3)
+                    pr.rec[(long long)t * pr.n + q] = (double)ld1(g.F + pr.comp[q] * g.fstride + pr.idx[q]) * pr.hscale;
+    }
+}
+
+// One thread per face cell.  A cell of component c gets, for split curl w
+// (axis a = (c+1+w)%3, sign s = +1 for w = 0, -1 for w = 1) and delta = dir*g:
+//   H:  Hs -= s*(kh_a + c'h_a)*delta,     psiH_a[w] += c'h_a*delta
+//   E:  E  += cb*s*(ke_a + c'e_a)*delta,  psiE_a[w] += c'e_a*delta
+// (c' terms only on slab cells), which is exactly the bulk update with the
+// curl difference d replaced by d + delta.  Boxes of one component and one
+// split curl never overlap; a first-curl and a second-curl box of the same E
+// component share the cells on the TF box edges (the reference applies both
+// corrections there), so the first-curl thread applies both and the
+// second-curl thread skips those cells.
+__device__ __forceinline__ bool box_has(const TfBox &b, const int ijk[3]) {
+    return ijk[0] >= b.lo[0] && ijk[0] <= b.hi[0] && ijk[1] >= b.lo[1] && ijk[1] <= b.hi[1] && ijk[2] >= b.lo[2] &&
+           ijk[2] <= b.hi[2];
+}
+
+template <typename T, bool IS_H>
+__device__ __forceinline__ float tf_term(const Grid<T> &g, const TfBox &bx, const IncParams &inc, const int ijk[3],
+                                         double tp) {
+    const double gi = inc_value(inc, ijk[0] + (double)bx.off[0], ijk[1] + (double)bx.off[1], ijk[2] + (double)bx.off[2],
+                                tp, IS_H);
+    const float delta = (float)(bx.dir * gi);
+    const int a = (bx.comp + 1 + bx.curl) % 3;
+    const int ia = ijk[a];
+    const int tn = g.tn;
+    float coef = g.tab[(a * 6 + (IS_H ? TK_KH : TK_KE)) * tn + ia];
+    if (g.pml && (ia < g.slo[a] || ia >= g.shi[a])) {
+        const float cp = g.tab[(a * 6 + (IS_H ? TK_CH : TK_CE)) * tn + ia];
+        float *q = psi_base(g, a, (IS_H ? 0 : 2) + bx.curl) + psi_off(g, a, ijk[0], ijk[1], ijk[2]);
+        *q = *q + cp * delta;
+        coef += cp;
+    }
+    return (bx.curl == 0 ? 1.f : -1.f) * coef * delta;
+}
+
+template <typename T, bool IS_H>
+__global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict__ tb, int ntf, int total,
+                                              IncParams inc, AuxLine aux, ProbeArgs pr, Launch l) {
+    const int t = g.counter[0] + l.toff;
+    const int nb = (total + 255) / 256;
+    if ((int)blockIdx.x >= nb) {
+        extra_block<T, IS_H>(g, aux, pr, l, t);
+        return;
+    }
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    if (gid >= total) return;
+    int b = 0;
+    while (b + 1 < ntf && gid >= tb[b + 1].start) ++b;
+    const TfBox &bx = tb[b];
+    const int fa = bx.fa, u = (fa + 1) % 3, v = (fa + 2) % 3;
+    const int nu = bx.hi[u] - bx.lo[u] + 1;
+    const int loc = gid - bx.start;
+    int ijk[3];
+    ijk[fa] = bx.lo[fa];
+    ijk[u] = bx.lo[u] + loc % nu;
+    ijk[v] = bx.lo[v] + loc / nu;
+    const double tp = IS_H ? t - 0.5 : (double)t;
+    const int c = bx.comp;
+    if (!IS_H && bx.curl == 1) {
+        for (int q = 0; q < ntf; ++q)
+            if (tb[q].comp == c && tb[q].curl == 0 && box_has(tb[q], ijk)) return;   // owned by that thread
+    }
+    float upd = tf_term<T, IS_H>(g, bx, inc, ijk, tp);
+    if (!IS_H) {
+        for (int q = 0; q < ntf; ++q)
+            if (tb[q].comp == c && tb[q].curl == 1 && box_has(tb[q], ijk)) upd += tf_term<T, IS_H>(g, tb[q], inc, ijk, tp);
+    }
+    const long long o = (long long)ijk[2] * g.plane + (long long)ijk[1] * g.px + ijk[0];
+    if (IS_H) {
+        T *F = g.F + (3 + c) * g.fstride;
+        st1(F + o, ld1(F + o) - upd);
+    } else {
+        T *F = g.F + c * g.fstride;
+        const float cb = g.cb[g.mat[o]];
+        st1(F + o, ld1(F + o) + cb * upd);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// point source, sequence end, gather, monitor, fills
+// ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_point_source(Grid<T> g, PointSrc ps) {
-    const int t = g.counter[0];
-    T *E = g.f.E[ps.comp];
+__global__ void k_point_source(Grid<T> g, PointSrc ps, int toff) {
+    const int t = g.counter[0] + toff;
+    T *E = g.F + ps.comp * g.fstride;
     if (ps.kind == LX_SRC_HARD_CW) {
         const int prod = (int)((unsigned)ps.omega * (unsigned)t);
-        st(E, ps.idx, (float)(ps.amp * sin(prod * ps.dt)));
+        st1(E + ps.idx, (float)(ps.amp * sin(prod * ps.dt)));
     } else {
         const double w = waveform(ps.wave, t * ps.dt, ps.c, ps.lambda0, ps.fwhm, ps.delay);
-        st(E, ps.idx, (float)((double)ld(E, ps.idx) + ps.amp * w));
+        st1(E + ps.idx, (float)((double)ld1(E + ps.idx) + ps.amp * w));
     }
 }
 
 template <typename T>
-__global__ void k_probes(Grid<T> g, ProbeSet p) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= p.n) return;
-    const int t = g.counter[0] - 1;
-    if (t < 0 || t >= p.cap) return;
-    const int c = p.comp[q];
-    const T *F = c < 3 ? g.f.E[c] : g.f.H[c - 3];
-    double v = (double)ld(F, p.idx[q]);
-    if (c >= 3) v *= p.hscale;
-    p.rec[(long long)t * p.n + q] = v;
+__global__ void k_finish(Grid<T> g, ProbeArgs pr, int n) {
+    const int tlast = g.counter[0] + n - 1;
+    for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
+        if (pr.comp[q] < 3 && tlast < pr.cap)
+            pr.rec[(long long)tlast * pr.n + q] = (double)ld1(g.F + pr.comp[q] * g.fstride + pr.idx[q]);
+    __syncthreads();
+    if (threadIdx.x == 0) g.counter[0] = tlast + 1;
 }
 
-// ---------------------------------------------------------------------------
-// gather / monitor / fills
-// ---------------------------------------------------------------------------
 template <typename T>
 __global__ void k_gather(Grid<T> g, const T *F, int ex, int ey, int ez, double scale, double *dst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     const int k = blockIdx.z;
     if (i >= ex) return;
-    const double v = (double)ld(F, (long long)k * g.plane + (long long)j * g.px + i) * scale;
+    const double v = (double)ld1(F + (long long)k * g.plane + (long long)j * g.px + i) * scale;
     dst[((long long)i * ey + j) * ez + k] = v;
 }
 
@@ -417,23 +688,27 @@ __global__ __launch_bounds__(256) void k_monitor(Grid<T> g, int planar, double *
     const int k = blockIdx.z;
     double mx = 0.0, sum = 0.0;
     int bad = 0;
+    const long long fs = g.fstride;
     if (i < g.nx) {
         const long long o = (long long)k * g.plane + (long long)j * g.px + i;
         if (planar) {
-            const double e = ld(g.f.E[1], o);
-            if (i < g.nx && k < g.nz) { mx = e * e; sum = e * e; }
-            const double hx = ld(g.f.H[0], o), hz = ld(g.f.H[2], o);
+            const double e = ld1(g.F + fs + o);
+            mx = e * e;
+            sum = e * e;
+            const double hx = ld1(g.F + 3 * fs + o), hz = ld1(g.F + 5 * fs + o);
             sum += hx * hx + hz * hz;
             bad = !isfinite(e) || !isfinite(hx) || !isfinite(hz);
         } else {
-            const double ex = ld(g.f.E[0], o), ey = ld(g.f.E[1], o), ez = ld(g.f.E[2], o);
-            const double hx = ld(g.f.H[0], o), hy = ld(g.f.H[1], o), hz = ld(g.f.H[2], o);
+            const double ex = ld1(g.F + o), ey = ld1(g.F + fs + o), ez = ld1(g.F + 2 * fs + o);
+            const double hx = ld1(g.F + 3 * fs + o), hy = ld1(g.F + 4 * fs + o), hz = ld1(g.F + 5 * fs + o);
             if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) mx = ex * ex + ey * ey + ez * ez;
             sum = ex * ex + ey * ey + ez * ez + hx * hx + hy * hy + hz * hz;
             bad = !isfinite(ex) || !isfinite(ey) || !isfinite(ez) || !isfinite(hx) || !isfinite(hy) || !isfinite(hz);
         }
     }
-    smax[tid] = mx; ssum[tid] = sum; sbad[tid] = bad;
+    smax[tid] = mx;
+    ssum[tid] = sum;
+    sbad[tid] = bad;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (tid < s) {
@@ -464,42 +739,48 @@ __global__ void k_zero_u32(uint32_t *p, size_t n) {
 // launchers
 // ---------------------------------------------------------------------------
 template <typename T>
-void launch_h3(const Grid<T> &g, const TfSet &tf, const IncParams &inc, hipStream_t s) {
-    dim3 blk(kBX, kBY), grd((g.nx + kBX - 1) / kBX, (g.ny + kBY - 1) / kBY, (g.nz + g.kz - 1) / g.kz);
-    hipLaunchKernelGGL(k_h3<T>, grd, blk, 0, s, g, tf, inc);
+void launch_bulk(const Grid<T> &g, int planar, int half, hipStream_t s) {
+    const dim3 grd((unsigned)g.ntiles), blk(256);
+    if (planar) {
+        if (half == 0) {
+            if (g.pml) hipLaunchKernelGGL((k_h2<T, true>), grd, blk, 0, s, g);
+            else hipLaunchKernelGGL((k_h2<T, false>), grd, blk, 0, s, g);
+        } else {
+            if (g.pml) hipLaunchKernelGGL((k_e2<T, true>), grd, blk, 0, s, g);
+            else hipLaunchKernelGGL((k_e2<T, false>), grd, blk, 0, s, g);
+        }
+    } else {
+        if (half == 0) {
+            if (g.pml) hipLaunchKernelGGL((k_h3<T, true>), grd, blk, 0, s, g);
+            else hipLaunchKernelGGL((k_h3<T, false>), grd, blk, 0, s, g);
+        } else {
+            if (g.pml) hipLaunchKernelGGL((k_e3<T, true>), grd, blk, 0, s, g);
+            else hipLaunchKernelGGL((k_e3<T, false>), grd, blk, 0, s, g);
+        }
+    }
+}
+
+template <typename T>
+void launch_tfsf(const Grid<T> &g, const TfList &tf, const IncParams &inc, const AuxLine &aux, const ProbeArgs &pr,
+                 Launch l, int half, hipStream_t s) {
+    const int nb = (tf.total + 255) / 256;
+    const unsigned blocks = (unsigned)nb + ((l.aux || l.probes) ? 1u : 0u);
+    if (blocks == 0) return;
+    if (half == 0) hipLaunchKernelGGL((k_tfsf<T, true>), dim3(blocks), dim3(256), 0, s, g, tf.dev, tf.n, tf.total, inc, aux, pr, l);
+    else hipLaunchKernelGGL((k_tfsf<T, false>), dim3(blocks), dim3(256), 0, s, g, tf.dev, tf.n, tf.total, inc, aux, pr, l);
+}
+
+template <typename T>
+void launch_finish(const Grid<T> &g, const ProbeArgs &pr, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_finish<T>, dim3(1), dim3(64), 0, s, g, pr, n);
 }
 template <typename T>
-void launch_e3(const Grid<T> &g, const TfSet &tf, const IncParams &inc, hipStream_t s) {
-    dim3 blk(kBX, kBY), grd((g.nx + kBX - 1) / kBX, (g.ny + kBY - 1) / kBY, (g.nz + g.kz - 1) / g.kz);
-    hipLaunchKernelGGL(k_e3<T>, grd, blk, 0, s, g, tf, inc);
+void launch_point_source(const Grid<T> &g, const PointSrc &ps, int toff, hipStream_t s) {
+    hipLaunchKernelGGL(k_point_source<T>, dim3(1), dim3(1), 0, s, g, ps, toff);
 }
 template <typename T>
-void launch_h2(const Grid<T> &g, const TfSet &tf, const IncParams &inc, hipStream_t s) {
-    dim3 blk(kBX, kBY), grd((g.nx + kBX - 1) / kBX, (g.nz + kBY - 1) / kBY);
-    hipLaunchKernelGGL(k_h2<T>, grd, blk, 0, s, g, tf, inc);
-}
-template <typename T>
-void launch_e2(const Grid<T> &g, const TfSet &tf, const IncParams &inc, hipStream_t s) {
-    dim3 blk(kBX, kBY), grd((g.nx + kBX - 1) / kBX, (g.nz + kBY - 1) / kBY);
-    hipLaunchKernelGGL(k_e2<T>, grd, blk, 0, s, g, tf, inc);
-}
-void launch_aux_h(const AuxLine &a, hipStream_t s) { hipLaunchKernelGGL(k_aux_h, dim3(1), dim3(1024), 0, s, a); }
-void launch_aux_e(const AuxLine &a, const int *counter, hipStream_t s) {
-    hipLaunchKernelGGL(k_aux_e, dim3(1), dim3(1024), 0, s, a, counter);
-}
-template <typename T>
-void launch_point_source(const Grid<T> &g, const PointSrc &ps, hipStream_t s) {
-    hipLaunchKernelGGL(k_point_source<T>, dim3(1), dim3(1), 0, s, g, ps);
-}
-template <typename T>
-void launch_probes(const Grid<T> &g, const ProbeSet &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_probes<T>, dim3((p.n + 63) / 64), dim3(64), 0, s, g, p);
-}
-template <typename T>
-void launch_gather(const Grid<T> &g, int comp, int ex, int ey, int ez, double scale, double *dst, int planar,
-                   hipStream_t s) {
-    const T *F = comp < 3 ? g.f.E[comp] : g.f.H[comp - 3];
-    (void)planar;
+void launch_gather(const Grid<T> &g, int comp, int ex, int ey, int ez, double scale, double *dst, hipStream_t s) {
+    const T *F = g.F + comp * g.fstride;
     dim3 blk(256), grd((ex + 255) / 256, ey, ez);
     hipLaunchKernelGGL(k_gather<T>, grd, blk, 0, s, g, F, ex, ey, ez, scale, dst);
 }
@@ -523,13 +804,12 @@ void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
 }
 
 #define LX_INST(T)                                                                                            \
-    template void launch_h3<T>(const Grid<T> &, const TfSet &, const IncParams &, hipStream_t);              \
-    template void launch_e3<T>(const Grid<T> &, const TfSet &, const IncParams &, hipStream_t);              \
-    template void launch_h2<T>(const Grid<T> &, const TfSet &, const IncParams &, hipStream_t);              \
-    template void launch_e2<T>(const Grid<T> &, const TfSet &, const IncParams &, hipStream_t);              \
-    template void launch_point_source<T>(const Grid<T> &, const PointSrc &, hipStream_t);                    \
-    template void launch_probes<T>(const Grid<T> &, const ProbeSet &, hipStream_t);                          \
-    template void launch_gather<T>(const Grid<T> &, int, int, int, int, double, double *, int, hipStream_t); \
+    template void launch_bulk<T>(const Grid<T> &, int, int, hipStream_t);                                    \
+    template void launch_tfsf<T>(const Grid<T> &, const TfList &, const IncParams &, const AuxLine &,        \
+                                 const ProbeArgs &, Launch, int, hipStream_t);                               \
+    template void launch_finish<T>(const Grid<T> &, const ProbeArgs &, int, hipStream_t);                    \
+    template void launch_point_source<T>(const Grid<T> &, const PointSrc &, int, hipStream_t);               \
+    template void launch_gather<T>(const Grid<T> &, int, int, int, int, double, double *, hipStream_t);      \
     template void launch_monitor<T>(const Grid<T> &, int, double *, hipStream_t);
 LX_INST(float)
 LX_INST(__half)

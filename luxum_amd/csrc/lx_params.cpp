@@ -312,8 +312,8 @@ extern "C" lx_status lx_params_check(const lx_params *p) {
         if (hx * hx + hy * hy + hz * hz == 0.0) return lx::fail(LX_EINVAL, "plane wave: k x E is zero");
         for (int a = 0; a < 3; ++a) {
             if (p->is_planar && a == 1) continue;
-            if (p->tf_lo[a] < 1 || p->tf_hi[a] > N[a] - 2 || p->tf_lo[a] > p->tf_hi[a])
-                return lx::fail(LX_EINVAL, "TF/SF box must satisfy 1 <= lo <= hi <= N-2 on every axis");
+            if (p->tf_lo[a] < 1 || p->tf_hi[a] > N[a] - 2 || p->tf_lo[a] >= p->tf_hi[a])
+                return lx::fail(LX_EINVAL, "TF/SF box must satisfy 1 <= lo < hi <= N-2 on every axis");
         }
         if (p->incident == LX_INC_AUX1D) {
             int ax = -1, n = 0;

@@ -122,6 +122,11 @@ class Simulation:
     def stream(self) -> int:
         return lib().lx_sim_stream(self._h) or 0
 
+    def autotune(self) -> int:
+        kz = C.c_int32()
+        check(lib().lx_sim_autotune(self._h, C.byref(kz)))
+        return kz.value
+
     def time_kernels(self, n: int):
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         check(lib().lx_sim_time_kernels(self._h, int(n), C.byref(a), C.byref(b), C.byref(c)))

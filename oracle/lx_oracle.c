@@ -927,6 +927,16 @@ int lxo_read_field(lxo_sim *s, int comp, double *dst, size_t n) {
     return 0;
 }
 
+int lxo_set_field(lxo_sim *s, int comp, const double *src, size_t n) {
+    if (comp < 0 || comp > 5) return -1;
+    fld *f = comp < 3 ? &s->E[comp] : &s->H[comp - 3];
+    if (!f->v) return -1;
+    size_t m = (size_t)f->n[0] * f->n[1] * f->n[2];
+    if (n != m) return -1;
+    memcpy(f->v, src, sizeof(double) * m);
+    return 0;
+}
+
 int lxo_read_probes(lxo_sim *s, double *dst, size_t n, int *n_steps) {
     size_t m = (size_t)s->rec_steps * s->nprobe;
     if (n < m) return -1;

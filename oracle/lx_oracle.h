@@ -24,6 +24,8 @@ int lxo_set_materials(lxo_sim *s, const uint8_t *ids, const double *eps_r, int n
 int lxo_set_probes(lxo_sim *s, const int32_t *comp_ijk, int n);
 int lxo_step(lxo_sim *s, int nsteps);
 int lxo_read_field(lxo_sim *s, int comp, double *dst, size_t n);
+/* Overwrite a component (reference layout, SI units); test set-up only. */
+int lxo_set_field(lxo_sim *s, int comp, const double *src, size_t n);
 int lxo_read_probes(lxo_sim *s, double *dst, size_t n, int *n_steps);
 int lxo_field_size(const lx_params *p, int comp, int ext[3]);
 /* 1D CPML profile along `dim` at E nodes (is_E=1, n = N) or H nodes (n = N-1). */

@@ -39,6 +39,7 @@ def lib():
         L.lxo_set_probes.argtypes = [vp, C.c_void_p, C.c_int]
         L.lxo_step.argtypes = [vp, C.c_int]
         L.lxo_read_field.argtypes = [vp, C.c_int, C.c_void_p, C.c_size_t]
+        L.lxo_set_field.argtypes = [vp, C.c_int, C.c_void_p, C.c_size_t]
         L.lxo_read_probes.argtypes = [vp, C.c_void_p, C.c_size_t, C.POINTER(C.c_int)]
         L.lxo_field_size.argtypes = [P, C.c_int, C.c_int * 3]
         L.lxo_profile.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int]
@@ -114,6 +115,13 @@ class OracleSim:
         if rc != 0:
             raise ValueError("bad component")
         return out
+
+    def set_field(self, comp: int, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == tuple(self.p.field_shape(comp))
+        rc = lib().lxo_set_field(self._h, comp, a.ctypes.data, a.size)
+        if rc != 0:
+            raise ValueError("bad component")
 
     def probes(self) -> np.ndarray:
         cap = 1 << 20
