@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-steps", type=int, default=50)
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (PMC traffic calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -62,6 +64,16 @@ def main():
     from luxum_amd import configs
     from luxum_amd.params import LX_FP16, LX_FP32
     from luxum_amd.sim import Context, Simulation
+
+    if args.copy_calib:
+        import torch
+        a = torch.ones(args.copy_calib * (1 << 18), device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"bench: copy calibration {args.copy_calib} MiB read + {args.copy_calib} MiB write",
+              file=sys.stderr, flush=True)
+        del a, b
 
     prec = LX_FP16 if args.fp16 else LX_FP32
     w = configs.config1(n=args.n, steps=max(args.steps + args.warmup + args.kernel_steps, 2000), precision=prec)

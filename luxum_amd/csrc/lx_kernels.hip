@@ -621,7 +621,7 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict
             if (tb[q].comp == c && tb[q].curl == 0 && box_has(tb[q], ijk)) return;   // owned by that thread
     }
     float upd = tf_term<T, IS_H>(g, bx, inc, ijk, tp);
-    if (!IS_H) {
+    if (!IS_H && bx.curl == 0) {
         for (int q = 0; q < ntf; ++q)
             if (tb[q].comp == c && tb[q].curl == 1 && box_has(tb[q], ijk)) upd += tf_term<T, IS_H>(g, tb[q], inc, ijk, tp);
     }
