@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -231,9 +232,19 @@ lx_status build_tables(lx_sim *s) {
             for (int i = lo; i < hi; ++i)
                 if (nz(i)) return fail(LX_EINVAL, "CPML profile along axis %d is not confined to the boundary slabs", a);
         }
-        s->slo[a] = lo;
-        s->shi[a] = hi;
-        s->sw[a] = lo + (N - hi);
+        if (a == 0 && on) {
+            // x slab: 4-aligned so that every 4-cell lane group is wholly in or out of it
+            // (the extra cells have c = 0, so their psi stays zero)
+            lo = (lo + 3) / 4 * 4;
+            hi = std::max(lo, hi / 4 * 4);
+            s->slo[a] = lo;
+            s->shi[a] = hi;
+            s->sw[a] = lo + (s->px - hi);
+        } else {
+            s->slo[a] = lo;
+            s->shi[a] = hi;
+            s->sw[a] = lo + (N - hi);
+        }
     }
     void *p = nullptr;
     lx_status st = dalloc(s, &p, host.size() * sizeof(float));
@@ -940,6 +951,8 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
                 float ms = 0;
                 HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
                 if (rep == 1 && ms < best) { best = ms; best_kz = kz; }
+                if (rep == 1 && std::getenv("LX_VERBOSE"))
+                    std::fprintf(stderr, "lx_sim_autotune: kz=%d  %.4f ms per step (bulk kernels)\n", kz, ms / 4);
             }
         }
         (void)hipEventDestroy(e0);

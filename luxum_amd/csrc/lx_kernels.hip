@@ -171,11 +171,13 @@ __global__ __launch_bounds__(256) void k_h3(Grid<T> g) {
     // CPML (slabs)
     const bool sy = PML && (j < g.slo[1] || j >= g.shi[1]);
     const float bhy = sy ? tab[(1 * 6 + TK_BH) * tn + j] : 0.f, chy = sy ? tab[(1 * 6 + TK_CH) * tn + j] : 0.f;
-    const bool anyx = PML && on && (i0 < g.slo[0] || i0 + 3 >= g.shi[0]);
-    const unsigned msx = anyx ? (xmask(i0, 0, g.slo[0] - 1) | xmask(i0, g.shi[0], nx - 1)) & mx_all : 0u;
+    // x slab: 4-aligned, so a lane's 4 cells are wholly in or out of it
+    const bool sx = PML && on && (i0 < g.slo[0] || i0 >= g.shi[0]);
+    const int xs = sx ? (i0 < g.slo[0] ? i0 : g.slo[0] + (i0 - g.shi[0])) : 0;
     float *psy0 = PML ? psi_base(g, 1, 0) : nullptr, *psy1 = PML ? psi_base(g, 1, 1) : nullptr;
     float *psz0 = PML ? psi_base(g, 2, 0) : nullptr, *psz1 = PML ? psi_base(g, 2, 1) : nullptr;
     float *psx0 = PML ? psi_base(g, 0, 0) : nullptr, *psx1 = PML ? psi_base(g, 0, 1) : nullptr;
+    const int siy = PML && sy ? slab_index(j, g.slo[1], g.shi[1]) : 0;
 
     long long o = (long long)k0 * plane + (long long)j * px + i0;
     const long long dj = (long long)(jp - j) * px;
@@ -194,8 +196,9 @@ __global__ __launch_bounds__(256) void k_h3(Grid<T> g) {
         const int kk = min(k, nz - 2);
         const float khz = tab[(2 * 6 + TK_KH) * tn + kk];
         const bool sz = PML && (k < g.slo[2] || k >= g.shi[2]);
-        const long long zoff = PML && sz ? ((long long)slab_index(k, g.slo[2], g.shi[2]) * ny + j) * px + i0 : 0;
-        const long long yoff = PML && sy ? ((long long)k * g.sw[1] + slab_index(j, g.slo[1], g.shi[1])) * px + i0 : 0;
+        const int zoff = sz ? (slab_index(k, g.slo[2], g.shi[2]) * ny + j) * (int)px + i0 : 0;
+        const int yoff = sy ? (k * g.sw[1] + siy) * (int)px + i0 : 0;
+        const int xoff = sx ? (k * ny + j) * g.sw[0] + xs : 0;
         const float bhz = sz ? tab[(2 * 6 + TK_BH) * tn + k] : 0.f, chz = sz ? tab[(2 * 6 + TK_CH) * tn + k] : 0.f;
         if (on) {
             // Hx: d1 = dEz/dy (y, psi y slot0), d2 = dEy/dz (z, psi z slot1)
@@ -229,15 +232,12 @@ __global__ __launch_bounds__(256) void k_h3(Grid<T> g) {
                     st4(psz0 + zoff, p);
                     acc = acc + p;
                 }
-                if (msx) {
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (msx & (1u << e)) {
-                            float *q = psx1 + psi_off(g, 0, i0 + e, j, k);
-                            const float v = tab[(0 * 6 + TK_BH) * tn + i0 + e] * *q + tab[(0 * 6 + TK_CH) * tn + i0 + e] * get(d2, e);
-                            *q = v;
-                            set(acc, e, get(acc, e) - v);
-                        }
+                if (sx) {
+                    float4 p = ld4(psx1 + xoff);
+                    p = p * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_BH) * tn + i0) +
+                        d2 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_CH) * tn + i0);
+                    st4(psx1 + xoff, p);
+                    acc = acc - p;
                 }
                 const float4 h = ld4(Hy + o);
                 st4(Hy + o, sel(mx_in, h - acc, h));
@@ -248,15 +248,12 @@ __global__ __launch_bounds__(256) void k_h3(Grid<T> g) {
                 const float4 exj = ld4(Ex + o + dj);
                 const float4 d1 = eyi - ey0, d2 = exj - ex0;
                 float4 acc = d1 * khx - d2 * khy;
-                if (msx) {
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (msx & (1u << e)) {
-                            float *q = psx0 + psi_off(g, 0, i0 + e, j, k);
-                            const float v = tab[(0 * 6 + TK_BH) * tn + i0 + e] * *q + tab[(0 * 6 + TK_CH) * tn + i0 + e] * get(d1, e);
-                            *q = v;
-                            set(acc, e, get(acc, e) + v);
-                        }
+                if (sx) {
+                    float4 p = ld4(psx0 + xoff);
+                    p = p * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_BH) * tn + i0) +
+                        d1 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_CH) * tn + i0);
+                    st4(psx0 + xoff, p);
+                    acc = acc + p;
                 }
                 if (sy) {
                     float4 p = ld4(psy1 + yoff);
@@ -308,11 +305,12 @@ __global__ __launch_bounds__(256) void k_e3(Grid<T> g) {
     const float key = tab[(1 * 6 + TK_KE) * tn + j];
     const bool sy = PML && (j < g.slo[1] || j >= g.shi[1]);
     const float bey = sy ? tab[(1 * 6 + TK_BE) * tn + j] : 0.f, cey = sy ? tab[(1 * 6 + TK_CE) * tn + j] : 0.f;
-    const bool anyx = PML && on && (i0 < g.slo[0] || i0 + 3 >= g.shi[0]);
-    const unsigned msx = anyx ? xmask(i0, 0, g.slo[0] - 1) | xmask(i0, g.shi[0], nx - 1) : 0u;
+    const bool sx = PML && on && (i0 < g.slo[0] || i0 >= g.shi[0]);
+    const int xs = sx ? (i0 < g.slo[0] ? i0 : g.slo[0] + (i0 - g.shi[0])) : 0;
     float *psy0 = PML ? psi_base(g, 1, 2) : nullptr, *psy1 = PML ? psi_base(g, 1, 3) : nullptr;
     float *psz0 = PML ? psi_base(g, 2, 2) : nullptr, *psz1 = PML ? psi_base(g, 2, 3) : nullptr;
     float *psx0 = PML ? psi_base(g, 0, 2) : nullptr, *psx1 = PML ? psi_base(g, 0, 3) : nullptr;
+    const int siy = PML && sy ? slab_index(j, g.slo[1], g.shi[1]) : 0;
 
     long long o = (long long)k0 * plane + (long long)j * px + i0;
     const long long dj = (long long)(j - jm) * px;
@@ -330,8 +328,9 @@ __global__ __launch_bounds__(256) void k_e3(Grid<T> g) {
         const bool kin = (k >= 1) && (k <= nz - 2);
         const float kez = tab[(2 * 6 + TK_KE) * tn + k];
         const bool sz = PML && (k < g.slo[2] || k >= g.shi[2]);
-        const long long zoff = PML && sz ? ((long long)slab_index(k, g.slo[2], g.shi[2]) * ny + j) * px + i0 : 0;
-        const long long yoff = PML && sy ? ((long long)k * g.sw[1] + slab_index(j, g.slo[1], g.shi[1])) * px + i0 : 0;
+        const int zoff = sz ? (slab_index(k, g.slo[2], g.shi[2]) * ny + j) * (int)px + i0 : 0;
+        const int yoff = sy ? (k * g.sw[1] + siy) * (int)px + i0 : 0;
+        const int xoff = sx ? (k * ny + j) * g.sw[0] + xs : 0;
         const float bez = sz ? tab[(2 * 6 + TK_BE) * tn + k] : 0.f, cez = sz ? tab[(2 * 6 + TK_CE) * tn + k] : 0.f;
         if (on) {
             const uchar4 id = *reinterpret_cast<const uchar4 *>(g.mat + o);
@@ -367,15 +366,12 @@ __global__ __launch_bounds__(256) void k_e3(Grid<T> g) {
                     st4(psz0 + zoff, p);
                     acc = acc + p;
                 }
-                if (msx & m_ey) {
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (msx & m_ey & (1u << e)) {
-                            float *q = psx1 + psi_off(g, 0, i0 + e, j, k);
-                            const float v = tab[(0 * 6 + TK_BE) * tn + i0 + e] * *q + tab[(0 * 6 + TK_CE) * tn + i0 + e] * get(d2, e);
-                            *q = v;
-                            set(acc, e, get(acc, e) - v);
-                        }
+                if (sx) {
+                    float4 p = ld4(psx1 + xoff);
+                    p = p * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_BE) * tn + i0) +
+                        d2 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_CE) * tn + i0);
+                    st4(psx1 + xoff, p);
+                    acc = acc - p;
                 }
                 const float4 e = ld4(Ey + o);
                 st4(Ey + o, sel(m_ey, e + cb * acc, e));
@@ -386,15 +382,12 @@ __global__ __launch_bounds__(256) void k_e3(Grid<T> g) {
                 const float4 hxj = ld4(Hx + o - dj);
                 const float4 d1 = hy0 - hyi, d2 = hx0 - hxj;
                 float4 acc = d1 * kex - d2 * key;
-                if (msx & m_ez) {
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (msx & m_ez & (1u << e)) {
-                            float *q = psx0 + psi_off(g, 0, i0 + e, j, k);
-                            const float v = tab[(0 * 6 + TK_BE) * tn + i0 + e] * *q + tab[(0 * 6 + TK_CE) * tn + i0 + e] * get(d1, e);
-                            *q = v;
-                            set(acc, e, get(acc, e) + v);
-                        }
+                if (sx) {
+                    float4 p = ld4(psx0 + xoff);
+                    p = p * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_BE) * tn + i0) +
+                        d1 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_CE) * tn + i0);
+                    st4(psx0 + xoff, p);
+                    acc = acc + p;
                 }
                 if (sy) {
                     float4 p = ld4(psy1 + yoff);
@@ -455,17 +448,13 @@ __global__ __launch_bounds__(256) void k_h2(Grid<T> g) {
         const float4 d1 = ei - e0;
         float4 acc = d1 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_KH) * tn + i0);
         const unsigned mi = xmask(i0, 0, nx - 2);
-        if (PML && (i0 < g.slo[0] || i0 + 3 >= g.shi[0])) {
-            const unsigned ms = (xmask(i0, 0, g.slo[0] - 1) | xmask(i0, g.shi[0], nx - 1)) & mi;
-            float *pb = psi_base(g, 0, 0);
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                if (ms & (1u << e)) {
-                    float *q = pb + psi_off(g, 0, i0 + e, 0, k);
-                    const float v = tab[(0 * 6 + TK_BH) * tn + i0 + e] * *q + tab[(0 * 6 + TK_CH) * tn + i0 + e] * get(d1, e);
-                    *q = v;
-                    set(acc, e, get(acc, e) + v);
-                }
+        if (PML && (i0 < g.slo[0] || i0 >= g.shi[0])) {
+            float *q = psi_base(g, 0, 0) + k * g.sw[0] + (i0 < g.slo[0] ? i0 : g.slo[0] + (i0 - g.shi[0]));
+            float4 p = ld4(q);
+            p = p * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_BH) * tn + i0) +
+                d1 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_CH) * tn + i0);
+            st4(q, p);
+            acc = acc + p;
         }
         const float4 h = ld4(Hz + o);
         st4(Hz + o, sel(mi, h - acc, h));
@@ -510,17 +499,13 @@ __global__ __launch_bounds__(256) void k_e2(Grid<T> g) {
             st4(q, p);
             acc = acc + p;
         }
-        if (i0 < g.slo[0] || i0 + 3 >= g.shi[0]) {
-            const unsigned ms = (xmask(i0, 0, g.slo[0] - 1) | xmask(i0, g.shi[0], nx - 1)) & m;
-            float *pb = psi_base(g, 0, 3);
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                if (ms & (1u << e)) {
-                    float *q = pb + psi_off(g, 0, i0 + e, 0, k);
-                    const float v = tab[(0 * 6 + TK_BE) * tn + i0 + e] * *q + tab[(0 * 6 + TK_CE) * tn + i0 + e] * get(d2, e);
-                    *q = v;
-                    set(acc, e, get(acc, e) - v);
-                }
+        if (i0 < g.slo[0] || i0 >= g.shi[0]) {
+            float *q = psi_base(g, 0, 3) + k * g.sw[0] + (i0 < g.slo[0] ? i0 : g.slo[0] + (i0 - g.shi[0]));
+            float4 p = ld4(q);
+            p = p * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_BE) * tn + i0) +
+                d2 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_CE) * tn + i0);
+            st4(q, p);
+            acc = acc - p;
         }
     }
     const uchar4 id = *reinterpret_cast<const uchar4 *>(g.mat + o);
