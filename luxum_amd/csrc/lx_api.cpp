@@ -29,6 +29,7 @@
 #include "../../include/luxum.h"
 #include "lx_internal.hpp"
 #include "lx_util.hpp"
+#include "lx_sim.hpp"
 
 namespace lx {
 char *error_buffer() {
@@ -45,63 +46,7 @@ using namespace lx;
         if (_e != hipSuccess) return fail(LX_EDEVICE, "%s failed: %s", #x, hipGetErrorString(_e));          \
     } while (0)
 
-struct lx_ctx {
-    int device;
-    int64_t total_mem;
-};
-
-struct lx_sim {
-    lx_ctx *ctx = nullptr;
-    lx_params p{};
-    int planar = 0, fp16 = 0, pml = 0;
-    int N[3] = {0, 0, 0};
-    int px = 0;
-    long long plane = 0;
-    size_t esize = 4;
-    // constants (maxwell.cc:63-84)
-    double eps0 = 0, mu0 = 0, c0 = 0, dt = 0, eta0 = 0, sigma_max = 0;
-    hipStream_t cs = nullptr, ss = nullptr;
-    std::vector<void *> allocs;
-    int64_t bytes = 0;
-    void *F = nullptr;          // six components, component c at F + c*fstride elements
-    long long fstride = 0;
-    uint8_t *mat = nullptr;
-    float *cb = nullptr;
-    float *tab = nullptr;
-    float *psi = nullptr;
-    int *counter = nullptr;
-    int tn = 0;
-    int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0}, sw[3] = {0, 0, 0};
-    int64_t slab_cells[3] = {0, 0, 0};
-    TfList tfH{}, tfE{};
-    int kz = 32;
-    IncParams inc{};
-    AuxLine aux{};
-    int use_aux = 0;
-    PointSrc ps{};
-    // probes
-    int np = 0, pcap = 0;
-    long long *pidx = nullptr;
-    int *pcomp = nullptr;
-    double *prec = nullptr;
-    std::vector<int32_t> probe_host;
-    // snapshots
-    double *stage[2] = {nullptr, nullptr};
-    size_t stage_n[2] = {0, 0};
-    hipEvent_t ready[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
-    int snap_next = 0;
-    double *rd_stage = nullptr;
-    size_t rd_n = 0;
-    double *mon = nullptr;
-    // graphs
-    hipGraphExec_t gexec = nullptr;
-    int gK = 0;
-    int graphs = 1;
-    int steps = 0;
-    float cb_host[256];
-};
-
-namespace {
+namespace lx {
 
 template <typename T>
 Grid<T> make_grid(const lx_sim *s) {
@@ -538,7 +483,16 @@ lx_status check_launch() {
     return LX_OK;
 }
 
-}  // namespace
+size_t psi_total(const lx_sim *s) {
+    size_t total = 0;
+    for (int a = 0; a < 3; ++a) total += 4 * (size_t)s->slab_cells[a];
+    return std::max<size_t>(total, 1);
+}
+
+template Grid<float> make_grid<float>(const lx_sim *s);
+template Grid<__half> make_grid<__half>(const lx_sim *s);
+
+}  // namespace lx
 
 // ===========================================================================
 // C ABI
@@ -578,6 +532,7 @@ lx_status lx_ctx_mem_info(lx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes
 
 void lx_sim_destroy(lx_sim *s) {
     if (!s) return;
+    if (s->batch) (void)lx_batch_remove(s->batch, s);
     (void)hipSetDevice(s->ctx->device);
     if (s->cs) (void)hipStreamSynchronize(s->cs);
     if (s->ss) (void)hipStreamSynchronize(s->ss);

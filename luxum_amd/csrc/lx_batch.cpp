@@ -1,53 +1,370 @@
 // lx_batch.cpp -- batches of independent simulations.
 //
-// A batch advances its member simulations together: every member's steps are
-// enqueued on the member's own compute stream, so the device overlaps their
-// kernels; lx_batch_sync waits for all of them.
+// Planar members that the temporally tiled kernel supports (CPML slabs of at
+// most 24 cells, no point source, aux-1D or no plane wave, both sides at least
+// 128 cells) advance together: per chunk of up to 8 steps, one launch of
+// k_aux_chunk (incident line of every member), one launch of k_tile2d over
+// the flattened tile list of every member, one launch of k_batch_finish.
+// Pairs of full chunks are captured once as a hipGraph and replayed.  Other
+// members step on their own streams with the single-simulation kernels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <new>
 #include <vector>
 
 #include "../../include/luxum.h"
+#include "lx_batch.hpp"
+#include "lx_sim.hpp"
 #include "lx_util.hpp"
 
-struct lx_batch {
-    lx_ctx *ctx;
+using namespace lx;
+
+#define HIP_TRY(x)                                                                                         \
+    do {                                                                                                   \
+        hipError_t _e = (x);                                                                               \
+        if (_e != hipSuccess) return fail(LX_EDEVICE, "%s failed: %s", #x, hipGetErrorString(_e));          \
+    } while (0)
+
+namespace {
+
+struct Group {
+    int fp16 = 0;
     std::vector<lx_sim *> sims;
+    BSim *d_sims = nullptr;
+    BJob *d_jobs = nullptr;
+    BAux *d_aux = nullptr;
+    int njobs = 0, naux = 0;
+    int par = 0;                              // buffer index holding the current state
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};   // two chunks starting at parity 0 / 1
 };
+
+}  // namespace
+
+struct lx_batch {
+    lx_ctx *ctx = nullptr;
+    std::vector<lx_sim *> sims;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr, pre = nullptr;
+    bool dirty = true;
+    Group grp[2];                   // FP32 and FP16 storage
+    std::vector<lx_sim *> others;
+    std::vector<void *> allocs;     // descriptor arrays and snapshot buffers
+    int graphs = 1;
+};
+
+namespace {
+
+bool tiled_ok(const lx_sim *s) {
+    const lx_params &p = s->p;
+    if (!p.is_planar || p.use_ps) return false;
+    if (p.use_pw && p.incident != LX_INC_AUX1D) return false;
+    if (s->N[0] < 128 || s->N[2] < 128) return false;
+    if (s->pml) {
+        if (s->slo[0] > kPSW || s->N[0] - s->shi[0] > kPSW) return false;
+        if (s->slo[2] > kPSW || s->N[2] - s->shi[2] > kPSW) return false;
+    }
+    if (s->np > 256) return false;
+    if (s->tfH.n > 4 || s->tfE.n > 4) return false;
+    return true;
+}
+
+void free_group(Group &g) {
+    for (hipGraphExec_t &x : g.gexec) {
+        if (x) (void)hipGraphExecDestroy(x);
+        x = nullptr;
+    }
+    g.par = 0;
+    g.sims.clear();
+    g.d_sims = nullptr;
+    g.d_jobs = nullptr;
+    g.d_aux = nullptr;
+    g.njobs = g.naux = 0;
+}
+
+void free_allocs(lx_batch *b) {
+    for (void *p : b->allocs) (void)hipFree(p);
+    b->allocs.clear();
+}
+
+lx_status balloc(lx_batch *b, void **out, size_t bytes) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(LX_ENOMEM, "batch allocation of %zu bytes failed", bytes);
+    }
+    b->allocs.push_back(p);
+    *out = p;
+    return LX_OK;
+}
+
+lx_status ensure_alt(lx_sim *s) {
+    if (s->F_alt) return LX_OK;
+    void *p = nullptr;
+    const size_t fbytes = (size_t)s->fstride * 6 * s->esize;
+    lx_status st = dalloc(s, &p, fbytes);
+    if (st) return st;
+    launch_zero_bytes(p, fbytes, s->cs);
+    s->F_alt = p;
+    st = dalloc(s, &p, psi_total(s) * sizeof(float));
+    if (st) return st;
+    launch_zero_bytes(p, psi_total(s) * sizeof(float), s->cs);
+    s->psi_alt = (float *)p;
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    return LX_OK;
+}
+
+lx_status rebuild(lx_batch *b) {
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (Group &g : b->grp) free_group(g);
+    free_allocs(b);
+    b->others.clear();
+    b->grp[0].fp16 = 0;
+    b->grp[1].fp16 = 1;
+    for (lx_sim *s : b->sims) {
+        if (tiled_ok(s)) b->grp[s->fp16].sims.push_back(s);
+        else b->others.push_back(s);
+    }
+    for (Group &g : b->grp) {
+        if (g.sims.empty()) continue;
+        std::vector<BSim> hs(g.sims.size());
+        std::vector<BJob> jobs;
+        std::vector<BAux> aux;
+        for (size_t q = 0; q < g.sims.size(); ++q) {
+            lx_sim *s = g.sims[q];
+            lx_status st = ensure_alt(s);
+            if (st) return st;
+            BSim &d = hs[q];
+            std::memset(&d, 0, sizeof(d));
+            d.nx = s->N[0];
+            d.nz = s->N[2];
+            d.px = s->px;
+            d.fstride = s->fstride;
+            d.F[0] = s->F;
+            d.F[1] = s->F_alt;
+            d.psi[0] = s->psi;
+            d.psi[1] = s->psi_alt;
+            d.mat = s->mat;
+            d.cb = s->cb;
+            d.tab = s->tab;
+            d.tn = s->tn;
+            for (int a = 0; a < 3; ++a) {
+                d.slo[a] = s->slo[a];
+                d.shi[a] = s->shi[a];
+                d.sw[a] = s->sw[a];
+                d.scells[a] = s->slab_cells[a];
+            }
+            d.pml = s->pml;
+            d.counter = s->counter;
+            d.ntfH = s->tfH.n;
+            d.ntfE = s->tfE.n;
+            for (int k = 0; k < s->tfH.n; ++k) d.tfH[k] = s->tfH.b[k];
+            for (int k = 0; k < s->tfE.n; ++k) d.tfE[k] = s->tfE.b[k];
+            if (s->use_aux) {
+                const int ax = s->inc.axis;
+                const int win = std::min(s->aux.L, s->p.tf_hi[ax] + 2 - s->inc.x0 + 1);
+                void *snap = nullptr;
+                st = balloc(b, &snap, sizeof(double) * 2 * kTT * (size_t)win);
+                if (st) return st;
+                d.esnap = (const double *)snap;
+                d.hsnap = (const double *)snap + (size_t)kTT * win;
+                d.win = win;
+                d.inc_axis = ax;
+                d.inc_x0 = s->inc.x0;
+                BAux a{};
+                a.e = s->aux.e;
+                a.h = s->aux.h;
+                a.L = s->aux.L;
+                a.S = s->aux.S;
+                a.wave = s->aux.wave;
+                a.c = s->aux.c;
+                a.dt = s->aux.dt;
+                a.lambda0 = s->aux.lambda0;
+                a.fwhm = s->aux.fwhm;
+                a.delay = s->aux.delay;
+                a.esnap = (double *)d.esnap;
+                a.hsnap = (double *)d.hsnap;
+                a.win = win;
+                a.counter = s->counter;
+                aux.push_back(a);
+            } else {
+                d.ntfH = d.ntfE = 0;
+            }
+            d.np = s->np;
+            d.pcomp = s->pcomp;
+            d.pidx = s->pidx;
+            d.prec = s->prec;
+            d.pcap = s->pcap;
+            d.hscale = 1.0 / s->eta0;
+            for (int tz = 0; tz < (s->N[2] + kTW - 1) / kTW; ++tz)
+                for (int tx = 0; tx < (s->N[0] + kTW - 1) / kTW; ++tx) jobs.push_back(BJob{(int)q, tx * kTW, tz * kTW});
+        }
+        // large simulations first: long jobs start early, small ones fill the tail
+        std::stable_sort(jobs.begin(), jobs.end(), [&](const BJob &x, const BJob &y) {
+            return g.sims[x.sim]->N[0] * g.sims[x.sim]->N[2] > g.sims[y.sim]->N[0] * g.sims[y.sim]->N[2];
+        });
+        void *p = nullptr;
+        lx_status st = balloc(b, &p, sizeof(BSim) * hs.size());
+        if (st) return st;
+        g.d_sims = (BSim *)p;
+        HIP_TRY(hipMemcpy(g.d_sims, hs.data(), sizeof(BSim) * hs.size(), hipMemcpyHostToDevice));
+        st = balloc(b, &p, sizeof(BJob) * jobs.size());
+        if (st) return st;
+        g.d_jobs = (BJob *)p;
+        HIP_TRY(hipMemcpy(g.d_jobs, jobs.data(), sizeof(BJob) * jobs.size(), hipMemcpyHostToDevice));
+        g.njobs = (int)jobs.size();
+        if (!aux.empty()) {
+            st = balloc(b, &p, sizeof(BAux) * aux.size());
+            if (st) return st;
+            g.d_aux = (BAux *)p;
+            HIP_TRY(hipMemcpy(g.d_aux, aux.data(), sizeof(BAux) * aux.size(), hipMemcpyHostToDevice));
+            g.naux = (int)aux.size();
+        }
+    }
+    b->dirty = false;
+    return LX_OK;
+}
+
+void enqueue_chunk(const Group &g, int nt, int par, hipStream_t s) {
+    if (g.naux) launch_aux_chunk(g.d_aux, g.naux, nt, s);
+    if (g.fp16) launch_tile2d<__half>(g.d_sims, g.d_jobs, g.njobs, nt, par, s);
+    else launch_tile2d<float>(g.d_sims, g.d_jobs, g.njobs, nt, par, s);
+    launch_batch_finish(g.d_sims, (int)g.sims.size(), nt, s);
+}
+
+lx_status ensure_pair_graph(lx_batch *b, Group &g, int par) {
+    if (g.gexec[par] || !b->graphs) return LX_OK;
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_chunk(g, kTT, par, b->stream);
+    enqueue_chunk(g, kTT, par ^ 1, b->stream);
+    hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (e != hipSuccess || !graph) {
+        (void)hipGetLastError();
+        b->graphs = 0;
+        return LX_OK;
+    }
+    e = hipGraphInstantiate(&g.gexec[par], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        g.gexec[par] = nullptr;
+        b->graphs = 0;
+    }
+    return LX_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
 lx_status lx_batch_create(lx_ctx *ctx, lx_batch **out) {
-    if (!ctx || !out) return lx::fail(LX_EINVAL, "lx_batch_create: null pointer");
+    if (!ctx || !out) return fail(LX_EINVAL, "lx_batch_create: null pointer");
     lx_batch *b = new (std::nothrow) lx_batch;
-    if (!b) return lx::fail(LX_ENOMEM, "out of host memory");
+    if (!b) return fail(LX_ENOMEM, "out of host memory");
     b->ctx = ctx;
+    if (hipSetDevice(ctx->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->pre, hipEventDisableTiming) != hipSuccess) {
+        delete b;
+        return fail(LX_EDEVICE, "batch stream creation failed");
+    }
     *out = b;
     return LX_OK;
 }
 
-void lx_batch_destroy(lx_batch *b) { delete b; }
+void lx_batch_destroy(lx_batch *b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (Group &g : b->grp) free_group(g);
+    free_allocs(b);
+    for (lx_sim *s : b->sims) s->batch = nullptr;
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->pre) (void)hipEventDestroy(b->pre);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
 
 lx_status lx_batch_add(lx_batch *b, lx_sim *s) {
-    if (!b || !s) return lx::fail(LX_EINVAL, "lx_batch_add: null pointer");
-    if (std::find(b->sims.begin(), b->sims.end(), s) != b->sims.end()) return lx::fail(LX_EINVAL, "already in batch");
+    if (!b || !s) return fail(LX_EINVAL, "lx_batch_add: null pointer");
+    if (s->batch) return fail(LX_EINVAL, "simulation already belongs to a batch");
+    if (s->ctx != b->ctx) return fail(LX_EINVAL, "simulation and batch use different contexts");
     b->sims.push_back(s);
+    s->batch = b;
+    b->dirty = true;
     return LX_OK;
 }
 
 lx_status lx_batch_remove(lx_batch *b, lx_sim *s) {
-    if (!b || !s) return lx::fail(LX_EINVAL, "lx_batch_remove: null pointer");
+    if (!b || !s) return fail(LX_EINVAL, "lx_batch_remove: null pointer");
     auto it = std::find(b->sims.begin(), b->sims.end(), s);
-    if (it == b->sims.end()) return lx::fail(LX_EINVAL, "not in batch");
+    if (it == b->sims.end()) return fail(LX_EINVAL, "not in batch");
+    HIP_TRY(hipStreamSynchronize(b->stream));
     b->sims.erase(it);
+    s->batch = nullptr;
+    b->dirty = true;
     return LX_OK;
 }
 
 lx_status lx_batch_step(lx_batch *b, int32_t nsteps) {
-    if (!b) return lx::fail(LX_EINVAL, "null batch");
+    if (!b || nsteps < 0) return fail(LX_EINVAL, "lx_batch_step: bad arguments");
+    if (nsteps == 0 || b->sims.empty()) return LX_OK;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    // probe buffers must hold the new steps before the descriptors are built
     for (lx_sim *s : b->sims) {
+        const int cap = s->pcap;
+        lx_status st = ensure_probe_cap(s, s->steps + nsteps);
+        if (st) return st;
+        if (s->pcap != cap) b->dirty = true;
+    }
+    if (b->dirty) {
+        lx_status st = rebuild(b);
+        if (st) return st;
+    }
+    // order after each member's pending work
+    for (Group &g : b->grp)
+        for (lx_sim *s : g.sims) {
+            HIP_TRY(hipEventRecord(b->pre, s->cs));
+            HIP_TRY(hipStreamWaitEvent(b->stream, b->pre, 0));
+        }
+    for (Group &g : b->grp) {
+        if (g.sims.empty()) continue;
+        int par = g.par;
+        int full = nsteps / kTT;
+        const int rem = nsteps % kTT;
+        if (full >= 2) {
+            lx_status st = ensure_pair_graph(b, g, par);
+            if (st) return st;
+            if (g.gexec[par])
+                for (; full >= 2; full -= 2) HIP_TRY(hipGraphLaunch(g.gexec[par], b->stream));
+        }
+        for (; full > 0; --full) {
+            enqueue_chunk(g, kTT, par, b->stream);
+            par ^= 1;
+        }
+        if (rem) {
+            enqueue_chunk(g, rem, par, b->stream);
+            par ^= 1;
+        }
+        lx_status st = check_launch();
+        if (st) return st;
+        if (par != g.par) {
+            // the current state moved to the other buffer of each member
+            for (lx_sim *s : g.sims) {
+                std::swap(s->F, s->F_alt);
+                std::swap(s->psi, s->psi_alt);
+                drop_graph(s);
+            }
+            g.par = par;
+        }
+        for (lx_sim *s : g.sims) s->steps += nsteps;
+    }
+    HIP_TRY(hipEventRecord(b->done, b->stream));
+    for (Group &g : b->grp)
+        for (lx_sim *s : g.sims) HIP_TRY(hipStreamWaitEvent(s->cs, b->done, 0));
+    for (lx_sim *s : b->others) {
         lx_status st = lx_sim_step(s, nsteps);
         if (st) return st;
     }
@@ -55,7 +372,9 @@ lx_status lx_batch_step(lx_batch *b, int32_t nsteps) {
 }
 
 lx_status lx_batch_sync(lx_batch *b) {
-    if (!b) return lx::fail(LX_EINVAL, "null batch");
+    if (!b) return fail(LX_EINVAL, "null batch");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
     for (lx_sim *s : b->sims) {
         lx_status st = lx_sim_sync(s);
         if (st) return st;

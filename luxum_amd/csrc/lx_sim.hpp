@@ -1,0 +1,82 @@
+// lx_sim.hpp -- host-side state of a simulation (shared by lx_api.cpp and lx_batch.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/luxum.h"
+#include "lx_internal.hpp"
+
+struct lx_ctx {
+    int device;
+    int64_t total_mem;
+};
+
+struct lx_sim {
+    // batch membership: alternate field/psi buffers for the ping-pong tiles
+    void *F_alt = nullptr;
+    float *psi_alt = nullptr;
+    lx_batch *batch = nullptr;
+    lx_ctx *ctx = nullptr;
+    lx_params p{};
+    int planar = 0, fp16 = 0, pml = 0;
+    int N[3] = {0, 0, 0};
+    int px = 0;
+    long long plane = 0;
+    size_t esize = 4;
+    // constants (maxwell.cc:63-84)
+    double eps0 = 0, mu0 = 0, c0 = 0, dt = 0, eta0 = 0, sigma_max = 0;
+    hipStream_t cs = nullptr, ss = nullptr;
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    void *F = nullptr;          // six components, component c at F + c*fstride elements
+    long long fstride = 0;
+    uint8_t *mat = nullptr;
+    float *cb = nullptr;
+    float *tab = nullptr;
+    float *psi = nullptr;
+    int *counter = nullptr;
+    int tn = 0;
+    int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0}, sw[3] = {0, 0, 0};
+    int64_t slab_cells[3] = {0, 0, 0};
+    lx::TfList tfH{}, tfE{};
+    int kz = 32;
+    lx::IncParams inc{};
+    lx::AuxLine aux{};
+    int use_aux = 0;
+    lx::PointSrc ps{};
+    // probes
+    int np = 0, pcap = 0;
+    long long *pidx = nullptr;
+    int *pcomp = nullptr;
+    double *prec = nullptr;
+    std::vector<int32_t> probe_host;
+    // snapshots
+    double *stage[2] = {nullptr, nullptr};
+    size_t stage_n[2] = {0, 0};
+    hipEvent_t ready[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
+    int snap_next = 0;
+    double *rd_stage = nullptr;
+    size_t rd_n = 0;
+    double *mon = nullptr;
+    // graphs
+    hipGraphExec_t gexec = nullptr;
+    int gK = 0;
+    int graphs = 1;
+    int steps = 0;
+    float cb_host[256];
+};
+
+
+namespace lx {
+// helpers implemented in lx_api.cpp
+template <typename T>
+Grid<T> make_grid(const lx_sim *s);
+lx_status ensure_probe_cap(lx_sim *s, int need);
+lx_status check_launch();
+lx_status dalloc(lx_sim *s, void **out, size_t bytes);
+size_t psi_total(const lx_sim *s);
+ProbeArgs probe_args(const lx_sim *s);
+void drop_graph(lx_sim *s);
+}  // namespace lx

@@ -1,0 +1,143 @@
+"""Batched simulations: the temporally tiled 2D kernel over a flattened tile
+list (several planar simulations of uneven sizes in one launch) against the
+single-simulation kernels and the FP64 oracle.
+
+Tolerance: relative L2 <= 1e-5 (FP32) of the E and H fields and of the probe
+time series; FP16 storage <= 2e-2 against the oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from luxum_amd import configs
+from luxum_amd.params import LX_EY, LX_FP16, LX_SRC_SOFT, default_params
+from oracle.lxoracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+TOL32 = 1e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def field_err(a, b):
+    """Relative L2 of the E and H vector fields of two simulations."""
+    out = {}
+    comps = a.p.components()
+    for name, group in (("E", [c for c in comps if c < 3]), ("H", [c for c in comps if c >= 3])):
+        num = den = 0.0
+        for c in group:
+            x, y = a.field(c), b.field(c)
+            assert x.shape == y.shape and np.all(np.isfinite(x))
+            num += float(np.sum((x - y) ** 2))
+            den += float(np.sum(y ** 2))
+        out[name] = (num / den) ** 0.5 if den > 0 else num ** 0.5
+    return max(out.values())
+
+
+def make(w, ctx):
+    from luxum_amd.sim import Simulation
+    s = Simulation(w.p, ctx)
+    s.set_materials(w.ids, w.eps_r)
+    if w.probes is not None and len(w.probes):
+        s.set_probes(w.probes)
+    return s
+
+
+def phc(n, seed, precision=None):
+    w = configs.phc2d(n, seed, steps=400) if precision is None else configs.phc2d(n, seed, steps=400, precision=precision)
+    w.p.pw_delay = 2.0   # pulse centre at step ~144
+    return w
+
+
+def soft_2d():
+    p = default_params(is_planar=1, nx=96, ny=1, nz=90, dy=0.0, use_pml=1, pml_thickness=8 * 5e-8, use_ps=1,
+                       src_kind=LX_SRC_SOFT, src_comp=LX_EY, src_amp=1.0, src_lambda0=20 * 5e-8,
+                       src_fwhm=3.0, src_delay=2.0, src_pos=(40, 0, 45), num_timesteps=400)
+    return configs.Workload("2D soft source", p, np.zeros((96, 1, 90), np.uint8), np.array([1.0]),
+                            np.array([[LX_EY, 50, 0, 45]], np.int32))
+
+
+def test_batch_matches_single_and_oracle(gpu_ctx):
+    from luxum_amd.sim import Batch
+    ws = [phc(128, 1), phc(176, 2), phc(208, 3), phc(256, 4), soft_2d(),
+          configs.config1(n=32, steps=400, pml=6, margin=4)]
+    b = Batch(gpu_ctx)
+    members = [make(w, gpu_ctx) for w in ws]
+    for s in members:
+        b.add(s)
+    say("batch: 6 members (4 tiled 2D, 1 point-source 2D, 1 3D), steps 37 + 200")
+    b.step(37)
+    b.step(200)
+    b.sync()
+    for w, s in zip(ws, members):
+        ref = make(w, gpu_ctx)
+        ref.step(237)
+        ref.sync()
+        e = field_err(s, ref)
+        pe = rel_l2(s.probes(), ref.probes())
+        say(f"{w.name}: batch vs single fields {e:.2e} probes {pe:.2e}")
+        assert e <= TOL32 and pe <= TOL32
+        assert s.info().step == 237
+    o = OracleSim(ws[0].p)
+    o.set_materials(ws[0].ids, ws[0].eps_r)
+    o.set_probes(ws[0].probes)
+    o.step(237)
+    e = field_err(members[0], o)
+    pe = rel_l2(members[0].probes(), o.probes())
+    say(f"{ws[0].name}: batch vs oracle fields {e:.2e} probes {pe:.2e}")
+    assert e <= TOL32 and pe <= TOL32
+    b.close()
+
+
+def test_batch_interleaved_with_single_steps(gpu_ctx):
+    """Members keep stepping alone between batch steps and after removal; the
+    ping-pong buffers stay consistent (odd chunk counts, graph replay from
+    either parity)."""
+    from luxum_amd.sim import Batch
+    ws = [phc(160, 7), phc(144, 8)]
+    members = [make(w, gpu_ctx) for w in ws]
+    b = Batch(gpu_ctx)
+    for s in members:
+        b.add(s)
+    say("interleave: batch 8, single 5, batch 40, remove, single 11, batch 19")
+    b.step(8)
+    members[0].step(5)
+    b.step(40)
+    b.remove(members[1])
+    members[1].step(11)
+    b.step(19)
+    b.sync()
+    counts = [8 + 5 + 40 + 19, 8 + 40 + 11]
+    for w, s, n in zip(ws, members, counts):
+        ref = make(w, gpu_ctx)
+        ref.step(n)
+        ref.sync()
+        e = field_err(s, ref)
+        pe = rel_l2(s.probes(), ref.probes())
+        say(f"{w.name}: {n} steps, batch/single mix vs single fields {e:.2e} probes {pe:.2e}")
+        assert e <= TOL32 and pe <= TOL32
+    b.close()
+
+
+def test_batch_fp16_bound(gpu_ctx):
+    from luxum_amd.sim import Batch
+    w = phc(192, 5, precision=LX_FP16)
+    s = make(w, gpu_ctx)
+    b = Batch(gpu_ctx)
+    b.add(s)
+    b.step(240)
+    b.sync()
+    o = OracleSim(w.p)
+    o.set_materials(w.ids, w.eps_r)
+    o.set_probes(w.probes)
+    o.step(240)
+    e = field_err(s, o)
+    pe = rel_l2(s.probes(), o.probes())
+    say(f"fp16 batch vs oracle fields {e:.2e} probes {pe:.2e}")
+    assert e <= 2e-2 and pe <= 2e-2
+    b.close()
