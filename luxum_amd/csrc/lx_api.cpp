@@ -190,6 +190,18 @@ lx_status build_tables(lx_sim *s) {
             s->shi[a] = hi;
             s->sw[a] = lo + (N - hi);
         }
+        // widest index range around the centre, outside the slabs, whose
+        // coefficients equal the centre's and carry no CPML term
+        const int mid = N / 2;
+        auto same = [&](int i) { return kh[i] == kh[mid] && ke[i] == ke[mid] && chh[i] == 0.f && ce[i] == 0.f; };
+        int ul = mid, uh = mid;
+        if (same(mid)) {
+            uh = mid + 1;
+            while (ul > 0 && same(ul - 1)) --ul;
+            while (uh < N && same(uh)) ++uh;
+        }
+        s->ulo[a] = std::max(ul, s->slo[a]);
+        s->uhi[a] = std::max(s->ulo[a], std::min(uh, s->shi[a]));
     }
     void *p = nullptr;
     lx_status st = dalloc(s, &p, host.size() * sizeof(float));

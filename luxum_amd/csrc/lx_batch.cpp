@@ -1,10 +1,11 @@
 // lx_batch.cpp -- batches of independent simulations.
 //
-// Planar members that the temporally tiled kernel supports (CPML slabs of at
-// most 24 cells, no point source, aux-1D or no plane wave, both sides at least
-// 128 cells) advance together: per chunk of up to 8 steps, one launch of
-// k_aux_chunk (incident line of every member), one launch of k_tile2d over
-// the flattened tile list of every member, one launch of k_batch_finish.
+// Planar members that the temporally tiled kernel supports (no point source,
+// aux-1D or no plane wave, at most 256 probes) advance together: per chunk of up to 8
+// steps, one launch of k_aux_chunk (incident line of every member), two
+// launches of k_tile2d over the flattened tile list of every member (jobs
+// whose region touches a CPML slab, then the rest), one launch of
+// k_batch_finish.
 // Pairs of full chunks are captured once as a hipGraph and replayed.  Other
 // members step on their own streams with the single-simulation kernels.
 #include <hip/hip_runtime.h>
@@ -35,7 +36,7 @@ struct Group {
     BSim *d_sims = nullptr;
     BJob *d_jobs = nullptr;
     BAux *d_aux = nullptr;
-    int njobs = 0, naux = 0;
+    int nouter = 0, ninner = 0, naux = 0;
     int par = 0;                              // buffer index holding the current state
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // two chunks starting at parity 0 / 1
 };
@@ -45,8 +46,8 @@ struct Group {
 struct lx_batch {
     lx_ctx *ctx = nullptr;
     std::vector<lx_sim *> sims;
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr, pre = nullptr;
+    hipStream_t stream = nullptr, side = nullptr;   // general-path jobs run on `side`, concurrently
+    hipEvent_t done = nullptr, pre = nullptr, fork = nullptr, join = nullptr;
     bool dirty = true;
     Group grp[2];                   // FP32 and FP16 storage
     std::vector<lx_sim *> others;
@@ -60,14 +61,17 @@ bool tiled_ok(const lx_sim *s) {
     const lx_params &p = s->p;
     if (!p.is_planar || p.use_ps) return false;
     if (p.use_pw && p.incident != LX_INC_AUX1D) return false;
-    if (s->N[0] < 128 || s->N[2] < 128) return false;
-    if (s->pml) {
-        if (s->slo[0] > kPSW || s->N[0] - s->shi[0] > kPSW) return false;
-        if (s->slo[2] > kPSW || s->N[2] - s->shi[2] > kPSW) return false;
-    }
     if (s->np > 256) return false;
     if (s->tfH.n > 4 || s->tfE.n > 4) return false;
     return true;
+}
+
+// is the region of the job starting at (i0, k0) inside [1, N-2] on both
+// axes and inside the uniform-coefficient range (no slab, one z coefficient)?
+bool job_inner(const lx_sim *s, int i0, int k0) {
+    const int x0 = i0 - kTT, x1 = x0 + 64, z0 = k0 - kTT, z1 = z0 + kRH;
+    return x0 >= std::max(1, s->ulo[0]) && x1 <= std::min(s->N[0] - 1, s->uhi[0]) && z0 >= std::max(1, s->ulo[2]) &&
+           z1 <= std::min(s->N[2] - 1, s->uhi[2]);
 }
 
 void free_group(Group &g) {
@@ -80,7 +84,7 @@ void free_group(Group &g) {
     g.d_sims = nullptr;
     g.d_jobs = nullptr;
     g.d_aux = nullptr;
-    g.njobs = g.naux = 0;
+    g.nouter = g.ninner = g.naux = 0;
 }
 
 void free_allocs(lx_batch *b) {
@@ -130,7 +134,7 @@ lx_status rebuild(lx_batch *b) {
     for (Group &g : b->grp) {
         if (g.sims.empty()) continue;
         std::vector<BSim> hs(g.sims.size());
-        std::vector<BJob> jobs;
+        std::vector<BJob> jobs, ojobs;
         std::vector<BAux> aux;
         for (size_t q = 0; q < g.sims.size(); ++q) {
             lx_sim *s = g.sims[q];
@@ -198,13 +202,22 @@ lx_status rebuild(lx_batch *b) {
             d.prec = s->prec;
             d.pcap = s->pcap;
             d.hscale = 1.0 / s->eta0;
-            for (int tz = 0; tz < (s->N[2] + kTW - 1) / kTW; ++tz)
-                for (int tx = 0; tx < (s->N[0] + kTW - 1) / kTW; ++tx) jobs.push_back(BJob{(int)q, tx * kTW, tz * kTW});
+            for (int tz = 0; tz < (s->N[2] + kTH - 1) / kTH; ++tz)
+                for (int tx = 0; tx < (s->N[0] + kTW - 1) / kTW; ++tx) {
+                    const BJob j{(int)q, tx * kTW, tz * kTH};
+                    (job_inner(s, j.i0, j.k0) ? jobs : ojobs).push_back(j);
+                }
         }
         // large simulations first: long jobs start early, small ones fill the tail
-        std::stable_sort(jobs.begin(), jobs.end(), [&](const BJob &x, const BJob &y) {
+        auto larger = [&](const BJob &x, const BJob &y) {
             return g.sims[x.sim]->N[0] * g.sims[x.sim]->N[2] > g.sims[y.sim]->N[0] * g.sims[y.sim]->N[2];
-        });
+        };
+        std::stable_sort(ojobs.begin(), ojobs.end(), larger);
+        std::stable_sort(jobs.begin(), jobs.end(), larger);
+        g.nouter = (int)ojobs.size();
+        g.ninner = (int)jobs.size();
+        ojobs.insert(ojobs.end(), jobs.begin(), jobs.end());
+        jobs.swap(ojobs);
         void *p = nullptr;
         lx_status st = balloc(b, &p, sizeof(BSim) * hs.size());
         if (st) return st;
@@ -214,7 +227,6 @@ lx_status rebuild(lx_batch *b) {
         if (st) return st;
         g.d_jobs = (BJob *)p;
         HIP_TRY(hipMemcpy(g.d_jobs, jobs.data(), sizeof(BJob) * jobs.size(), hipMemcpyHostToDevice));
-        g.njobs = (int)jobs.size();
         if (!aux.empty()) {
             st = balloc(b, &p, sizeof(BAux) * aux.size());
             if (st) return st;
@@ -227,10 +239,28 @@ lx_status rebuild(lx_batch *b) {
     return LX_OK;
 }
 
-void enqueue_chunk(const Group &g, int nt, int par, hipStream_t s) {
+// One chunk: incident lines, then the general-path jobs (side stream) and the
+// inner jobs (main stream) concurrently, then the step counters.
+void enqueue_chunk(const lx_batch *b, const Group &g, int nt, int par) {
+    hipStream_t s = b->stream;
     if (g.naux) launch_aux_chunk(g.d_aux, g.naux, nt, s);
-    if (g.fp16) launch_tile2d<__half>(g.d_sims, g.d_jobs, g.njobs, nt, par, s);
-    else launch_tile2d<float>(g.d_sims, g.d_jobs, g.njobs, nt, par, s);
+    const bool fork = g.nouter && g.ninner;
+    if (fork) {
+        (void)hipEventRecord(b->fork, s);
+        (void)hipStreamWaitEvent(b->side, b->fork, 0);
+    }
+    hipStream_t so = fork ? b->side : s;
+    if (g.fp16) {
+        launch_tile2d<__half>(g.d_sims, g.d_jobs, g.nouter, false, nt, par, so);
+        launch_tile2d<__half>(g.d_sims, g.d_jobs + g.nouter, g.ninner, true, nt, par, s);
+    } else {
+        launch_tile2d<float>(g.d_sims, g.d_jobs, g.nouter, false, nt, par, so);
+        launch_tile2d<float>(g.d_sims, g.d_jobs + g.nouter, g.ninner, true, nt, par, s);
+    }
+    if (fork) {
+        (void)hipEventRecord(b->join, b->side);
+        (void)hipStreamWaitEvent(s, b->join, 0);
+    }
     launch_batch_finish(g.d_sims, (int)g.sims.size(), nt, s);
 }
 
@@ -238,8 +268,8 @@ lx_status ensure_pair_graph(lx_batch *b, Group &g, int par) {
     if (g.gexec[par] || !b->graphs) return LX_OK;
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_chunk(g, kTT, par, b->stream);
-    enqueue_chunk(g, kTT, par ^ 1, b->stream);
+    enqueue_chunk(b, g, kTT, par);
+    enqueue_chunk(b, g, kTT, par ^ 1);
     hipError_t e = hipStreamEndCapture(b->stream, &graph);
     if (e != hipSuccess || !graph) {
         (void)hipGetLastError();
@@ -266,9 +296,13 @@ lx_status lx_batch_create(lx_ctx *ctx, lx_batch **out) {
     if (!b) return fail(LX_ENOMEM, "out of host memory");
     b->ctx = ctx;
     if (hipSetDevice(ctx->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->pre, hipEventDisableTiming) != hipSuccess) {
-        delete b;
+        hipEventCreateWithFlags(&b->pre, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        lx_batch_destroy(b);
         return fail(LX_EDEVICE, "batch stream creation failed");
     }
     *out = b;
@@ -283,6 +317,9 @@ void lx_batch_destroy(lx_batch *b) {
     for (lx_sim *s : b->sims) s->batch = nullptr;
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->pre) (void)hipEventDestroy(b->pre);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->side) (void)hipStreamDestroy(b->side);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -341,11 +378,11 @@ lx_status lx_batch_step(lx_batch *b, int32_t nsteps) {
                 for (; full >= 2; full -= 2) HIP_TRY(hipGraphLaunch(g.gexec[par], b->stream));
         }
         for (; full > 0; --full) {
-            enqueue_chunk(g, kTT, par, b->stream);
+            enqueue_chunk(b, g, kTT, par);
             par ^= 1;
         }
         if (rem) {
-            enqueue_chunk(g, rem, par, b->stream);
+            enqueue_chunk(b, g, rem, par);
             par ^= 1;
         }
         lx_status st = check_launch();

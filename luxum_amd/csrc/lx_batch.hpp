@@ -8,11 +8,11 @@
 
 namespace lx {
 
-constexpr int kTW = 48;              // tile interior (cells per side)
 constexpr int kTT = 8;               // timesteps per chunk = halo width
-constexpr int kRW = kTW + 2 * kTT;   // region side in LDS (64)
-constexpr int kLP = kRW + 1;         // LDS row pitch
-constexpr int kPSW = 24;             // max CPML slab cells of one region per axis
+constexpr int kNW = 4;               // wavefronts per block, 16 region rows each
+constexpr int kRH = 16 * kNW;        // region rows (z)
+constexpr int kTW = 64 - 2 * kTT;    // tile interior columns (x): region of 64 columns, one per lane
+constexpr int kTH = kRH - 2 * kTT;   // tile interior rows (z)
 
 // One planar simulation of a batch.
 struct BSim {
@@ -55,8 +55,10 @@ struct BAux {
     int *counter;
 };
 
+// inner: every job's region lies inside the domain, outside the CPML slabs,
+// on rows with one z coefficient; otherwise the general path
 template <typename T>
-void launch_tile2d(const BSim *sims, const BJob *jobs, int njobs, int nt, int par, hipStream_t s);
+void launch_tile2d(const BSim *sims, const BJob *jobs, int njobs, bool inner, int nt, int par, hipStream_t s);
 void launch_aux_chunk(const BAux *aux, int naux, int nt, hipStream_t s);
 void launch_batch_finish(const BSim *sims, int nsims, int nt, hipStream_t s);
 

@@ -1,15 +1,15 @@
 // lx_batch_kernels.hip -- batched 2D (x-z plane, Ey Hx Hz) timesteps with
-// temporal tiling in LDS.
+// temporal tiling.
 //
 // Many planar simulations of uneven sizes advance together: one launch covers
 // a flattened list of tile jobs from every simulation.  A job is a 48 x 48
 // tile of one simulation; its block loads the tile plus an 8-cell halo
-// (64 x 64 cells of Ey, Hx, Hz, material ids and the CPML psi of slab cells)
-// into LDS, advances up to 8 timesteps there, and writes the 48 x 48 interior
-// back.  After s steps the cells within s of the halo edge are stale (the
-// Yee dependency cone grows one cell per half step), so the interior is exact
-// after 8.  Neighbouring jobs read the halo of step t while others write step
-// t+8, so fields and psi ping-pong between two buffers.
+// (a 64 x 64 region of Ey, Hx, Hz and the cell's material coefficient) into
+// registers, advances up to 8 timesteps there, and writes the 48 x 48
+// interior back.  After s steps the cells within s of the region edge are
+// stale (the Yee dependency cone grows one cell per half step), so the
+// interior is exact after 8.  Neighbouring jobs read the halo of step t while
+// others write step t+8, so fields and psi ping-pong between two buffers.
 //
 // Per step the block does what the single-simulation kernels do
 // (maxwell_two_dim_pml.cc:144-176): H update (+psi), TF/SF face corrections
@@ -17,8 +17,8 @@
 // probe records.  The incident values of the aux-1D line for the chunk's 8
 // steps are precomputed per simulation by k_aux_chunk.
 //
-// HBM traffic per cell-step is ~(64^2*13 + 48^2*12)/(48^2*8) = 4.4 bytes,
-// against 37 for the half-step kernels.
+// HBM traffic per cell-step is (64^2*13 + 48^2*12)/(48^2*8) = 4.4 bytes
+// (FP32), against 37 for the half-step kernels.
 #include "lx_batch.hpp"
 #include "../../include/luxum.h"
 
@@ -76,272 +76,367 @@ __global__ void k_batch_finish(const BSim *__restrict__ sims, int nsims, int nt)
     if (q < nsims) sims[q].counter[0] += nt;
 }
 
-// incident value of a TF/SF box cell at region-independent global (i, k)
-__device__ __forceinline__ double snap_value(const BSim &S, const TfBox &b, int i, int k, const double *snap) {
+// Incident line values of a region: for each step of the chunk, the line
+// nodes from 2 before to 2 after the region along the incidence axis (box
+// node offsets are within half a cell, so every face cell of the region
+// reads inside this window).
+constexpr int kGW = (kRH > 64 ? kRH : 64) + 4;
+
+// incident value of a TF/SF box cell at global (i, k) from the region's window
+__device__ __forceinline__ double snap_value(const BSim &S, const TfBox &b, int i, int k, const double *win_s,
+                                             int mbase) {
     const double cc = S.inc_axis == 0 ? i + (double)b.off[0] : k + (double)b.off[2];
-    const int m = (int)floor(cc) - S.inc_x0;
-    return (m >= 0 && m < S.win) ? snap[m] : 0.0;
+    const int j = (int)floor(cc) - S.inc_x0 - mbase;
+    return win_s[j];
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_tile2d(const BSim *__restrict__ sims, const BJob *__restrict__ jobs, int nt,
-                                                int par) {
-    __shared__ float sEy[kRW * kLP], sHx[kRW * kLP], sHz[kRW * kLP];
-    __shared__ uint8_t sid[kRW * kRW];
+// DPP wavefront shifts: lane l receives the value of lane l+1 (next) or
+// lane l-1 (prev); the lane without a source receives 0.
+__device__ __forceinline__ float wave_next(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float wave_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+
+__device__ __forceinline__ float rdlane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Region of a job: 64 columns (one per lane) x kRH rows; wave w owns rows
+// 16w .. 16w+15.  Fields Ey, Hx, Hz, the cell's cb and (general path) the
+// CPML psi stay in VGPRs for the whole chunk; x coefficients are per lane,
+// z coefficients per row.  Per half step the waves exchange one boundary row
+// through LDS (Ey row 16w before the H update, Hx row 16w+15 before the E
+// update); x neighbours come from DPP wavefront shifts.
+//
+// No cell update is masked and no psi update is conditional: the CPML terms
+// run on every cell with c = 0 (psi stays 0 and adds nothing) off the slabs,
+// which is exactly the single-simulation result there.  Cells that the
+// reference never updates (the PEC walls: E on the first and last index of an
+// axis, H on the last) and cells outside the domain get zero coefficients:
+// cb = 0 for such E cells, kh = b = c = 0 for such H rows and columns,
+// b = c = 0 for E psi, so fields and psi keep their values.  Cells near the
+// region edge, whose neighbours lie outside the region, take finite garbage
+// values; they are stale either way and never written back.
+//
+// INNER jobs have their region inside [1, N-2] on both axes, outside the
+// CPML slabs and on rows with one z coefficient (the host checks): no psi,
+// one z coefficient per wave.
+template <typename T, bool INNER>
+__global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ sims, const BJob *__restrict__ jobs,
+                                                     int nt, int par) {
+    constexpr int NT = 64 * kNW;
     __shared__ float scb[256];
-    __shared__ float spx[2][kRW * kPSW];   // x-slab psi: [0] Hz (H slot0), [1] Ey (E slot1); [rk][col]
-    __shared__ float spz[2][kPSW * kRW];   // z-slab psi: [0] Hx (H slot1), [1] Ey (E slot0); [row][ri]
+    __shared__ float sEtop[kNW][64], sHbot[kNW][64];
+    __shared__ int sprb[256];                 // probe index
+    __shared__ int sprd[256];                 // packed component << 16 | region row << 8 | region column
+    __shared__ int snprb;
+    __shared__ float4 szh[INNER ? 1 : kRH], sze[INNER ? 1 : kRH];   // z {kh, b_h, c_h}, {ke, b_e, c_e} per row
+    __shared__ double sg[2][kTT][kGW];        // incident E (before H half step) / H (before E half step)
 
     const BJob job = jobs[blockIdx.x];
     const BSim &S = sims[job.sim];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nx = S.nx, nz = S.nz;
     const int gi0 = job.i0 - kTT, gk0 = job.k0 - kTT;   // region origin
-    const T *Fin = (const T *)S.F[par];
-    T *Fout = (T *)S.F[par ^ 1];
-    const float *psin = S.psi[par];
-    float *psout = S.psi[par ^ 1];
+    const int i = gi0 + lane;
+    const int kb = gk0 + 16 * w;                        // first row of this wave
     const long long px = S.px, fs = S.fstride;
     const float *tab = S.tab;
     const int tn = S.tn;
-    const bool pml = S.pml;
-    // x-slab columns and z-slab rows of the region (one contiguous range each)
-    int xs0 = 0, xs1 = 0, zs0 = 0, zs1 = 0;
-    if (pml) {
-        const int lo0 = max(0, -gi0), hi0 = min(kRW, nx - gi0);
-        int a = min(hi0, max(lo0, S.slo[0] - gi0)), b = max(lo0, min(hi0, S.shi[0] - gi0));
-        if (a > lo0) { xs0 = lo0; xs1 = a; } else if (b < hi0) { xs0 = b; xs1 = hi0; }
-        const int lz = max(0, -gk0), hz = min(kRW, nz - gk0);
-        int c = min(hz, max(lz, S.slo[2] - gk0)), d = max(lz, min(hz, S.shi[2] - gk0));
-        if (c > lz) { zs0 = lz; zs1 = c; } else if (d < hz) { zs0 = d; zs1 = hz; }
-    }
-    const float *px0 = psin, *pz0 = psin + 4 * (S.scells[0] + S.scells[1]);
-    float *qx0 = psout, *qz0 = psout + 4 * (S.scells[0] + S.scells[1]);
+    const T *Fin = (const T *)S.F[par];
+    T *Fout = (T *)S.F[par ^ 1];
+    const bool colin = i >= 0 && i < nx;
+    const bool pml = !INNER && S.pml;
 
-    // ---- load the region
-    scb[tid] = S.cb[tid];
-    for (int q = tid; q < kRW * kRW; q += 256) {
-        const int ri = q & (kRW - 1), rk = q >> 6;
-        const int i = gi0 + ri, k = gk0 + rk;
-        float ey = 0.f, hx = 0.f, hz = 0.f;
-        uint8_t id = 0;
-        if (i >= 0 && i < nx && k >= 0 && k < nz) {
-            const long long o = (long long)k * px + i;
-            ey = bld(Fin + fs + o);
-            hx = bld(Fin + 3 * fs + o);
-            hz = bld(Fin + 5 * fs + o);
-            id = S.mat[o];
+    for (int q = tid; q < 256; q += NT) scb[q] = S.cb[q];
+    if (tid == 0) snprb = 0;
+    if (!INNER) {
+        for (int q = tid; q < kRH; q += NT) {
+            const int k = gk0 + q, kc = min(max(k, 0), tn - 1);
+            const bool hok = k >= 0 && k <= nz - 2, eok = k >= 1 && k <= nz - 2;
+            szh[q] = hok ? make_float4(tab[(2 * 6 + TK_KH) * tn + kc], tab[(2 * 6 + TK_BH) * tn + kc],
+                                       tab[(2 * 6 + TK_CH) * tn + kc], 0.f)
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
+            sze[q] = make_float4(tab[(2 * 6 + TK_KE) * tn + kc], eok ? tab[(2 * 6 + TK_BE) * tn + kc] : 0.f,
+                                 eok ? tab[(2 * 6 + TK_CE) * tn + kc] : 0.f, 0.f);
         }
-        sEy[rk * kLP + ri] = ey;
-        sHx[rk * kLP + ri] = hx;
-        sHz[rk * kLP + ri] = hz;
-        sid[q] = id;
     }
-    if (pml) {
-        for (int q = tid; q < kRW * kPSW; q += 256) {
-            const int rk = q / kPSW, col = q % kPSW;
-            const int ri = xs0 + col, i = gi0 + ri, k = gk0 + rk;
-            float a = 0.f, b = 0.f;
-            if (ri < xs1 && k >= 0 && k < nz) {
-                const long long off = (long long)k * S.sw[0] + bslab(i, S.slo[0], S.shi[0]);
-                a = px0[off];                          // H slot0 (Hz)
-                b = px0[3 * S.scells[0] + off];        // E slot1 (Ey)
-            }
-            spx[0][q] = a;
-            spx[1][q] = b;
-        }
-        for (int q = tid; q < kPSW * kRW; q += 256) {
-            const int row = q >> 6, ri = q & (kRW - 1);
-            const int rk = zs0 + row, i = gi0 + ri, k = gk0 + rk;
-            float a = 0.f, b = 0.f;
-            if (rk < zs1 && i >= 0 && i < nx) {
-                const long long off = (long long)bslab(k, S.slo[2], S.shi[2]) * px + i;
-                a = pz0[S.scells[2] + off];            // H slot1 (Hx)
-                b = pz0[2 * S.scells[2] + off];        // E slot0 (Ey)
-            }
-            spz[0][q] = a;
-            spz[1][q] = b;
+    // incident line window of the region, when a TF/SF box touches it
+    bool tf_any = false;
+    for (int bq = 0; bq < S.ntfH; ++bq) {
+        const TfBox &bx = S.tfH[bq];
+        tf_any |= bx.hi[0] >= gi0 && bx.lo[0] < gi0 + 64 && bx.hi[2] >= gk0 && bx.lo[2] < gk0 + kRH;
+    }
+    for (int bq = 0; bq < S.ntfE; ++bq) {
+        const TfBox &bx = S.tfE[bq];
+        tf_any |= bx.hi[0] >= gi0 && bx.lo[0] < gi0 + 64 && bx.hi[2] >= gk0 && bx.lo[2] < gk0 + kRH;
+    }
+    const int mbase = (S.inc_axis == 0 ? gi0 : gk0) - 2 - S.inc_x0;
+    if (tf_any) {
+        for (int q = tid; q < nt * kGW; q += NT) {
+            const int st = q / kGW, j = q % kGW, m = mbase + j;
+            const bool in = m >= 0 && m < S.win;
+            sg[0][st][j] = in ? S.esnap[(long long)st * S.win + m] : 0.0;
+            sg[1][st][j] = in ? S.hsnap[(long long)st * S.win + m] : 0.0;
         }
     }
     __syncthreads();
+    // probes on interior cells of this job
+    for (int q = tid; q < S.np; q += NT) {
+        const long long o = S.pidx[q];
+        const int rI = (int)(o % px) - gi0, rK = (int)(o / px) - gk0;
+        if (rI >= kTT && rI < 64 - kTT && rK >= kTT && rK < kRH - kTT) {
+            const int slot = atomicAdd(&snprb, 1);
+            sprb[slot] = q;
+            sprd[slot] = S.pcomp[q] << 16 | rK << 8 | rI;
+        }
+    }
 
-    const int ri = tid & (kRW - 1);
-    const int i = gi0 + ri;
+    // coefficients: x per lane, zeroed where the cell is never updated
+    const int ic = min(max(i, 0), tn - 1);
+    const bool hxok = INNER || (i >= 0 && i <= nx - 2), exok = INNER || (i >= 1 && i <= nx - 2);
+    const float khx = hxok ? tab[(0 * 6 + TK_KH) * tn + ic] : 0.f, kex = tab[(0 * 6 + TK_KE) * tn + ic];
+    float bhx = 0.f, chx = 0.f, bex = 0.f, cex = 0.f;
+    if (!INNER) {
+        bhx = hxok ? tab[(0 * 6 + TK_BH) * tn + ic] : 0.f;
+        chx = hxok ? tab[(0 * 6 + TK_CH) * tn + ic] : 0.f;
+        bex = exok ? tab[(0 * 6 + TK_BE) * tn + ic] : 0.f;
+        cex = exok ? tab[(0 * 6 + TK_CE) * tn + ic] : 0.f;
+    }
+    const int kc0 = min(max(kb, 0), tn - 1);
+    const float khz0 = tab[(2 * 6 + TK_KH) * tn + kc0], kez0 = tab[(2 * 6 + TK_KE) * tn + kc0];
+
+    // fields, cb and psi of this lane's 16 cells
+    const int slo0 = S.slo[0], shi0 = S.shi[0], slo2 = S.slo[2], shi2 = S.shi[2];
+    const bool xslab = pml && colin && (i < slo0 || i >= shi0);
+    const float *px0 = S.psi[par], *pz0 = S.psi[par] + 4 * (S.scells[0] + S.scells[1]);
+    float ey[16], hx[16], hz[16], cb[16];
+    float pxh[16], pxe[16], pzh[16], pze[16];   // x psi (Hz, Ey), z psi (Hx, Ey); unused when INNER
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int k = kb + m;
+        ey[m] = hx[m] = hz[m] = cb[m] = 0.f;
+        pxh[m] = pxe[m] = pzh[m] = pze[m] = 0.f;
+        if (INNER || (colin && k >= 0 && k < nz)) {
+            const long long o = (long long)k * px + i;
+            ey[m] = bld(Fin + fs + o);
+            hx[m] = bld(Fin + 3 * fs + o);
+            hz[m] = bld(Fin + 5 * fs + o);
+            cb[m] = scb[S.mat[o]];
+            if (!INNER && !(exok && k >= 1 && k <= nz - 2)) cb[m] = 0.f;   // PEC wall
+            if (!INNER && pml) {
+                if (xslab) {
+                    const long long off = (long long)k * S.sw[0] + bslab(i, slo0, shi0);
+                    pxh[m] = px0[off];                         // H slot0 (Hz)
+                    pxe[m] = px0[3 * S.scells[0] + off];       // E slot1 (Ey)
+                }
+                if (k < slo2 || k >= shi2) {
+                    const long long off = (long long)bslab(k, slo2, shi2) * px + i;
+                    pzh[m] = pz0[S.scells[2] + off];           // H slot1 (Hx)
+                    pze[m] = pz0[2 * S.scells[2] + off];       // E slot0 (Ey)
+                }
+            }
+        }
+    }
+    __syncthreads();   // probe list complete
+    const int nprb = snprb;
+    // TF/SF boxes that touch this wave's 16 x 64 cells (wave-uniform bit masks)
+    unsigned hmask = 0, emask = 0;
+    for (int bq = 0; bq < S.ntfH; ++bq) {
+        const TfBox &bx = S.tfH[bq];
+        if (bx.hi[0] >= gi0 && bx.lo[0] < gi0 + 64 && bx.hi[2] >= kb && bx.lo[2] < kb + 16) hmask |= 1u << bq;
+    }
+    for (int bq = 0; bq < S.ntfE; ++bq) {
+        const TfBox &bx = S.tfE[bq];
+        if (bx.hi[0] >= gi0 && bx.lo[0] < gi0 + 64 && bx.hi[2] >= kb && bx.lo[2] < kb + 16) emask |= 1u << bq;
+    }
+    hmask = __builtin_amdgcn_readfirstlane(hmask);
+    emask = __builtin_amdgcn_readfirstlane(emask);
+    const int pcap = S.pcap, np = S.np;
+    double *prec = S.prec;
+
     const int t0 = S.counter[0];
     for (int s = 0; s < nt; ++s) {
         const int t = t0 + s;
-        // ---- H half step
-#pragma unroll 4
-        for (int m = 0; m < kRW / 4; ++m) {
-            const int rk = (tid >> 6) + 4 * m, k = gk0 + rk;
-            const float e0 = sEy[rk * kLP + ri];
-            if (rk + 1 < kRW && i >= 0 && i < nx && k >= 0 && k < nz - 1) {   // Hx
-                const float d2 = sEy[(rk + 1) * kLP + ri] - e0;
-                float acc = -d2 * tab[(2 * 6 + TK_KH) * tn + k];
-                if (pml && rk >= zs0 && rk < zs1) {
-                    float &p = spz[0][(rk - zs0) * kRW + ri];
-                    p = tab[(2 * 6 + TK_BH) * tn + k] * p + tab[(2 * 6 + TK_CH) * tn + k] * d2;
-                    acc -= p;
-                }
-                sHx[rk * kLP + ri] -= acc;
-            }
-            if (ri + 1 < kRW && i >= 0 && i < nx - 1 && k >= 0 && k < nz) {   // Hz
-                const float d1 = sEy[rk * kLP + ri + 1] - e0;
-                float acc = d1 * tab[(0 * 6 + TK_KH) * tn + i];
-                if (pml && ri >= xs0 && ri < xs1) {
-                    float &p = spx[0][rk * kPSW + (ri - xs0)];
-                    p = tab[(0 * 6 + TK_BH) * tn + i] * p + tab[(0 * 6 + TK_CH) * tn + i] * d1;
-                    acc += p;
-                }
-                sHz[rk * kLP + ri] -= acc;
-            }
-        }
+        // ---------------- H half step
+        sEtop[w][lane] = ey[0];
         __syncthreads();
-        // TF/SF corrections of the H half step: box b = tid/64, cell along the in-plane transverse axis
-        if (tid < 64 * S.ntfH) {
-            const TfBox &bx = S.tfH[tid >> 6];
-            const int u = tid & 63;
-            const int fa = bx.fa;                 // 0 (x-face) or 2 (z-face)
-            int gi, gk;
-            if (fa == 0) { gi = bx.lo[0]; gk = gk0 + u; } else { gk = bx.lo[2]; gi = gi0 + u; }
-            const int rI = gi - gi0, rK = gk - gk0;
-            if (rI >= 0 && rI < kRW && rK >= 0 && rK < kRW && gi >= bx.lo[0] && gi <= bx.hi[0] && gk >= bx.lo[2] &&
-                gk <= bx.hi[2]) {
-                const float delta = (float)(bx.dir * snap_value(S, bx, gi, gk, S.esnap + (long long)s * S.win));
-                const int a = (bx.comp + 1 + bx.curl) % 3;   // 0 (Hz curl 0) or 2 (Hx curl 1)
-                const int ia = a == 0 ? gi : gk;
-                float coef = tab[(a * 6 + TK_KH) * tn + ia];
-                if (pml) {
-                    if (a == 0 && rI >= xs0 && rI < xs1) {
-                        const float cp = tab[(0 * 6 + TK_CH) * tn + ia];
-                        spx[0][rK * kPSW + (rI - xs0)] += cp * delta;
-                        coef += cp;
-                    } else if (a == 2 && rK >= zs0 && rK < zs1) {
-                        const float cp = tab[(2 * 6 + TK_CH) * tn + ia];
-                        spz[0][(rK - zs0) * kRW + rI] += cp * delta;
-                        coef += cp;
-                    }
-                }
-                const float sgn = bx.curl == 0 ? 1.f : -1.f;
-                if (bx.comp == 2) sHz[rK * kLP + rI] -= sgn * coef * delta;
-                else sHx[rK * kLP + rI] -= sgn * coef * delta;
+        const float eyn = (w + 1 < kNW) ? sEtop[w + 1][lane] : 0.f;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const float e0 = ey[m];
+            const float eu = m < 15 ? ey[m + 1] : eyn;
+            const float er = wave_next(e0);
+            const float d2 = eu - e0, d1 = er - e0;
+            if (INNER) {
+                hx[m] -= -d2 * khz0;
+                hz[m] -= d1 * khx;
+            } else {
+                const float4 z = szh[16 * w + m];
+                float acc = -d2 * z.x;
+                pzh[m] = z.y * pzh[m] + z.z * d2;
+                acc -= pzh[m];
+                hx[m] -= acc;
+                float acc2 = d1 * khx;
+                pxh[m] = bhx * pxh[m] + chx * d1;
+                acc2 += pxh[m];
+                hz[m] -= acc2;
             }
         }
+        // TF/SF corrections of the H half step (incident E of this step).
+        // One incident value per box and lane: for an x face, lane l reads
+        // the value of row l and the face lane takes it with readlane.
+        for (unsigned mk = hmask; mk; mk &= mk - 1) {
+            const TfBox &bx = S.tfH[__builtin_ctz(mk)];
+            const double *snap = sg[0][s];
+            const int a = (bx.comp + 1 + bx.curl) % 3;
+            const float sgn = bx.curl == 0 ? 1.f : -1.f;
+            const bool xf = bx.fa == 0;
+            const int kl = kb + (lane & 15);
+            const bool ld = xf ? (lane < 16 && kl >= bx.lo[2] && kl <= bx.hi[2])
+                               : (i >= bx.lo[0] && i <= bx.hi[0]);
+            const float dl = ld ? (float)(bx.dir * snap_value(S, bx, xf ? bx.lo[0] : i, xf ? kl : bx.lo[2], snap, mbase)) : 0.f;
+            const bool face_lane = xf ? i == bx.lo[0] : ld;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int k = kb + m;
+                if (!(xf ? (k >= bx.lo[2] && k <= bx.hi[2]) : k == bx.lo[2])) continue;   // wave-uniform
+                const float delta = xf ? rdlane(dl, m) : dl;
+                float coef;
+                if (a == 0) {
+                    coef = khx;
+                    if (!INNER && face_lane) {
+                        pxh[m] += chx * delta;
+                        coef += chx;
+                    }
+                } else if (INNER) {
+                    coef = khz0;
+                } else {
+                    const float4 z = szh[16 * w + m];
+                    if (face_lane) pzh[m] += z.z * delta;
+                    coef = z.x + z.z;
+                }
+                const float v = sgn * coef * delta;
+                if (face_lane) {
+                    if (bx.comp == 2) hz[m] -= v;
+                    else hx[m] -= v;
+                }
+            }
+        }
+        // H probes of step t
+        for (int q = 0; q < nprb; ++q) {
+            const int d = sprd[q], c = d >> 16, rK = (d >> 8) & 255, rI = d & 255;
+            if (c >= 3 && t < pcap && lane == rI && (rK >> 4) == w) {
+                float v = 0.f;
+#pragma unroll
+                for (int m = 0; m < 16; ++m)
+                    if (m == (rK & 15)) v = c == 3 ? hx[m] : (c == 5 ? hz[m] : 0.f);
+                prec[(long long)t * np + sprb[q]] = (double)v * S.hscale;
+            }
+        }
+        // ---------------- E half step
+        sHbot[w][lane] = hx[15];
         __syncthreads();
-        // H probes of step t (owned interior cells)
-        if (tid < S.np && t < S.pcap) {
-            const int c = S.pcomp[tid];
-            const long long o = S.pidx[tid];
-            const int pi = (int)(o % px), pk = (int)(o / px);
-            const int rI = pi - gi0, rK = pk - gk0;
-            if (c >= 3 && rI >= kTT && rI < kTT + kTW && rK >= kTT && rK < kTT + kTW)
-                S.prec[(long long)t * S.np + tid] = (double)(c == 3 ? sHx[rK * kLP + rI] : sHz[rK * kLP + rI]) * S.hscale;
-        }
-        // ---- E half step
-#pragma unroll 4
-        for (int m = 0; m < kRW / 4; ++m) {
-            const int rk = (tid >> 6) + 4 * m, k = gk0 + rk;
-            if (ri >= 1 && rk >= 1 && i >= 1 && i <= nx - 2 && k >= 1 && k <= nz - 2) {
-                const int o = rk * kLP + ri;
-                const float d1 = sHx[o] - sHx[o - kLP];
-                const float d2 = sHz[o] - sHz[o - 1];
-                float acc = d1 * tab[(2 * 6 + TK_KE) * tn + k] - d2 * tab[(0 * 6 + TK_KE) * tn + i];
-                if (pml) {
-                    if (rk >= zs0 && rk < zs1) {
-                        float &p = spz[1][(rk - zs0) * kRW + ri];
-                        p = tab[(2 * 6 + TK_BE) * tn + k] * p + tab[(2 * 6 + TK_CE) * tn + k] * d1;
-                        acc += p;
-                    }
-                    if (ri >= xs0 && ri < xs1) {
-                        float &p = spx[1][rk * kPSW + (ri - xs0)];
-                        p = tab[(0 * 6 + TK_BE) * tn + i] * p + tab[(0 * 6 + TK_CE) * tn + i] * d2;
-                        acc -= p;
-                    }
-                }
-                sEy[o] += scb[sid[rk * kRW + ri]] * acc;
+        const float hxp = w > 0 ? sHbot[w - 1][lane] : 0.f;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const float hd = m > 0 ? hx[m - 1] : hxp;
+            const float hl = wave_prev(hz[m]);
+            const float d1 = hx[m] - hd;
+            const float d2 = hz[m] - hl;
+            if (INNER) {
+                ey[m] += cb[m] * (d1 * kez0 - d2 * kex);
+            } else {
+                const float4 z = sze[16 * w + m];
+                float acc = d1 * z.x - d2 * kex;
+                pze[m] = z.y * pze[m] + z.z * d1;
+                acc += pze[m];
+                pxe[m] = bex * pxe[m] + cex * d2;
+                acc -= pxe[m];
+                ey[m] += cb[m] * acc;
             }
         }
-        __syncthreads();
-        // TF/SF corrections of the E half step: first-curl boxes, then second-curl
-        // boxes (they share the corner cells of the TF box)
-        for (int pass = 0; pass < 2; ++pass) {
-            if (tid < 64 * S.ntfE) {
-                const TfBox &bx = S.tfE[tid >> 6];
-                const int u = tid & 63;
-                if (bx.curl == pass) {
-                    const int fa = bx.fa;
-                    int gi, gk;
-                    if (fa == 0) { gi = bx.lo[0]; gk = gk0 + u; } else { gk = bx.lo[2]; gi = gi0 + u; }
-                    const int rI = gi - gi0, rK = gk - gk0;
-                    if (rI >= 0 && rI < kRW && rK >= 0 && rK < kRW && gi >= bx.lo[0] && gi <= bx.hi[0] &&
-                        gk >= bx.lo[2] && gk <= bx.hi[2]) {
-                        const float delta =
-                            (float)(bx.dir * snap_value(S, bx, gi, gk, S.hsnap + (long long)s * S.win));
-                        const int a = (bx.comp + 1 + bx.curl) % 3;   // 2 (curl 0) or 0 (curl 1)
-                        const int ia = a == 0 ? gi : gk;
-                        float coef = tab[(a * 6 + TK_KE) * tn + ia];
-                        if (pml) {
-                            if (a == 0 && rI >= xs0 && rI < xs1) {
-                                const float cp = tab[(0 * 6 + TK_CE) * tn + ia];
-                                spx[1][rK * kPSW + (rI - xs0)] += cp * delta;
-                                coef += cp;
-                            } else if (a == 2 && rK >= zs0 && rK < zs1) {
-                                const float cp = tab[(2 * 6 + TK_CE) * tn + ia];
-                                spz[1][(rK - zs0) * kRW + rI] += cp * delta;
-                                coef += cp;
-                            }
-                        }
-                        const float sgn = bx.curl == 0 ? 1.f : -1.f;
-                        sEy[rK * kLP + rI] += scb[sid[rK * kRW + rI]] * (sgn * coef * delta);
+        // TF/SF corrections of the E half step (incident H of this step)
+        for (unsigned mk = emask; mk; mk &= mk - 1) {
+            const TfBox &bx = S.tfE[__builtin_ctz(mk)];
+            const double *snap = sg[1][s];
+            const int a = (bx.comp + 1 + bx.curl) % 3;
+            const float sgn = bx.curl == 0 ? 1.f : -1.f;
+            const bool xf = bx.fa == 0;
+            const int kl = kb + (lane & 15);
+            const bool ld = xf ? (lane < 16 && kl >= bx.lo[2] && kl <= bx.hi[2])
+                               : (i >= bx.lo[0] && i <= bx.hi[0]);
+            const float dl = ld ? (float)(bx.dir * snap_value(S, bx, xf ? bx.lo[0] : i, xf ? kl : bx.lo[2], snap, mbase)) : 0.f;
+            const bool face_lane = xf ? i == bx.lo[0] : ld;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int k = kb + m;
+                if (!(xf ? (k >= bx.lo[2] && k <= bx.hi[2]) : k == bx.lo[2])) continue;   // wave-uniform
+                const float delta = xf ? rdlane(dl, m) : dl;
+                float coef;
+                if (a == 0) {
+                    coef = kex;
+                    if (!INNER && face_lane) {
+                        pxe[m] += cex * delta;
+                        coef += cex;
                     }
+                } else if (INNER) {
+                    coef = kez0;
+                } else {
+                    const float4 z = sze[16 * w + m];
+                    if (face_lane) pze[m] += z.z * delta;
+                    coef = z.x + z.z;
                 }
+                if (face_lane) ey[m] += cb[m] * (sgn * coef * delta);
             }
-            __syncthreads();
         }
         // E probes of step t
-        if (tid < S.np && t < S.pcap) {
-            const int c = S.pcomp[tid];
-            const long long o = S.pidx[tid];
-            const int pi = (int)(o % px), pk = (int)(o / px);
-            const int rI = pi - gi0, rK = pk - gk0;
-            if (c == 1 && rI >= kTT && rI < kTT + kTW && rK >= kTT && rK < kTT + kTW)
-                S.prec[(long long)t * S.np + tid] = (double)sEy[rK * kLP + rI];
+        for (int q = 0; q < nprb; ++q) {
+            const int d = sprd[q], c = d >> 16, rK = (d >> 8) & 255, rI = d & 255;
+            if (c < 3 && t < pcap && lane == rI && (rK >> 4) == w) {
+                float v = 0.f;
+#pragma unroll
+                for (int m = 0; m < 16; ++m)
+                    if (m == (rK & 15) && c == 1) v = ey[m];
+                prec[(long long)t * np + sprb[q]] = (double)v;
+            }
         }
     }
 
-    // ---- write the interior back into the other buffer
-    for (int q = tid; q < kTW * kTW; q += 256) {
-        const int ri2 = kTT + q % kTW, rk2 = kTT + q / kTW;
-        const int gi = gi0 + ri2, gk = gk0 + rk2;
-        if (gi < nx && gk < nz) {
-            const long long o = (long long)gk * px + gi;
-            bst(Fout + fs + o, sEy[rk2 * kLP + ri2]);
-            bst(Fout + 3 * fs + o, sHx[rk2 * kLP + ri2]);
-            bst(Fout + 5 * fs + o, sHz[rk2 * kLP + ri2]);
-        }
-    }
-    if (pml) {
-        for (int q = tid; q < kTW * kTW; q += 256) {
-            const int ri2 = kTT + q % kTW, rk2 = kTT + q / kTW;
-            const int gi = gi0 + ri2, gk = gk0 + rk2;
-            if (gi >= nx || gk >= nz) continue;
-            if (ri2 >= xs0 && ri2 < xs1) {
-                const long long off = (long long)gk * S.sw[0] + bslab(gi, S.slo[0], S.shi[0]);
-                qx0[off] = spx[0][rk2 * kPSW + (ri2 - xs0)];
-                qx0[3 * S.scells[0] + off] = spx[1][rk2 * kPSW + (ri2 - xs0)];
-            }
-            if (rk2 >= zs0 && rk2 < zs1) {
-                const long long off = (long long)bslab(gk, S.slo[2], S.shi[2]) * px + gi;
-                qz0[S.scells[2] + off] = spz[0][(rk2 - zs0) * kRW + ri2];
-                qz0[2 * S.scells[2] + off] = spz[1][(rk2 - zs0) * kRW + ri2];
+    // ---------------- write the interior back into the other buffers
+    float *qx0 = S.psi[par ^ 1], *qz0 = S.psi[par ^ 1] + 4 * (S.scells[0] + S.scells[1]);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int k = kb + m, rk = 16 * w + m;
+        if (rk >= kTT && rk < kRH - kTT && lane >= kTT && lane < 64 - kTT && (INNER || (i < nx && k < nz))) {
+            const long long o = (long long)k * px + i;
+            bst(Fout + fs + o, ey[m]);
+            bst(Fout + 3 * fs + o, hx[m]);
+            bst(Fout + 5 * fs + o, hz[m]);
+            if (!INNER && pml) {
+                if (xslab) {
+                    const long long off = (long long)k * S.sw[0] + bslab(i, slo0, shi0);
+                    qx0[off] = pxh[m];
+                    qx0[3 * S.scells[0] + off] = pxe[m];
+                }
+                if (k < slo2 || k >= shi2) {
+                    const long long off = (long long)bslab(k, slo2, shi2) * px + i;
+                    qz0[S.scells[2] + off] = pzh[m];
+                    qz0[2 * S.scells[2] + off] = pze[m];
+                }
             }
         }
     }
 }
 
 template <typename T>
-void launch_tile2d(const BSim *sims, const BJob *jobs, int njobs, int nt, int par, hipStream_t s) {
-    hipLaunchKernelGGL(k_tile2d<T>, dim3((unsigned)njobs), dim3(256), 0, s, sims, jobs, nt, par);
+void launch_tile2d(const BSim *sims, const BJob *jobs, int njobs, bool inner, int nt, int par, hipStream_t s) {
+    if (!njobs) return;
+    if (inner) hipLaunchKernelGGL((k_tile2d<T, true>), dim3((unsigned)njobs), dim3(64 * kNW), 0, s, sims, jobs, nt, par);
+    else hipLaunchKernelGGL((k_tile2d<T, false>), dim3((unsigned)njobs), dim3(64 * kNW), 0, s, sims, jobs, nt, par);
 }
 void launch_aux_chunk(const BAux *aux, int naux, int nt, hipStream_t s) {
     hipLaunchKernelGGL(k_aux_chunk, dim3((unsigned)naux), dim3(256), 0, s, aux, nt);
@@ -349,7 +444,7 @@ void launch_aux_chunk(const BAux *aux, int naux, int nt, hipStream_t s) {
 void launch_batch_finish(const BSim *sims, int nsims, int nt, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_finish, dim3((unsigned)((nsims + 255) / 256)), dim3(256), 0, s, sims, nsims, nt);
 }
-template void launch_tile2d<float>(const BSim *, const BJob *, int, int, int, hipStream_t);
-template void launch_tile2d<__half>(const BSim *, const BJob *, int, int, int, hipStream_t);
+template void launch_tile2d<float>(const BSim *, const BJob *, int, bool, int, int, hipStream_t);
+template void launch_tile2d<__half>(const BSim *, const BJob *, int, bool, int, int, hipStream_t);
 
 }  // namespace lx

@@ -39,6 +39,7 @@ struct lx_sim {
     int *counter = nullptr;
     int tn = 0;
     int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0}, sw[3] = {0, 0, 0};
+    int ulo[3] = {0, 0, 0}, uhi[3] = {0, 0, 0};   // uniform-coefficient range per axis, inside [slo, shi)
     int64_t slab_cells[3] = {0, 0, 0};
     lx::TfList tfH{}, tfE{};
     int kz = 32;

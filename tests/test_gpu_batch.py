@@ -48,9 +48,23 @@ def make(w, ctx):
     return s
 
 
-def phc(n, seed, precision=None):
+def phc(n, seed, precision=None, delay=2.0):
     w = configs.phc2d(n, seed, steps=400) if precision is None else configs.phc2d(n, seed, steps=400, precision=precision)
-    w.p.pw_delay = 2.0   # pulse centre at step ~144
+    w.p.pw_delay = delay   # pulse centre at step ~72*delay
+    return w
+
+
+def thick_pml(n):
+    """TF/SF plane wave with a 30-cell CPML (a slab wider than the tile halo)."""
+    w = phc(n, 11)
+    w.p.pml_thickness = 30 * w.p.dx
+    return w
+
+
+def no_pml(n):
+    """TF/SF plane wave in a PEC box without CPML."""
+    w = phc(n, 12)
+    w.p.use_pml = 0
     return w
 
 
@@ -64,13 +78,13 @@ def soft_2d():
 
 def test_batch_matches_single_and_oracle(gpu_ctx):
     from luxum_amd.sim import Batch
-    ws = [phc(128, 1), phc(176, 2), phc(208, 3), phc(256, 4), soft_2d(),
+    ws = [phc(128, 1), phc(176, 2), phc(208, 3), phc(256, 4), thick_pml(200), no_pml(150), soft_2d(),
           configs.config1(n=32, steps=400, pml=6, margin=4)]
     b = Batch(gpu_ctx)
     members = [make(w, gpu_ctx) for w in ws]
     for s in members:
         b.add(s)
-    say("batch: 6 members (4 tiled 2D, 1 point-source 2D, 1 3D), steps 37 + 200")
+    say("batch: 8 members (6 tiled 2D, 1 point-source 2D, 1 3D), steps 37 + 200")
     b.step(37)
     b.step(200)
     b.sync()
@@ -99,20 +113,20 @@ def test_batch_interleaved_with_single_steps(gpu_ctx):
     ping-pong buffers stay consistent (odd chunk counts, graph replay from
     either parity)."""
     from luxum_amd.sim import Batch
-    ws = [phc(160, 7), phc(144, 8)]
+    ws = [phc(160, 7, delay=1.0), phc(144, 8, delay=1.0)]
     members = [make(w, gpu_ctx) for w in ws]
     b = Batch(gpu_ctx)
     for s in members:
         b.add(s)
-    say("interleave: batch 8, single 5, batch 40, remove, single 11, batch 19")
-    b.step(8)
+    say("interleave: batch 64, single 5, batch 100, remove, single 11, batch 43")
+    b.step(64)
     members[0].step(5)
-    b.step(40)
+    b.step(100)
     b.remove(members[1])
     members[1].step(11)
-    b.step(19)
+    b.step(43)
     b.sync()
-    counts = [8 + 5 + 40 + 19, 8 + 40 + 11]
+    counts = [64 + 5 + 100 + 43, 64 + 100 + 11]
     for w, s, n in zip(ws, members, counts):
         ref = make(w, gpu_ctx)
         ref.step(n)
