@@ -211,14 +211,87 @@ lx_status lx_sim_autotune(lx_sim *sim, int32_t *kz_out);
 
 /* ---- batches of independent simulations ------------------------------- */
 
-/* No reference counterpart: advance many simulations of uneven sizes in one
- * persistent kernel per half-step over a flattened tile list. */
+/* No reference counterpart: advance many simulations of uneven sizes
+ * together.  Planar members (aux-1D or no plane wave, no point source) run
+ * as one temporally tiled launch over a flattened tile list of every member
+ * per chunk of 8 steps; other members step with their own kernels on their
+ * own streams.  Members may still be stepped, read and snapshotted on their
+ * own between batch steps; lx_sim_destroy removes a member. */
 lx_status lx_batch_create(lx_ctx *ctx, lx_batch **out);
 void lx_batch_destroy(lx_batch *b);
 lx_status lx_batch_add(lx_batch *b, lx_sim *sim);
 lx_status lx_batch_remove(lx_batch *b, lx_sim *sim);
 lx_status lx_batch_step(lx_batch *b, int32_t nsteps);
 lx_status lx_batch_sync(lx_batch *b);
+
+/* ---- serving: continuous batching of simulation requests --------------- */
+
+/* No reference counterpart.  A server takes simulation requests with arrival
+ * times, admits arrived requests (first fit in arrival order) while the sum
+ * of their device footprints stays within an HBM cap and the number of
+ * active requests within a slot limit, and advances every active request
+ * together in ticks (LX_SERVE_BATCHED: one lx_batch; a tick is the smallest
+ * remaining step count of the active requests, at most 128 steps).  Finished
+ * requests leave at the end of their tick; their probe series (and fields,
+ * when asked) stay readable.  LX_SERVE_PLAIN runs the requests one at a time
+ * in arrival order with the single-simulation kernels (the per-request
+ * baseline). */
+typedef struct lx_server lx_server;
+
+enum { LX_SERVE_BATCHED = 0, LX_SERVE_PLAIN = 1 };
+enum { LX_REQ_QUEUED = 0, LX_REQ_ACTIVE = 1, LX_REQ_DONE = 2, LX_REQ_FAILED = 3 };
+
+typedef struct lx_request {
+    const lx_params *params;
+    const uint8_t *ids;       /* material ids, C order over the grid (as lx_sim_set_materials) */
+    size_t n_cells;
+    const double *eps_r;      /* eps_r per id */
+    int32_t n_ids;
+    const int32_t *probes;    /* n_probes x (component, i, j, k), may be NULL */
+    int32_t n_probes;
+    int32_t steps;            /* timesteps to run */
+    double arrival;           /* seconds after the start of lx_server_run */
+    int32_t keep_fields;      /* keep the final fields of every component on the host */
+} lx_request;
+
+typedef struct lx_request_info {
+    int32_t state;            /* LX_REQ_* */
+    int32_t steps_done;
+    double arrival, admitted, finished;   /* seconds after the start of lx_server_run */
+    int64_t footprint;        /* device bytes reserved at admission */
+    int64_t cells;
+    int32_t n_probes;
+} lx_request_info;
+
+typedef struct lx_server_stats {
+    int32_t submitted, completed, failed;
+    int32_t peak_active;
+    int64_t peak_bytes;       /* largest sum of reserved footprints */
+    int32_t ticks;
+    double makespan;          /* seconds from the start of the run to the last completion */
+    double cell_updates;      /* sum over requests of cells x steps */
+    double alg_bytes;         /* sum over requests of bytes_per_step x steps (half-step method) */
+} lx_server_stats;
+
+/* Upper bound of the device bytes a simulation with these parameters uses
+ * (batched: including the second field/psi buffers of a tiled batch member),
+ * with `steps` probe records of `n_probes` probes. */
+lx_status lx_params_footprint(const lx_params *p, int32_t batched, int32_t steps, int32_t n_probes,
+                              int64_t *bytes);
+lx_status lx_server_create(lx_ctx *ctx, int32_t mode, int64_t hbm_cap_bytes, int32_t max_active,
+                           lx_server **out);
+void lx_server_destroy(lx_server *srv);
+/* Copies the request (parameters, ids, eps_r, probes); *id is its index. */
+lx_status lx_server_submit(lx_server *srv, const lx_request *req, int64_t *id);
+/* Runs until every submitted request is done or failed. */
+lx_status lx_server_run(lx_server *srv);
+lx_status lx_server_info(lx_server *srv, int64_t id, lx_request_info *info);
+/* Probe series of a finished request: steps x n_probes doubles. */
+lx_status lx_server_probes(lx_server *srv, int64_t id, double *dst, size_t n);
+/* Final field of a finished request kept with keep_fields (layout as
+ * lx_sim_read_field). */
+lx_status lx_server_field(lx_server *srv, int64_t id, int32_t comp, double *dst, size_t n);
+lx_status lx_server_get_stats(lx_server *srv, lx_server_stats *st);
 
 /* ---- errors ------------------------------------------------------------- */
 const char *lx_last_error(void);

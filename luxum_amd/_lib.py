@@ -28,6 +28,8 @@ EXPORTS = (
     "lx_sim_time_kernels", "lx_sim_autotune",
     "lx_batch_create", "lx_batch_destroy", "lx_batch_add", "lx_batch_remove",
     "lx_batch_step", "lx_batch_sync",
+    "lx_params_footprint", "lx_server_create", "lx_server_destroy", "lx_server_submit", "lx_server_run",
+    "lx_server_info", "lx_server_probes", "lx_server_field", "lx_server_get_stats",
     "lx_last_error", "lx_version",
 )
 
@@ -49,6 +51,44 @@ class SimInfo(C.Structure):
         ("step", C.c_int32),
         ("slab_lo", C.c_int32 * 3),
         ("slab_hi", C.c_int32 * 3),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("params", C.c_void_p),
+        ("ids", C.c_void_p),
+        ("n_cells", C.c_size_t),
+        ("eps_r", C.c_void_p),
+        ("n_ids", C.c_int32),
+        ("probes", C.c_void_p),
+        ("n_probes", C.c_int32),
+        ("steps", C.c_int32),
+        ("arrival", C.c_double),
+        ("keep_fields", C.c_int32),
+    ]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32),
+        ("steps_done", C.c_int32),
+        ("arrival", C.c_double), ("admitted", C.c_double), ("finished", C.c_double),
+        ("footprint", C.c_int64),
+        ("cells", C.c_int64),
+        ("n_probes", C.c_int32),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int32), ("completed", C.c_int32), ("failed", C.c_int32),
+        ("peak_active", C.c_int32),
+        ("peak_bytes", C.c_int64),
+        ("ticks", C.c_int32),
+        ("makespan", C.c_double),
+        ("cell_updates", C.c_double),
+        ("alg_bytes", C.c_double),
     ]
 
 
@@ -95,6 +135,15 @@ def lib():
         "lx_batch_remove": ([vp, vp], st),
         "lx_batch_step": ([vp, C.c_int32], st),
         "lx_batch_sync": ([vp], st),
+        "lx_params_footprint": ([P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64)], st),
+        "lx_server_create": ([vp, C.c_int32, C.c_int64, C.c_int32, C.POINTER(vp)], st),
+        "lx_server_destroy": ([vp], None),
+        "lx_server_submit": ([vp, C.POINTER(Request), C.POINTER(C.c_int64)], st),
+        "lx_server_run": ([vp], st),
+        "lx_server_info": ([vp, C.c_int64, C.POINTER(RequestInfo)], st),
+        "lx_server_probes": ([vp, C.c_int64, vp, C.c_size_t], st),
+        "lx_server_field": ([vp, C.c_int64, C.c_int32, vp, C.c_size_t], st),
+        "lx_server_get_stats": ([vp, C.POINTER(ServerStats)], st),
         "lx_last_error": ([], C.c_char_p),
         "lx_version": ([], C.c_char_p),
     }

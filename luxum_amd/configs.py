@@ -174,6 +174,74 @@ def batch_sizes(count: int = 256, seed: int = 2024):
     return [int(x) for x in rng.choice(sizes, size=count)]
 
 
+def scatterer2d(n: int, rng, precision: int = LX_FP32, steps: int = 3000) -> Workload:
+    """2D request of the served stream: config 0's set-up (CPML 16, TF/SF
+    plane wave +x, Ey) around one dielectric square or disk, eps_r ~
+    U[1.5, 12], size 10-40% of the domain, at a random place inside the TF
+    box."""
+    w = config0(n=n, steps=steps, precision=precision)
+    lo, hi = w.p.tf_lo[0], w.p.tf_hi[0]
+    size = max(2, int(rng.uniform(0.1, 0.4) * n))
+    size = min(size, hi - lo - 2)
+    ci, ck = (int(rng.integers(lo + 1 + size // 2, hi - size // 2)) for _ in range(2))
+    er = float(rng.uniform(1.5, 12.0))
+    ii, kk = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    if rng.random() < 0.5:
+        body = (np.abs(ii - ci) <= size // 2) & (np.abs(kk - ck) <= size // 2)
+        shape = "square"
+    else:
+        body = (ii - ci) ** 2 + (kk - ck) ** 2 <= (size / 2) ** 2
+        shape = "disk"
+    ids = body.astype(np.uint8)[:, None, :]
+    return Workload("2D %s %d^2 eps %.2f" % (shape, n, er), w.p, ids, np.array([1.0, er]), w.probes)
+
+
+def scatterer3d(n: int, rng, precision: int = LX_FP32, steps: int = 3000) -> Workload:
+    """3D request of the served stream: config 1's set-up (CPML 12, TF/SF 16
+    inside, plane wave +x, Ez) around one dielectric cube or sphere, eps_r ~
+    U[1.5, 12], size 10-40% of the domain."""
+    w = config1(n=n, steps=steps, precision=precision)
+    lo, hi = w.p.tf_lo[0], w.p.tf_hi[0]
+    size = max(2, int(rng.uniform(0.1, 0.4) * n))
+    size = min(size, hi - lo - 2)
+    c = [int(rng.integers(lo + 1 + size // 2, hi - size // 2)) for _ in range(3)]
+    er = float(rng.uniform(1.5, 12.0))
+    ax = np.arange(n)
+    if rng.random() < 0.5:
+        m = [np.abs(ax - c[a]) <= size // 2 for a in range(3)]
+        body = m[0][:, None, None] & m[1][None, :, None] & m[2][None, None, :]
+        shape = "cube"
+    else:
+        d = [(ax - c[a]) ** 2 for a in range(3)]
+        body = (d[0][:, None, None] + d[1][None, :, None] + d[2][None, None, :]) <= (size / 2) ** 2
+        shape = "sphere"
+    return Workload("3D %s %d^3 eps %.2f" % (shape, n, er), w.p, body.astype(np.uint8), np.array([1.0, er]),
+                    w.probes)
+
+
+def serve_stream(count: int, rate: float, seed: int = 7, steps_range=(1000, 5000), frac3d: float = 0.2,
+                 frac_fp16: float = 0.5, sizes2d=None, sizes3d=None):
+    """Seeded request stream of the served workload: Poisson arrivals at
+    `rate` requests/s; 80% 2D (256^2-2048^2), 20% 3D (64^3-320^3) scatterers;
+    step counts uniform over steps_range; FP32 or FP16 storage.  Returns a
+    list of (workload, steps, arrival seconds)."""
+    rng = np.random.default_rng(seed)
+    sizes2d = list(range(256, 2049, 128)) if sizes2d is None else list(sizes2d)
+    sizes3d = list(range(64, 321, 32)) if sizes3d is None else list(sizes3d)
+    t = 0.0
+    out = []
+    for _ in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+        prec = LX_FP16 if rng.random() < frac_fp16 else LX_FP32
+        if rng.random() < frac3d:
+            w = scatterer3d(int(rng.choice(sizes3d)), rng, prec, steps)
+        else:
+            w = scatterer2d(int(rng.choice(sizes2d)), rng, prec, steps)
+        out.append((w, steps, t))
+    return out
+
+
 def workload_bytes_per_step(p: LxParams, slab_cells) -> int:
     """Algorithmic HBM bytes of one full timestep: every stored field value is
     read twice and written once, plus the uint8 id and the CPML psi traffic
@@ -188,4 +256,5 @@ def workload_bytes_per_step(p: LxParams, slab_cells) -> int:
 
 
 __all__ = ["Workload", "config0", "config1", "config2", "phc2d", "batch_sizes", "workload_bytes_per_step",
+           "scatterer2d", "scatterer3d", "serve_stream",
            "LX_FP16", "LX_FP32"]

@@ -968,3 +968,38 @@ lx_status lx_sim_time_kernels(lx_sim *s, int32_t nsteps, double *ms_h, double *m
 }
 
 }  // extern "C"
+
+// ---- footprint estimate (admission control) --------------------------------
+extern "C" lx_status lx_params_footprint(const lx_params *p, int32_t batched, int32_t steps, int32_t n_probes,
+                                         int64_t *bytes) {
+    if (!p || !bytes) return fail(LX_EINVAL, "lx_params_footprint: null pointer");
+    lx_status st = lx_params_check(p);
+    if (st) return st;
+    const int64_t N[3] = {p->nx, p->is_planar ? 1 : p->ny, p->nz};
+    const int64_t px = (N[0] + 63) / 64 * 64;
+    const int64_t ncell = px * N[1] * N[2];
+    const int64_t fstride = (ncell + 127) / 128 * 128;
+    const int64_t esize = p->precision == LX_FP16 ? 2 : 4;
+    const int64_t fields = fstride * 6 * esize;
+    int64_t psi = 0;
+    if (p->use_pml) {
+        const double d[3] = {p->dx, p->dy, p->dz};
+        const int64_t ext[3] = {px, N[1], N[2]};
+        for (int a = 0; a < 3; ++a) {
+            if (p->is_planar && a == 1) continue;
+            const int64_t w = std::min<int64_t>(ext[a], 2 * ((int64_t)std::ceil(p->pml_thickness / d[a]) + 5));
+            int64_t other = 1;
+            for (int b = 0; b < 3; ++b)
+                if (b != a) other *= ext[b];
+            psi += 4 * w * other * (int64_t)sizeof(float);
+        }
+    }
+    const int64_t nmax = std::max(px, std::max(N[1], N[2])) + 4;
+    const int64_t tables = 3 * 6 * nmax * 4;
+    const int64_t line = 2 * 8 * (std::max(N[0], N[2]) + (int64_t)std::ceil(p->courant * std::max(steps, p->num_timesteps)) + 64);
+    const int64_t probes = (int64_t)n_probes * std::max(steps, p->num_timesteps) * 8;
+    int64_t total = fields + ncell + psi + tables + line + probes + (2 << 20);   // 2 MiB of allocation slack
+    if (batched && p->is_planar) total += fields + psi + 2 * 8 * 8 * (std::max(N[0], N[2]) + 8);
+    *bytes = total;
+    return LX_OK;
+}

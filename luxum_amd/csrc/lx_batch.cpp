@@ -169,11 +169,15 @@ lx_status rebuild(lx_batch *b) {
             if (s->use_aux) {
                 const int ax = s->inc.axis;
                 const int win = std::min(s->aux.L, s->p.tf_hi[ax] + 2 - s->inc.x0 + 1);
-                void *snap = nullptr;
-                st = balloc(b, &snap, sizeof(double) * 2 * kTT * (size_t)win);
-                if (st) return st;
-                d.esnap = (const double *)snap;
-                d.hsnap = (const double *)snap + (size_t)kTT * win;
+                if (!s->bsnap || s->bsnap_win != win) {
+                    void *snap = nullptr;
+                    st = dalloc(s, &snap, sizeof(double) * 2 * kTT * (size_t)win);
+                    if (st) return st;
+                    s->bsnap = (double *)snap;
+                    s->bsnap_win = win;
+                }
+                d.esnap = s->bsnap;
+                d.hsnap = s->bsnap + (size_t)kTT * win;
                 d.win = win;
                 d.inc_axis = ax;
                 d.inc_x0 = s->inc.x0;

@@ -17,6 +17,8 @@ struct lx_sim {
     // batch membership: alternate field/psi buffers for the ping-pong tiles
     void *F_alt = nullptr;
     float *psi_alt = nullptr;
+    double *bsnap = nullptr;    // incident line snapshots of one chunk: E then H, [8][win] each
+    int bsnap_win = 0;
     lx_batch *batch = nullptr;
     lx_ctx *ctx = nullptr;
     lx_params p{};

@@ -108,3 +108,25 @@ def test_yaml_extension_document_and_errors(tmp_path):
     with pytest.raises(_lib.LuxumError) as e:
         load_yaml(str(tmp_path / "nope.yaml"))
     assert e.value.status == _lib.LX_EIO
+
+
+def test_footprint_estimate():
+    """Admission footprints: grow with the grid, count the second buffers of a
+    tiled batch member (2D only), cover the fields and ids, reject invalid
+    parameters."""
+    from luxum_amd import configs
+    from luxum_amd.server import footprint
+    from luxum_amd._lib import LuxumError
+    w1 = configs.config0(n=256)
+    w2 = configs.config0(n=512)
+    a, b = footprint(w1.p, False, 1000, 4), footprint(w2.p, False, 1000, 4)
+    assert b > a > 256 * 256 * (6 * 4 + 1)
+    assert footprint(w1.p, True, 1000, 4) > a + 6 * 4 * 256 * 256
+    w3 = configs.config1(n=64)
+    assert footprint(w3.p, True, 100, 8) == footprint(w3.p, False, 100, 8) >= 64 ** 3 * (6 * 4 + 1)
+    w3.p.num_timesteps = 10   # probe buffers hold max(steps, num_timesteps) records
+    assert footprint(w3.p, False, 1000, 8) - footprint(w3.p, False, 100, 8) >= 900 * 8 * 8
+    bad = configs.config0(n=256).p
+    bad.nx = 0
+    with pytest.raises(LuxumError):
+        footprint(bad, True, 10, 0)
