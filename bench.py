@@ -9,6 +9,18 @@ Prints ONE JSON line: Gcell-updates/s (whole job), the roofline of the
 dominant kernel (algorithmic bytes per launch / average launch duration,
 timed with HIP events on the simulation's stream) and a CPU baseline (the
 FP64 reference-structured oracle on a bounded sample of the same workload).
+
+Two more objects ride on the same line:
+  "batched": 256 seeded 2D photonic-crystal simulations of sizes
+     {256..2048}^2 (FP32, CPML 16, TF/SF plane wave), 3000 steps, advanced
+     together by one batch (temporally tiled kernel over a flattened tile
+     list, hipGraph-captured chunk pairs), against the same simulations run
+     one at a time with the single-simulation kernels: solves/s of both, and
+     the relative L2 between their final fields on a sample.
+  "serving": a seeded Poisson request stream (80% 2D 256^2-2048^2, 20% 3D
+     64^3-320^3 dielectric scatterers, 1000-5000 steps, FP32/FP16 storage,
+     200 GB admission cap) through the continuous-batching server and through
+     the plain per-request server: solves/s, p50/p99 latency, Gcell-updates/s.
 """
 import argparse
 import json
@@ -43,6 +55,107 @@ def cpu_baseline(args):
                       f"{args.cpu_steps} timesteps ({dt:.1f} s), {threads} OpenMP threads"}
 
 
+def batch_section(args):
+    """Batched 2D solves/s against one-at-a-time single-simulation solves/s."""
+    import numpy as np
+    from luxum_amd import configs
+    from luxum_amd.sim import Batch, Context, Simulation
+    ctx = Context(0)
+    sizes = configs.batch_sizes(args.batch_sims)
+    ws = [configs.phc2d(n, i, steps=args.batch_steps) for i, n in enumerate(sizes)]
+    cells = sum(w.cells for w in ws)
+    print(f"bench: batched {len(ws)} sims, {cells / 1e6:.1f} Mcells, {args.batch_steps} steps", file=sys.stderr,
+          flush=True)
+
+    def make(w):
+        s = Simulation(w.p, ctx)
+        s.set_materials(w.ids, w.eps_r)
+        s.set_probes(w.probes)
+        return s
+
+    sample = sorted(range(len(ws)), key=lambda q: sizes[q])[:: max(1, len(ws) // 6)][:6]
+    sims = [make(w) for w in ws]
+    b = Batch(ctx)
+    for s in sims:
+        b.add(s)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(args.batch_steps)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    ref = {q: {c: sims[q].field(c) for c in ws[q].p.components()} for q in sample}
+    b.close()
+    for s in sims:
+        s.close()
+    del sims
+    print(f"bench: batched {t_batch:.3f} s; one at a time ...", file=sys.stderr, flush=True)
+    t_single = 0.0
+    num = den = 0.0
+    for q, w in enumerate(ws):
+        s = make(w)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(args.batch_steps)
+        s.sync()
+        t_single += time.perf_counter() - t0
+        if q in ref:
+            for c, a in ref[q].items():
+                o = s.field(c)
+                num += float(np.sum((a - o) ** 2))
+                den += float(np.sum(o ** 2))
+        s.close()
+    print(f"bench: one at a time {t_single:.3f} s", file=sys.stderr, flush=True)
+    n = len(ws)
+    return {"workload": f"{n} seeded 2D PhC sims, sizes {{256..2048}}^2, FP32, CPML 16, TF/SF aux-1D plane wave, "
+                        f"{args.batch_steps} steps",
+            "sims": n, "cells": cells, "steps": args.batch_steps,
+            "solves_per_s": round(n / t_batch, 3), "gcell_per_s": round(cells * args.batch_steps / t_batch / 1e9, 2),
+            "one_at_a_time_solves_per_s": round(n / t_single, 3),
+            "one_at_a_time_gcell_per_s": round(cells * args.batch_steps / t_single / 1e9, 2),
+            "speedup": round(t_single / t_batch, 2),
+            "rel_l2_vs_one_at_a_time": (num / den) ** 0.5 if den > 0 else None,
+            "l2_sample_sims": len(sample)}
+
+
+def serve_section(args):
+    """Served request stream: continuous batching against the plain server."""
+    import numpy as np
+    from luxum_amd import configs
+    from luxum_amd.server import Server
+    from luxum_amd.sim import Context
+    ctx = Context(0)
+    stream = configs.serve_stream(args.serve_requests, args.serve_rate, seed=11)
+    cells_steps = sum(w.cells * steps for w, steps, _ in stream)
+    n3 = sum(1 for w, _, _ in stream if not w.p.is_planar)
+    print(f"bench: serving {len(stream)} requests ({n3} 3D), {cells_steps / 1e9:.1f} Gcell-updates, "
+          f"rate {args.serve_rate}/s", file=sys.stderr, flush=True)
+    out = {"workload": f"{len(stream)} seeded requests, Poisson {args.serve_rate}/s, 80% 2D 256^2-2048^2 / "
+                       f"20% 3D 64^3-320^3 dielectric scatterers, 1000-5000 steps, FP32/FP16 storage, "
+                       f"200 GB admission cap",
+           "requests": len(stream), "requests_3d": n3, "rate_per_s": args.serve_rate,
+           "cell_updates": cells_steps}
+    for mode in ("batched", "plain"):
+        srv = Server(ctx, hbm_cap=200 * 10 ** 9, plain=(mode == "plain"))
+        ids = [srv.submit(w.p, w.ids, w.eps_r, steps, arr, w.probes) for w, steps, arr in stream]
+        srv.run()
+        st = srv.stats()
+        lat = np.array([srv.result(i).latency for i in ids])
+        res = {"solves_per_s": round(st.completed / st.makespan, 3),
+               "p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
+               "makespan_s": round(st.makespan, 3), "gcell_per_s": round(st.cell_updates / st.makespan / 1e9, 2),
+               "completed": st.completed, "failed": st.failed, "peak_active": st.peak_active,
+               "peak_reserved_GB": round(st.peak_bytes / 1e9, 2), "ticks": st.ticks}
+        if mode == "batched":
+            eff = st.alg_bytes / st.makespan / 1e9
+            res["effective_GBps"] = round(eff, 1)
+            res["effective_roofline_frac"] = round(eff / HBM_PEAK_GBS, 4)
+        out[mode] = res
+        print(f"bench: serving {mode}: {res}", file=sys.stderr, flush=True)
+        srv.close()
+    out["speedup_solves_per_s"] = round(out["batched"]["solves_per_s"] / out["plain"]["solves_per_s"], 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +166,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-steps", type=int, default=50)
+    ap.add_argument("--batch-sims", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=3000)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (PMC traffic calibration)")
     args = ap.parse_args()
@@ -132,6 +251,10 @@ def main():
                      "bytes_per_launch": dbytes, "ms_per_launch": round(dms, 5)},
     }
     sim.close()
+    if not args.no_batch:
+        out["batched"] = batch_section(args)
+    if not args.no_serve:
+        out["serving"] = serve_section(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)
