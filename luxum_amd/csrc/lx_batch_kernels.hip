@@ -135,7 +135,7 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
     __shared__ float4 szh[INNER ? 1 : kRH], sze[INNER ? 1 : kRH];   // z {kh, b_h, c_h}, {ke, b_e, c_e} per row
     __shared__ double sg[2][kTT][kGW];        // incident E (before H half step) / H (before E half step)
 
-    const BJob job = jobs[blockIdx.x];
+    const BJob job = jobs[xcd_remap(blockIdx.x, gridDim.x)];
     const BSim &S = sims[job.sim];
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);

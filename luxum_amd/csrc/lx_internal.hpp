@@ -23,6 +23,14 @@
 
 namespace lx {
 
+// Hardware sends block b to XCD b % 8; give XCD x a contiguous range of the
+// n tiles (or jobs), so that neighbouring tiles share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    const int q = n >> 3, r = n & 7;
+    const int x = b & 7, l = b >> 3;
+    return x * q + min(x, r) + l;
+}
+
 constexpr int kBX = 64;      // threads along x per block (one wavefront)
 constexpr int kBY = 4;       // rows per block
 constexpr int kMaxTf = 12;   // TF/SF correction boxes per half step

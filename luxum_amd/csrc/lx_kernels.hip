@@ -89,12 +89,6 @@ __device__ __forceinline__ unsigned xmask(int i0, int lo, int hi) {
     return m;
 }
 
-// Hardware sends block b to XCD b % 8; give XCD x a contiguous tile range.
-__device__ __forceinline__ int xcd_remap(int b, int n) {
-    const int q = n >> 3, r = n & 7;
-    const int x = b & 7, l = b >> 3;
-    return x * q + min(x, r) + l;
-}
 
 __device__ __forceinline__ int slab_index(int x, int lo, int hi) { return x < lo ? x : lo + (x - hi); }
 
