@@ -88,6 +88,13 @@ def batch_section(args):
     for s in sims:
         s.close()
     del sims
+    n = len(ws)
+    out = {"workload": f"{n} seeded 2D PhC sims, sizes {{256..2048}}^2, FP32, CPML 16, TF/SF aux-1D plane wave, "
+                       f"{args.batch_steps} steps",
+           "sims": n, "cells": cells, "steps": args.batch_steps,
+           "solves_per_s": round(n / t_batch, 3), "gcell_per_s": round(cells * args.batch_steps / t_batch / 1e9, 2)}
+    if args.no_single:
+        return out
     print(f"bench: batched {t_batch:.3f} s; one at a time ...", file=sys.stderr, flush=True)
     t_single = 0.0
     num = den = 0.0
@@ -105,16 +112,13 @@ def batch_section(args):
                 den += float(np.sum(o ** 2))
         s.close()
     print(f"bench: one at a time {t_single:.3f} s", file=sys.stderr, flush=True)
-    n = len(ws)
-    return {"workload": f"{n} seeded 2D PhC sims, sizes {{256..2048}}^2, FP32, CPML 16, TF/SF aux-1D plane wave, "
-                        f"{args.batch_steps} steps",
-            "sims": n, "cells": cells, "steps": args.batch_steps,
-            "solves_per_s": round(n / t_batch, 3), "gcell_per_s": round(cells * args.batch_steps / t_batch / 1e9, 2),
+    out.update({
             "one_at_a_time_solves_per_s": round(n / t_single, 3),
             "one_at_a_time_gcell_per_s": round(cells * args.batch_steps / t_single / 1e9, 2),
             "speedup": round(t_single / t_batch, 2),
             "rel_l2_vs_one_at_a_time": (num / den) ** 0.5 if den > 0 else None,
-            "l2_sample_sims": len(sample)}
+            "l2_sample_sims": len(sample)})
+    return out
 
 
 def serve_section(args):
@@ -169,6 +173,7 @@ def main():
     ap.add_argument("--batch-sims", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3000)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-single", action="store_true", help="skip the one-at-a-time baseline of the batch section")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true")
