@@ -123,6 +123,8 @@ typedef struct lx_sim_info {
     int64_t bytes_per_step;     /* algorithmic HBM bytes per full timestep */
     int32_t step;               /* timesteps taken so far */
     int32_t slab_lo[3], slab_hi[3];  /* CPML slab: [0,lo) and [hi,n) per axis */
+    int32_t fused_step;         /* 3D: one fused launch per timestep (lx_sim_time_kernels then reports it as ms_h) */
+    int32_t kz, waves;          /* z planes and wavefronts per block of the 3D kernels (autotuned) */
 } lx_sim_info;
 
 /* ---- parameters ------------------------------------------------------- */

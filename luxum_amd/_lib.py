@@ -51,6 +51,9 @@ class SimInfo(C.Structure):
         ("step", C.c_int32),
         ("slab_lo", C.c_int32 * 3),
         ("slab_hi", C.c_int32 * 3),
+        ("fused_step", C.c_int32),
+        ("kz", C.c_int32),
+        ("waves", C.c_int32),
     ]
 
 

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -413,11 +414,41 @@ ProbeArgs probe_args(const lx_sim *s) {
 // One launch sequence of n timesteps: per step [point source] H launch,
 // E launch; then the sequence-end kernel.  Kernels read the step index as
 // counter[0] + toff, so the same sequence can be captured once and replayed.
+// Fused 3D: per step [point source on the current buffer], the fused step
+// (current -> other buffer), the E TF/SF corrections on the other buffer;
+// the buffers alternate, so the state sits in F_alt after an odd count.
 template <typename T>
 void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
     Grid<T> g = make_grid<T>(s);
     const ProbeArgs pr = probe_args(s);
     const bool corr = s->tfH.n > 0 || s->use_aux || s->np > 0;
+    if (s->fused) {
+        T *buf[2] = {(T *)s->F, (T *)s->F_alt};
+        float *pbuf[2] = {s->psi, s->psi_alt};
+        for (int q = 0; q < n; ++q) {
+            Launch l{q, s->use_aux, s->np > 0};
+            Grid<T> gq = g;
+            gq.F = buf[q & 1];
+            gq.psi = pbuf[q & 1];
+            gq.Fn = buf[(q + 1) & 1];
+            gq.psin = pbuf[(q + 1) & 1];
+            if (s->p.use_ps) launch_point_source(gq, s->ps, q, s->cs);
+            if (corr) launch_tfsf_hprep(gq, s->tfH, s->inc, s->aux, pr, l, s->tf_dh, s->tf_dp, s->cs);
+            if (ev) (void)hipEventRecord((*ev)[4 * q + 0], s->cs);
+            launch_step3(gq, s->nw, s->tfH, s->tf_dh, s->tf_dp, s->cs);
+            if (ev) (void)hipEventRecord((*ev)[4 * q + 1], s->cs);
+            Grid<T> gn = g;
+            gn.F = gq.Fn;
+            gn.psi = gq.psin;
+            if (ev) (void)hipEventRecord((*ev)[4 * q + 2], s->cs);
+            if (corr) launch_tfsf(gn, s->tfE, s->inc, s->aux, pr, l, 1, s->cs);
+            if (ev) (void)hipEventRecord((*ev)[4 * q + 3], s->cs);
+        }
+        Grid<T> gf = g;
+        gf.F = buf[n & 1];
+        launch_finish(gf, pr, n, s->cs);
+        return;
+    }
     for (int q = 0; q < n; ++q) {
         Launch l{q, s->use_aux, s->np > 0};
         if (s->p.use_ps) launch_point_source(g, s->ps, q, s->cs);
@@ -431,6 +462,15 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
         if (corr) launch_tfsf(g, s->tfE, s->inc, s->aux, pr, l, 1, s->cs);
     }
     launch_finish(g, pr, n, s->cs);
+}
+
+// after an enqueued sequence of n fused steps: the current state is in the
+// other buffer when n is odd
+void settle_parity(lx_sim *s, int n) {
+    if (!s->fused || !(n & 1)) return;
+    std::swap(s->F, s->F_alt);
+    std::swap(s->psi, s->psi_alt);
+    drop_graph(s);
 }
 
 void enqueue_seq(lx_sim *s, int n, std::vector<hipEvent_t> *ev = nullptr) {
@@ -499,6 +539,51 @@ size_t psi_total(const lx_sim *s) {
     size_t total = 0;
     for (int a = 0; a < 3; ++a) total += 4 * (size_t)s->slab_cells[a];
     return std::max<size_t>(total, 1);
+}
+
+// Second field/psi buffer set and the per-face-cell H corrections of the
+// fused 3D step.
+lx_status ensure_fused_buffers(lx_sim *s) {
+    if (s->F_alt) return LX_OK;
+    const size_t fbytes = (size_t)s->fstride * 6 * s->esize;
+    lx_status st = dalloc(s, &s->F_alt, fbytes);
+    if (st) return st;
+    launch_zero_bytes(s->F_alt, fbytes, s->cs);
+    void *pp = nullptr;
+    if ((st = dalloc(s, &pp, psi_total(s) * sizeof(float)))) return st;
+    launch_zero_bytes(pp, psi_total(s) * sizeof(float), s->cs);
+    s->psi_alt = (float *)pp;
+    if (s->tfH.total > 0) {
+        if ((st = dalloc(s, &pp, sizeof(float) * (size_t)s->tfH.total))) return st;
+        s->tf_dh = (float *)pp;
+        if ((st = dalloc(s, &pp, sizeof(float) * (size_t)s->tfH.total))) return st;
+        s->tf_dp = (float *)pp;
+    }
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    return LX_OK;
+}
+
+static void dfree(lx_sim *s, void *p) {
+    if (!p) return;
+    auto it = std::find(s->allocs.begin(), s->allocs.end(), p);
+    if (it == s->allocs.end()) return;
+    size_t bytes = 0;
+    (void)hipMemPtrGetInfo(p, &bytes);
+    (void)hipFree(p);
+    s->allocs.erase(it);
+    s->bytes -= (int64_t)bytes;
+}
+
+void release_fused_buffers(lx_sim *s) {
+    (void)hipStreamSynchronize(s->cs);
+    dfree(s, s->F_alt);
+    dfree(s, s->psi_alt);
+    dfree(s, s->tf_dh);
+    dfree(s, s->tf_dp);
+    s->F_alt = nullptr;
+    s->psi_alt = nullptr;
+    s->tf_dh = s->tf_dp = nullptr;
+    s->fused = 0;
 }
 
 template Grid<float> make_grid<float>(const lx_sim *s);
@@ -622,6 +707,15 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
     if ((st = build_tables(s))) return bail(st);
     if ((st = build_tfsf(s))) return bail(st);
     if ((st = build_incident(s))) return bail(st);
+    if (!s->planar) {
+        // 3D: the half-step pair by default; LX_FUSED=1 selects the fused step
+        // (lx_sim_autotune times both and keeps the faster)
+        const char *fz = std::getenv("LX_FUSED");
+        if (fz && *fz && *fz != '0') {
+            if ((st = ensure_fused_buffers(s))) return bail(st);
+            s->fused = 1;
+        }
+    }
     if (p->use_ps) {
         PointSrc &ps = s->ps;
         ps.kind = p->src_kind;
@@ -727,6 +821,7 @@ lx_status lx_sim_step(lx_sim *s, int32_t nsteps) {
             }
         }
         enqueue_seq(s, n - done);
+        settle_parity(s, n - done);
         if ((st = check_launch())) return st;
         s->steps += n;
         left -= n;
@@ -851,6 +946,9 @@ lx_status lx_sim_get_info(lx_sim *s, lx_sim_info *info) {
     info->bytes_per_step = info->cells * per + slab_bytes;
     info->device_bytes = s->bytes;
     info->step = s->steps;
+    info->fused_step = s->fused;
+    info->kz = s->kz;
+    info->waves = s->fused ? s->nw : 4;
     return LX_OK;
 }
 
@@ -858,9 +956,9 @@ void *lx_sim_stream(lx_sim *s) { return s ? (void *)s->cs : nullptr; }
 
 static lx_status reset_state(lx_sim *s) {
     launch_zero_bytes(s->F, (size_t)s->fstride * 6 * s->esize, s->cs);
-    size_t total = 0;
-    for (int a = 0; a < 3; ++a) total += 4 * (size_t)s->slab_cells[a];
-    launch_zero_bytes(s->psi, std::max<size_t>(total, 1) * sizeof(float), s->cs);
+    launch_zero_bytes(s->psi, psi_total(s) * sizeof(float), s->cs);
+    if (s->F_alt) launch_zero_bytes(s->F_alt, (size_t)s->fstride * 6 * s->esize, s->cs);
+    if (s->psi_alt) launch_zero_bytes(s->psi_alt, psi_total(s) * sizeof(float), s->cs);
     launch_zero_bytes(s->counter, 64, s->cs);
     if (s->prec) launch_zero_bytes(s->prec, sizeof(double) * (size_t)s->pcap * s->np, s->cs);
     if (s->use_aux) {
@@ -877,15 +975,23 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
     if (!s) return fail(LX_EINVAL, "null simulation");
     if (s->steps != 0) return fail(LX_EINVAL, "lx_sim_autotune must run before the first step");
     HIP_TRY(hipSetDevice(s->ctx->device));
-    using Key = std::tuple<int, int, int, int, int, int>;
+    using Key = std::tuple<int, int, int, int, int, int, int>;
     static std::mutex mu;
-    static std::map<Key, int> cache;
-    const Key key{s->N[0], s->N[1], s->N[2], s->fp16, s->pml, s->planar};
+    static std::map<Key, std::array<int, 3>> cache;   // (kz, nw, fused)
+    const Key key{s->N[0], s->N[1], s->N[2], s->fp16, s->pml, s->planar, 0};
     {
         std::lock_guard<std::mutex> lk(mu);
         auto it = cache.find(key);
         if (it != cache.end()) {
-            s->kz = it->second;
+            s->kz = it->second[0];
+            s->nw = it->second[1];
+            if (it->second[2]) {
+                lx_status st = ensure_fused_buffers(s);
+                if (st) return st;
+                s->fused = 1;
+            } else if (s->F_alt && !s->planar) {
+                release_fused_buffers(s);
+            }
             drop_graph(s);
             if (kz_out) *kz_out = s->kz;
             return LX_OK;
@@ -896,42 +1002,71 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         float best = 1e30f;
-        int best_kz = s->kz;
-        for (int kz : {4, 8, 16, 32, 64}) {
-            if (kz > s->N[2] && kz != 4) continue;
-            s->kz = kz;
-            for (int rep = 0; rep < 2; ++rep) {
-                HIP_TRY(hipEventRecord(e0, s->cs));
-                for (int q = 0; q < 4; ++q) {
-                    if (s->fp16) {
-                        const Grid<__half> g = make_grid<__half>(s);
-                        launch_bulk(g, 0, 0, s->cs);
-                        launch_bulk(g, 0, 1, s->cs);
-                    } else {
-                        const Grid<float> g = make_grid<float>(s);
-                        launch_bulk(g, 0, 0, s->cs);
-                        launch_bulk(g, 0, 1, s->cs);
+        int best_kz = s->kz, best_nw = s->nw, best_fused = s->fused;
+        // candidates: the half-step pair (nw = 0) and the fused step with 8 or
+        // 4 wavefronts per block (its second buffer set is kept only if it wins)
+        const int can_fuse = ensure_fused_buffers(s) == LX_OK;
+        for (int nw : {0, 8, 4}) {
+            if (nw && !can_fuse) continue;
+            s->fused = nw != 0;
+            for (int kz : {4, 8, 16, 32, 64}) {
+                if (kz > s->N[2] && kz != 4) continue;
+                s->kz = kz;
+                if (nw) s->nw = nw;
+                for (int rep = 0; rep < 2; ++rep) {
+                    HIP_TRY(hipEventRecord(e0, s->cs));
+                    for (int q = 0; q < 4; ++q) {
+                        if (s->fp16) {
+                            Grid<__half> g = make_grid<__half>(s);
+                            if (s->fused) {
+                                g.Fn = (__half *)s->F_alt;
+                                g.psin = s->psi_alt;
+                                launch_step3(g, nw, s->tfH, s->tf_dh, s->tf_dp, s->cs);
+                            } else {
+                                launch_bulk(g, 0, 0, s->cs);
+                                launch_bulk(g, 0, 1, s->cs);
+                            }
+                        } else {
+                            Grid<float> g = make_grid<float>(s);
+                            if (s->fused) {
+                                g.Fn = (float *)s->F_alt;
+                                g.psin = s->psi_alt;
+                                launch_step3(g, nw, s->tfH, s->tf_dh, s->tf_dp, s->cs);
+                            } else {
+                                launch_bulk(g, 0, 0, s->cs);
+                                launch_bulk(g, 0, 1, s->cs);
+                            }
+                        }
                     }
+                    HIP_TRY(hipEventRecord(e1, s->cs));
+                    HIP_TRY(hipEventSynchronize(e1));
+                    float ms = 0;
+                    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                    if (rep == 1 && ms < best) {
+                        best = ms;
+                        best_kz = kz;
+                        best_fused = s->fused;
+                        if (nw) best_nw = nw;
+                    }
+                    if (rep == 1 && std::getenv("LX_VERBOSE"))
+                        std::fprintf(stderr, "lx_sim_autotune: %s nw=%d kz=%d  %.4f ms per step\n",
+                                     s->fused ? "fused" : "half-step", nw, kz, ms / 4);
                 }
-                HIP_TRY(hipEventRecord(e1, s->cs));
-                HIP_TRY(hipEventSynchronize(e1));
-                float ms = 0;
-                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                if (rep == 1 && ms < best) { best = ms; best_kz = kz; }
-                if (rep == 1 && std::getenv("LX_VERBOSE"))
-                    std::fprintf(stderr, "lx_sim_autotune: kz=%d  %.4f ms per step (bulk kernels)\n", kz, ms / 4);
             }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         s->kz = best_kz;
+        s->nw = best_nw;
+        s->fused = best_fused;
+        if (!s->fused) release_fused_buffers(s);
     }
     drop_graph(s);
     lx_status st = reset_state(s);
     if (st) return st;
     {
         std::lock_guard<std::mutex> lk(mu);
-        cache[key] = s->kz;
+        cache[key] = {s->kz, s->nw, s->fused};
     }
     if (kz_out) *kz_out = s->kz;
     return LX_OK;
@@ -946,6 +1081,7 @@ lx_status lx_sim_time_kernels(lx_sim *s, int32_t nsteps, double *ms_h, double *m
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(ev[4 * nsteps], s->cs));
     enqueue_seq(s, nsteps, &ev);
+    settle_parity(s, nsteps);
     HIP_TRY(hipEventRecord(ev[4 * nsteps + 1], s->cs));
     if ((st = check_launch())) return st;
     HIP_TRY(hipStreamSynchronize(s->cs));

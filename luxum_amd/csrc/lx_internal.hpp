@@ -111,6 +111,8 @@ struct Grid {
     int kz;                   // z planes per block (3D) / rows per block (2D)
     int ntx, nty, ntz, ntiles;
     int *counter;             // counter[0] = steps completed before this launch sequence
+    T *Fn;                    // fused 3D step: fields written (F is read)
+    float *psin;              // fused 3D step: psi written (psi is read)
 };
 
 struct Launch {
@@ -127,6 +129,21 @@ void launch_bulk(const Grid<T> &g, int planar, int half, hipStream_t s);
 template <typename T>
 void launch_tfsf(const Grid<T> &g, const TfList &tf, const IncParams &inc, const AuxLine &aux, const ProbeArgs &pr,
                  Launch l, int half, hipStream_t s);
+
+// Fused 3D timestep (H and E half steps in one launch, F/psi read, Fn/psin
+// written), nw wavefronts per block.  The H TF/SF corrections come
+// precomputed per face cell (dh, dp from launch_tfsf_hprep; may be NULL when
+// there are none); the E ones are applied by launch_tfsf on the written
+// buffer.
+template <typename T>
+void launch_step3(const Grid<T> &g, int nw, const TfList &tfH, const float *dh, const float *dp, hipStream_t s);
+
+// Per H face cell of the step: dh = s*(kh + c'h)*delta (subtracted from H)
+// and dp = c'h*delta (added to psi); an extra block advances the aux line's
+// H phase and records the E probes of the previous step (g.F is read only).
+template <typename T>
+void launch_tfsf_hprep(const Grid<T> &g, const TfList &tf, const IncParams &inc, const AuxLine &aux,
+                       const ProbeArgs &pr, Launch l, float *dh, float *dp, hipStream_t s);
 
 // End of a launch sequence of n steps: records the E probes of the last step
 // and advances counter[0] by n.

@@ -21,116 +21,9 @@
 // Block ids are remapped so that each of the 8 XCDs (which receive block ids
 // round robin) works on a contiguous range of tiles: tiles adjacent in y and z
 // share their halo rows and planes in that XCD's L2.
-#include "lx_internal.hpp"
-#include "../../include/luxum.h"
+#include "lx_device.hpp"
 
 namespace lx {
-
-// ---------------------------------------------------------------------------
-// loads and stores
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ float ld1(const T *p);
-template <>
-__device__ __forceinline__ float ld1<float>(const float *p) { return *p; }
-template <>
-__device__ __forceinline__ float ld1<__half>(const __half *p) { return __half2float(*p); }
-
-template <typename T>
-__device__ __forceinline__ void st1(T *p, float v);
-template <>
-__device__ __forceinline__ void st1<float>(float *p, float v) { *p = v; }
-template <>
-__device__ __forceinline__ void st1<__half>(__half *p, float v) { *p = __float2half(v); }
-
-template <typename T>
-__device__ __forceinline__ float4 ld4(const T *p);
-template <>
-__device__ __forceinline__ float4 ld4<float>(const float *p) { return *reinterpret_cast<const float4 *>(p); }
-template <>
-__device__ __forceinline__ float4 ld4<__half>(const __half *p) {
-    const uint2 u = *reinterpret_cast<const uint2 *>(p);
-    const __half2 a = *reinterpret_cast<const __half2 *>(&u.x), b = *reinterpret_cast<const __half2 *>(&u.y);
-    const float2 fa = __half22float2(a), fb = __half22float2(b);
-    return make_float4(fa.x, fa.y, fb.x, fb.y);
-}
-
-template <typename T>
-__device__ __forceinline__ void st4(T *p, float4 v);
-template <>
-__device__ __forceinline__ void st4<float>(float *p, float4 v) { *reinterpret_cast<float4 *>(p) = v; }
-template <>
-__device__ __forceinline__ void st4<__half>(__half *p, float4 v) {
-    const __half2 a = __floats2half2_rn(v.x, v.y), b = __floats2half2_rn(v.z, v.w);
-    uint2 u;
-    u.x = *reinterpret_cast<const unsigned *>(&a);
-    u.y = *reinterpret_cast<const unsigned *>(&b);
-    *reinterpret_cast<uint2 *>(p) = u;
-}
-
-__device__ __forceinline__ float4 f4(float v) { return make_float4(v, v, v, v); }
-__device__ __forceinline__ float4 operator-(float4 a, float4 b) { return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
-__device__ __forceinline__ float4 operator+(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
-__device__ __forceinline__ float4 operator*(float4 a, float4 b) { return make_float4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w); }
-__device__ __forceinline__ float4 operator*(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
-__device__ __forceinline__ float get(const float4 &v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
-__device__ __forceinline__ void set(float4 &v, int e, float s) {
-    if (e == 0) v.x = s; else if (e == 1) v.y = s; else if (e == 2) v.z = s; else v.w = s;
-}
-// keep the old value where the cell is outside the component's update range
-__device__ __forceinline__ float4 sel(unsigned m, float4 nw, float4 od) {
-    return make_float4((m & 1) ? nw.x : od.x, (m & 2) ? nw.y : od.y, (m & 4) ? nw.z : od.z, (m & 8) ? nw.w : od.w);
-}
-// 4-bit mask of elements e (x = i0+e) with lo <= x <= hi
-__device__ __forceinline__ unsigned xmask(int i0, int lo, int hi) {
-    unsigned m = 0;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) m |= (i0 + e >= lo && i0 + e <= hi) ? (1u << e) : 0u;
-    return m;
-}
-
-
-__device__ __forceinline__ int slab_index(int x, int lo, int hi) { return x < lo ? x : lo + (x - hi); }
-
-// psi arrays: axis a holds [H slot0, H slot1, E slot0, E slot1]; slot w of
-// axis a belongs to split curl w of component (a + 2 - w) % 3.
-template <typename T>
-__device__ __forceinline__ float *psi_base(const Grid<T> &g, int a, int type_slot) {
-    long long base = a == 0 ? 0 : (a == 1 ? 4 * g.scells[0] : 4 * (g.scells[0] + g.scells[1]));
-    return g.psi + base + (long long)type_slot * g.scells[a];
-}
-// psi cell offset inside one array of axis a
-template <typename T>
-__device__ __forceinline__ long long psi_off(const Grid<T> &g, int a, int i, int j, int k) {
-    if (a == 0) return ((long long)k * g.ny + j) * g.sw[0] + slab_index(i, g.slo[0], g.shi[0]);
-    if (a == 1) return ((long long)k * g.sw[1] + slab_index(j, g.slo[1], g.shi[1])) * g.px + i;
-    return ((long long)slab_index(k, g.slo[2], g.shi[2]) * g.ny + j) * g.px + i;
-}
-
-__device__ double waveform(int kind, double t, double c, double lambda0, double fwhm, double delay) {
-    const double f0 = c / lambda0, T0 = 1.0 / f0;
-    const double tau = fwhm * T0 / (2.0 * sqrt(log(2.0)));
-    const double t0 = delay * tau;
-    const double x = (t - t0) / tau;
-    const double env = exp(-x * x);
-    if (kind == LX_WAVE_GAUSSIAN) return env;
-    return env * sin(2.0 * M_PI * f0 * (t - t0));
-}
-
-// Incident scalar at node coordinates (cells) and time level tp (steps).
-__device__ double inc_value(const IncParams &in, double cx, double cy, double cz, double tp, bool e_node) {
-    if (in.mode == LX_INC_AUX1D) {
-        const double cc = in.axis == 0 ? cx : (in.axis == 1 ? cy : cz);
-        const int m = (int)floor(cc) - in.x0;
-        if (m < 0 || m >= in.L) return 0.0;
-        if (e_node) return in.e[m];
-        return m < in.L - 1 ? in.h[m] : 0.0;
-    }
-    const double pos = in.planar ? (cx * in.dx * in.kx + cz * in.dz * in.kz)
-                                 : (cx * in.dx * in.kx + cy * in.dy * in.ky + cz * in.dz * in.kz);
-    const double arg = pos + in.init_dx - tp * in.cdt;
-    return exp(-(arg * arg) * in.inv_sigma);
-}
 
 // ---------------------------------------------------------------------------
 // 3D H update:  Hs -= (d1*kh_a1 - d2*kh_a2) + (psi1 - psi2)
@@ -511,33 +404,6 @@ __global__ __launch_bounds__(256) void k_e2(Grid<T> g) {
 // ---------------------------------------------------------------------------
 // TF/SF face corrections + aux line + probes (one launch per half step)
 // ---------------------------------------------------------------------------
-// H half (IS_H): aux line H phase, E probes of step t-1.
-// E half:        aux line E phase (source sample t+1), H probes of step t.
-template <typename T, bool IS_H>
-__device__ void extra_block(const Grid<T> &g, const AuxLine &aux, const ProbeArgs &pr, Launch l, int t) {
-    if (IS_H) {
-        if (l.aux)
-            for (int m = threadIdx.x; m < aux.L - 1; m += blockDim.x) aux.h[m] = aux.h[m] - aux.S * (aux.e[m + 1] - aux.e[m]);
-        if (l.probes && t >= 1 && t - 1 < pr.cap)
-            for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
-                if (pr.comp[q] < 3)
-                    pr.rec[(long long)(t - 1) * pr.n + q] = (double)ld1(g.F + pr.comp[q] * g.fstride + pr.idx[q]);
-    } else {
-        if (l.aux) {
-            for (int m = 1 + threadIdx.x; m < aux.L - 1; m += blockDim.x)
-                aux.e[m] = aux.e[m] - aux.S * (aux.h[m] - aux.h[m - 1]);
-            if (threadIdx.x == 0) {
-                aux.e[0] = waveform(aux.wave, (t + 1) * aux.dt, aux.c, aux.lambda0, aux.fwhm, aux.delay);
-                aux.e[aux.L - 1] = 0.0;
-            }
-        }
-        if (l.probes && t < pr.cap)
-            for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
-                if (pr.comp[q] >= 3)
-                    pr.rec[(long long)t * pr.n + q] = (double)ld1(g.F + pr.comp[q] * g.fstride + pr.idx[q]) * pr.hscale;
-    }
-}
-
 // One thread per face cell.  A cell of component c gets, for split curl w
 // (axis a = (c+1+w)%3, sign s = +1 for w = 0, -1 for w = 1) and delta = dir*g:
 //   H:  Hs -= s*(kh_a + c'h_a)*delta,     psiH_a[w] += c'h_a*delta
@@ -613,6 +479,48 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict
         const float cb = g.cb[g.mat[o]];
         st1(F + o, ld1(F + o) + cb * upd);
     }
+}
+
+// Fused 3D step: the H corrections of the step, per face cell, for k_step3
+// (which adds them to the cells it updates, halo cells included):
+//   dh = s*(kh_a + c'h_a)*delta,  dp = c'h_a*delta  (c' only on slab cells)
+// with the same delta and coefficients as k_tfsf<H>.  The extra block runs
+// the aux line's H phase (it writes h; the face cells read e) and records
+// the E probes of the previous step (E is read only in this launch).
+template <typename T>
+__global__ __launch_bounds__(256) void k_tfsf_hprep(Grid<T> g, const TfBox *__restrict__ tb, int ntf, int total,
+                                                    IncParams inc, AuxLine aux, ProbeArgs pr, Launch l,
+                                                    float *__restrict__ dh, float *__restrict__ dp) {
+    const int t = g.counter[0] + l.toff;
+    const int nb = (total + 255) / 256;
+    if ((int)blockIdx.x >= nb) {
+        extra_block<T, true>(g, aux, pr, l, t);
+        return;
+    }
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    if (gid >= total) return;
+    int b = 0;
+    while (b + 1 < ntf && gid >= tb[b + 1].start) ++b;
+    const TfBox &bx = tb[b];
+    const int fa = bx.fa, u = (fa + 1) % 3, v = (fa + 2) % 3;
+    const int nu = bx.hi[u] - bx.lo[u] + 1;
+    const int loc = gid - bx.start;
+    int ijk[3];
+    ijk[fa] = bx.lo[fa];
+    ijk[u] = bx.lo[u] + loc % nu;
+    ijk[v] = bx.lo[v] + loc / nu;
+    const double gi = inc_value(inc, ijk[0] + (double)bx.off[0], ijk[1] + (double)bx.off[1], ijk[2] + (double)bx.off[2],
+                                t - 0.5, true);
+    const float delta = (float)(bx.dir * gi);
+    const int a = (bx.comp + 1 + bx.curl) % 3;
+    const int ia = ijk[a];
+    float coef = g.tab[(a * 6 + TK_KH) * g.tn + ia], cp = 0.f;
+    if (g.pml && (ia < g.slo[a] || ia >= g.shi[a])) {
+        cp = g.tab[(a * 6 + TK_CH) * g.tn + ia];
+        coef += cp;
+    }
+    dh[gid] = (bx.curl == 0 ? 1.f : -1.f) * coef * delta;
+    dp[gid] = cp * delta;
 }
 
 // ---------------------------------------------------------------------------
@@ -750,6 +658,15 @@ void launch_tfsf(const Grid<T> &g, const TfList &tf, const IncParams &inc, const
 }
 
 template <typename T>
+void launch_tfsf_hprep(const Grid<T> &g, const TfList &tf, const IncParams &inc, const AuxLine &aux,
+                       const ProbeArgs &pr, Launch l, float *dh, float *dp, hipStream_t s) {
+    const int nb = (tf.total + 255) / 256;
+    const unsigned blocks = (unsigned)nb + ((l.aux || l.probes) ? 1u : 0u);
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_tfsf_hprep<T>, dim3(blocks), dim3(256), 0, s, g, tf.dev, tf.n, tf.total, inc, aux, pr, l, dh, dp);
+}
+
+template <typename T>
 void launch_finish(const Grid<T> &g, const ProbeArgs &pr, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_finish<T>, dim3(1), dim3(64), 0, s, g, pr, n);
 }
@@ -787,6 +704,8 @@ void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
     template void launch_tfsf<T>(const Grid<T> &, const TfList &, const IncParams &, const AuxLine &,        \
                                  const ProbeArgs &, Launch, int, hipStream_t);                               \
     template void launch_finish<T>(const Grid<T> &, const ProbeArgs &, int, hipStream_t);                    \
+    template void launch_tfsf_hprep<T>(const Grid<T> &, const TfList &, const IncParams &, const AuxLine &,  \
+                                       const ProbeArgs &, Launch, float *, float *, hipStream_t);           \
     template void launch_point_source<T>(const Grid<T> &, const PointSrc &, int, hipStream_t);               \
     template void launch_gather<T>(const Grid<T> &, int, int, int, int, double, double *, hipStream_t);      \
     template void launch_monitor<T>(const Grid<T> &, int, double *, hipStream_t);

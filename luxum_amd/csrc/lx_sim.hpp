@@ -45,6 +45,9 @@ struct lx_sim {
     int64_t slab_cells[3] = {0, 0, 0};
     lx::TfList tfH{}, tfE{};
     int kz = 32;
+    int fused = 0;              // 3D: one fused launch per step over ping-pong buffers (F/psi and F_alt/psi_alt)
+    int nw = 8;                 // fused step: wavefronts per block
+    float *tf_dh = nullptr, *tf_dp = nullptr;   // fused step: H TF/SF corrections per face cell
     lx::IncParams inc{};
     lx::AuxLine aux{};
     int use_aux = 0;
@@ -82,4 +85,6 @@ lx_status dalloc(lx_sim *s, void **out, size_t bytes);
 size_t psi_total(const lx_sim *s);
 ProbeArgs probe_args(const lx_sim *s);
 void drop_graph(lx_sim *s);
+lx_status ensure_fused_buffers(lx_sim *s);
+void release_fused_buffers(lx_sim *s);
 }  // namespace lx

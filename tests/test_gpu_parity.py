@@ -171,3 +171,47 @@ def test_chunked_stepping_matches(gpu_ctx):
         b.step(1)
     for c in w.p.components():
         assert np.array_equal(a.field(c), b.field(c))
+
+
+def test_fused_step_matches_half_steps(gpu_ctx, monkeypatch):
+    """The fused 3D timestep (one launch, ping-pong buffers) against the two
+    half-step kernels on the same inputs: CPML + aux TF/SF + dielectric, an
+    odd number of steps (state in the second buffer), probes; and the oblique
+    analytic plane wave.  Same arithmetic, so the tolerance is 1e-6."""
+    from luxum_amd.sim import Simulation
+
+    def run(p, steps, w=None, fused=True):
+        monkeypatch.setenv("LX_FUSED", "1" if fused else "0")
+        s = Simulation(p, gpu_ctx)
+        if w is not None:
+            s.set_materials(w.ids, w.eps_r)
+            s.set_probes(w.probes)
+        assert bool(s.info().fused_step) == fused
+        s.step(steps)
+        s.sync()
+        return s
+
+    w = configs.config1(n=48, steps=201, pml=8, margin=6)
+    a, b = run(w.p, 201, w, True), run(w.p, 201, w, False)
+    num = den = 0.0
+    for c in w.p.components():
+        x, y = a.field(c), b.field(c)
+        num += float(np.sum((x - y) ** 2))
+        den += float(np.sum(y ** 2))
+    e = (num / den) ** 0.5
+    pe = rel_l2(a.probes(), b.probes())
+    say(f"fused vs half-step (cpml aux, 201 steps): fields {e:.2e} probes {pe:.2e}")
+    assert e <= 1e-6 and pe <= 1e-6
+    p = default_params(nx=40, ny=36, nz=44, use_pw=1, incident=LX_INC_ANALYTIC, off_pw=90.0,
+                       pw_sigma=(20 * 5e-8) ** 2, tf_lo=(8, 7, 9), tf_hi=(30, 27, 33), courant=0.5)
+    p.k_pw[:] = (0.6, 0.48, 0.64)
+    p.e_pw[:] = (0.6246950475544243, -0.7808688094430304, 0.0)
+    a, b = run(p, 150), run(p, 150, fused=False)
+    num = den = 0.0
+    for c in p.components():
+        x, y = a.field(c), b.field(c)
+        num += float(np.sum((x - y) ** 2))
+        den += float(np.sum(y ** 2))
+    e = (num / den) ** 0.5
+    say(f"fused vs half-step (plain, oblique analytic): fields {e:.2e}")
+    assert e <= 1e-6
