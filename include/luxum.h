@@ -295,6 +295,42 @@ lx_status lx_server_probes(lx_server *srv, int64_t id, double *dst, size_t n);
 lx_status lx_server_field(lx_server *srv, int64_t id, int32_t comp, double *dst, size_t n);
 lx_status lx_server_get_stats(lx_server *srv, lx_server_stats *st);
 
+/* ---- HDF5 output and input (reference layout) --------------------------- */
+
+/* Replaces fileio (lib/io/fileio.cc:55-247) for one process: one file per
+ * output, one FP64 dataset per snapshot named "t=<t>" (t formatted as the
+ * reference's ostream does), shaped by the field's global extents (staggered
+ * axes n-1), C order with k fastest.  LX_OUT_E2 is the 3D solvers' output:
+ * |E|^2 at equal indices in an Ez-shaped array, zero outside i<Nx-1,
+ * j<Ny-1, k<Nz-1 (maxwell_three_dim_pml.cc:264-280); the 2D solvers write
+ * the field Ey (maxwell_two_dim_pml.cc:123,185).  libhdf5 (1.10 C API) is
+ * loaded on first use from LX_HDF5_LIB or the usual sonames. */
+enum { LX_OUT_FIELD = 0, LX_OUT_E2 = 1 };
+typedef struct lx_writer lx_writer;
+
+/* Create (truncate) the file; comp is used for LX_OUT_FIELD.  Close every
+ * writer of a simulation before destroying the simulation. */
+lx_status lx_writer_open(lx_sim *sim, const char *path, int32_t what, int32_t comp, lx_writer **out);
+/* Snapshot of the state after the steps enqueued so far, as dataset "t=<t>".
+ * Asynchronous: the step loop waits only for a gather kernel; the copy to
+ * host and the file write run on the side stream and a writer thread (at
+ * most two snapshots in flight per writer). */
+lx_status lx_writer_write(lx_writer *w, double t);
+/* Drain pending snapshots and close the file; reports the first write error. */
+lx_status lx_writer_close(lx_writer *w);
+/* Snapshots written to the file so far. */
+int64_t lx_writer_count(lx_writer *w);
+
+/* Write one FP64 dataset "t=<t>" of shape dims (create the file, or append
+ * to an existing one). */
+lx_status lx_h5_write(const char *path, double t, const double *data, const int64_t dims[3], int32_t append);
+/* Replaces fileio::readHDF5 (fileio.cc:224-247): read an FP64 dataset; dims
+ * receives its shape (rank < 3 padded with 1).  dst may be NULL to query
+ * the shape only. */
+lx_status lx_h5_read(const char *path, const char *dataset, double *dst, size_t n, int64_t dims[3]);
+/* The dataset name the reference gives snapshot t ("t=100", "t=1e+06"). */
+lx_status lx_h5_dataset_name(double t, char *buf, size_t n);
+
 /* ---- errors ------------------------------------------------------------- */
 const char *lx_last_error(void);
 const char *lx_version(void);

@@ -848,6 +848,22 @@ static lx_status gather(lx_sim *s, int comp, double *dst_dev, int ext[3]) {
     return check_launch();
 }
 
+}  // extern "C"
+
+namespace lx {
+lx_status gather_output(lx_sim *s, int what, int comp, double *dst_dev, const int ext[3]) {
+    if (what == LX_OUT_FIELD) {
+        int e[3] = {ext[0], ext[1], ext[2]};
+        return gather(s, comp, dst_dev, e);
+    }
+    if (s->fp16) launch_gather_e2(make_grid<__half>(s), ext[0], ext[1], ext[2], dst_dev, s->cs);
+    else launch_gather_e2(make_grid<float>(s), ext[0], ext[1], ext[2], dst_dev, s->cs);
+    return check_launch();
+}
+}  // namespace lx
+
+extern "C" {
+
 lx_status lx_sim_read_field(lx_sim *s, int32_t comp, double *dst, size_t n) {
     if (!s || !dst) return fail(LX_EINVAL, "lx_sim_read_field: null pointer");
     int ext[3];

@@ -559,6 +559,23 @@ __global__ void k_gather(Grid<T> g, const T *F, int ex, int ey, int ez, double s
     dst[((long long)i * ey + j) * ez + k] = v;
 }
 
+// |E|^2 at equal storage indices into the Ez-shaped host layout, zero
+// outside i < nx-1, j < ny-1, k < nz-1 (maxwell_three_dim_pml.cc:264-272).
+template <typename T>
+__global__ void k_gather_e2(Grid<T> g, int ex, int ey, int ez, double *dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    if (i >= ex) return;
+    double v = 0.0;
+    if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
+        const long long o = (long long)k * g.plane + (long long)j * g.px + i;
+        const double a = ld1(g.F + o), b = ld1(g.F + g.fstride + o), c = ld1(g.F + 2 * g.fstride + o);
+        v = c * c + b * b + a * a;
+    }
+    dst[((long long)i * ey + j) * ez + k] = v;
+}
+
 __device__ __forceinline__ void atomic_max_pos(double *addr, double v) {
     // non-negative doubles order like their bit patterns as unsigned integers
     atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
@@ -681,6 +698,11 @@ void launch_gather(const Grid<T> &g, int comp, int ex, int ey, int ez, double sc
     hipLaunchKernelGGL(k_gather<T>, grd, blk, 0, s, g, F, ex, ey, ez, scale, dst);
 }
 template <typename T>
+void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double *dst, hipStream_t s) {
+    dim3 blk(256), grd((ex + 255) / 256, ey, ez);
+    hipLaunchKernelGGL(k_gather_e2<T>, grd, blk, 0, s, g, ex, ey, ez, dst);
+}
+template <typename T>
 void launch_monitor(const Grid<T> &g, int planar, double *out, hipStream_t s) {
     dim3 grd((g.nx + 255) / 256, g.ny, g.nz);
     hipLaunchKernelGGL(k_monitor<T>, grd, dim3(256), 0, s, g, planar, out);
@@ -708,7 +730,8 @@ void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
                                        const ProbeArgs &, Launch, float *, float *, hipStream_t);           \
     template void launch_point_source<T>(const Grid<T> &, const PointSrc &, int, hipStream_t);               \
     template void launch_gather<T>(const Grid<T> &, int, int, int, int, double, double *, hipStream_t);      \
-    template void launch_monitor<T>(const Grid<T> &, int, double *, hipStream_t);
+    template void launch_monitor<T>(const Grid<T> &, int, double *, hipStream_t);                            \
+    template void launch_gather_e2<T>(const Grid<T> &, int, int, int, double *, hipStream_t);
 LX_INST(float)
 LX_INST(__half)
 

@@ -86,5 +86,8 @@ size_t psi_total(const lx_sim *s);
 ProbeArgs probe_args(const lx_sim *s);
 void drop_graph(lx_sim *s);
 lx_status ensure_fused_buffers(lx_sim *s);
+// enqueue on the compute stream: one component (what == LX_OUT_FIELD) or
+// |E|^2 (LX_OUT_E2) into dst_dev in the FP64 host layout of extents ext
+lx_status gather_output(lx_sim *s, int what, int comp, double *dst_dev, const int ext[3]);
 void release_fused_buffers(lx_sim *s);
 }  // namespace lx

@@ -160,6 +160,59 @@ class Batch:
             self._h = C.c_void_p()
 
 
+LX_OUT_FIELD, LX_OUT_E2 = 0, 1
+
+
+class Writer:
+    """Asynchronous HDF5 snapshot writer in the reference's layout (one file,
+    datasets "t=<t>"): LX_OUT_FIELD writes one component, LX_OUT_E2 the 3D
+    solvers' |E|^2 at equal indices."""
+
+    def __init__(self, sim: "Simulation", path: str, what: int = LX_OUT_FIELD, comp: int = 2):
+        self.sim = sim
+        self._h = C.c_void_p()
+        check(lib().lx_writer_open(sim._h, path.encode(), int(what), int(comp), C.byref(self._h)))
+
+    def write(self, t: float) -> None:
+        check(lib().lx_writer_write(self._h, float(t)))
+
+    def count(self) -> int:
+        return int(lib().lx_writer_count(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            h, self._h = self._h, C.c_void_p()
+            check(lib().lx_writer_close(h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def h5_dataset_name(t: float) -> str:
+    buf = C.create_string_buffer(64)
+    check(lib().lx_h5_dataset_name(float(t), buf, 64))
+    return buf.value.decode()
+
+
+def h5_write(path: str, t: float, data: np.ndarray, append: bool = False) -> None:
+    a = np.ascontiguousarray(data, dtype=np.float64)
+    if a.ndim != 3:
+        raise ValueError("datasets are 3D")
+    dims = (C.c_int64 * 3)(*a.shape)
+    check(lib().lx_h5_write(path.encode(), float(t), a.ctypes.data, dims, int(append)))
+
+
+def h5_read(path: str, dataset: str) -> np.ndarray:
+    dims = (C.c_int64 * 3)()
+    check(lib().lx_h5_read(path.encode(), dataset.encode(), None, 0, dims))
+    out = np.empty(tuple(dims), dtype=np.float64)
+    check(lib().lx_h5_read(path.encode(), dataset.encode(), out.ctypes.data, out.size, dims))
+    return out
+
+
 def load_yaml(path: str) -> LxParams:
     """reader::readYAML + checkData (lib/io/reader.cc:68-162) through the C ABI."""
     p = default_params()
@@ -171,4 +224,5 @@ def params_check(p: LxParams) -> None:
     check(lib().lx_params_check(C.byref(p)))
 
 
-__all__ = ["Context", "Simulation", "Batch", "load_yaml", "params_check", "default_context", "_lib"]
+__all__ = ["Context", "Simulation", "Batch", "Writer", "load_yaml", "params_check", "default_context", "h5_read",
+           "h5_write", "h5_dataset_name", "LX_OUT_FIELD", "LX_OUT_E2", "_lib"]

@@ -1,0 +1,36 @@
+"""HDF5 output in the reference's layout (lib/io/fileio.cc): dataset names,
+FP64 round trip, append, errors.  Runs on the CPU (no simulation needed)."""
+import numpy as np
+import pytest
+
+from luxum_amd._lib import LuxumError
+from luxum_amd.sim import h5_dataset_name, h5_read, h5_write
+
+
+def test_dataset_names_follow_the_reference_ostream():
+    # fileio.cc:175 builds the name with  temp << "t=" << t  (t a double,
+    # default ostream formatting: %g with 6 significant digits)
+    for t, name in [(0, "t=0"), (3, "t=3"), (100, "t=100"), (999999, "t=999999"), (1000000, "t=1e+06"),
+                    (1234567, "t=1.23457e+06"), (0.5, "t=0.5"), (2000, "t=2000")]:
+        assert h5_dataset_name(t) == name
+
+
+def test_round_trip_append_and_errors(tmp_path):
+    path = str(tmp_path / "Ez_data.h5")
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((5, 4, 3))
+    b = rng.standard_normal((5, 4, 3))
+    h5_write(path, 0, a)
+    h5_write(path, 3, b, append=True)
+    assert np.array_equal(h5_read(path, "t=0"), a)
+    assert np.array_equal(h5_read(path, "t=3"), b)
+    with pytest.raises(LuxumError):
+        h5_write(path, 3, b, append=True)          # dataset exists
+    with pytest.raises(LuxumError):
+        h5_read(path, "t=6")                       # no such dataset
+    with pytest.raises(LuxumError):
+        h5_read(str(tmp_path / "missing.h5"), "t=0")
+    h5_write(path, 6, a)                           # truncates
+    with pytest.raises(LuxumError):
+        h5_read(path, "t=0")
+    assert np.array_equal(h5_read(path, "t=6"), a)
