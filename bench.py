@@ -4,6 +4,8 @@ Workload: 3D Yee + CPML, FP32, 256^3 nodes, CPML 12 cells on all faces,
 TF/SF plane wave along +x (FP64 aux-1D line, Ez-polarised Gaussian-modulated
 pulse), centred eps_r = 6 cube of side 64, 8 Ez probes recorded every step.
 A step is one full timestep (H half step + E half step) of the whole grid.
+--workload phc_slab measures the 1280x1280x640 photonic-crystal slab
+waveguide (soft Ez dipole source) instead, main line only.
 
 Prints ONE JSON line: Gcell-updates/s (whole job), the roofline of the
 dominant kernel (algorithmic bytes per launch / average launch duration,
@@ -165,6 +167,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--workload", choices=["cube", "phc_slab"], default="cube",
+                    help="cube: 3D CPML dielectric cube with a TF/SF plane wave (--n^3); phc_slab: 1280x1280x640 "
+                         "photonic-crystal slab waveguide driven by a soft Ez dipole (no batch, serving or CPU "
+                         "sections: its FP64 CPU state would not fit in host memory)")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--fp16", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=150)
@@ -200,7 +206,16 @@ def main():
         del a, b
 
     prec = LX_FP16 if args.fp16 else LX_FP32
-    w = configs.config1(n=args.n, steps=max(args.steps + args.warmup + args.kernel_steps, 2000), precision=prec)
+    total = max(args.steps + args.warmup + args.kernel_steps, 2000)
+    tag = "FP16 storage" if args.fp16 else "FP32"
+    if args.workload == "phc_slab":
+        w = configs.config2(steps=total, precision=prec)
+        args.no_batch = args.no_serve = args.no_cpu = True
+        desc = (f"3D PhC slab waveguide {'x'.join(map(str, w.p.dims))}, rods eps_r 12 pitch 32 side 13 in a "
+                f"64-cell slab, one row removed, CPML 16, soft Ez dipole (a/lambda 0.3), 4 probes, {tag}")
+    else:
+        w = configs.config1(n=args.n, steps=total, precision=prec)
+        desc = f"3D Yee+CPML {args.n}^3, CPML 12, TF/SF aux-1D plane wave +x, eps_r=6 cube, 8 probes, {tag}"
     ctx = Context(0)
     sim = Simulation(w.p, ctx)
     sim.set_materials(w.ids, w.eps_r)
@@ -250,8 +265,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f16-storage/f32" if args.fp16 else "f32",
         "data": "synthetic",
-        "config": {"workload": f"3D Yee+CPML {args.n}^3, CPML 12, TF/SF aux-1D plane wave +x, eps_r=6 cube, "
-                               f"8 probes, {'FP16 storage' if args.fp16 else 'FP32'}",
+        "config": {"workload": desc,
                    "cells": info.cells, "bytes_per_step": info.bytes_per_step,
                    "step_GBps": round(step_gbs, 1), "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
                    "ms_h": round(ms_h, 5), "ms_e": round(ms_e, 5), "max_E2": mx, "kz": info.kz,

@@ -324,6 +324,10 @@ int64_t lx_writer_count(lx_writer *w);
 /* Write one FP64 dataset "t=<t>" of shape dims (create the file, or append
  * to an existing one). */
 lx_status lx_h5_write(const char *path, double t, const double *data, const int64_t dims[3], int32_t append);
+/* The same under any dataset name, e.g. the "data" of the material inputs
+ * the 3D CPML solver reads (maxwell_three_dim_pml.cc:109-118). */
+lx_status lx_h5_write_dataset(const char *path, const char *name, const double *data, const int64_t dims[3],
+                              int32_t append);
 /* Replaces fileio::readHDF5 (fileio.cc:224-247): read an FP64 dataset; dims
  * receives its shape (rank < 3 padded with 1).  dst may be NULL to query
  * the shape only. */

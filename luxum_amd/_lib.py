@@ -31,7 +31,7 @@ EXPORTS = (
     "lx_params_footprint", "lx_server_create", "lx_server_destroy", "lx_server_submit", "lx_server_run",
     "lx_server_info", "lx_server_probes", "lx_server_field", "lx_server_get_stats",
     "lx_writer_open", "lx_writer_write", "lx_writer_close", "lx_writer_count",
-    "lx_h5_write", "lx_h5_read", "lx_h5_dataset_name",
+    "lx_h5_write", "lx_h5_write_dataset", "lx_h5_read", "lx_h5_dataset_name",
     "lx_last_error", "lx_version",
 )
 
@@ -154,6 +154,7 @@ def lib():
         "lx_writer_close": ([vp], st),
         "lx_writer_count": ([vp], C.c_int64),
         "lx_h5_write": ([C.c_char_p, C.c_double, vp, C.POINTER(C.c_int64), C.c_int32], st),
+        "lx_h5_write_dataset": ([C.c_char_p, C.c_char_p, vp, C.POINTER(C.c_int64), C.c_int32], st),
         "lx_h5_read": ([C.c_char_p, C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_int64)], st),
         "lx_h5_dataset_name": ([C.c_double, C.c_char_p, C.c_size_t], st),
         "lx_last_error": ([], C.c_char_p),

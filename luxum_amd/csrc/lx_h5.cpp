@@ -199,8 +199,9 @@ static void writer_loop(lx_writer *w) {
 
 extern "C" {
 
-lx_status lx_h5_write(const char *path, double t, const double *data, const int64_t dims[3], int32_t append) {
-    if (!path || !data || !dims) return fail(LX_EINVAL, "lx_h5_write: null pointer");
+lx_status lx_h5_write_dataset(const char *path, const char *name, const double *data, const int64_t dims[3],
+                             int32_t append) {
+    if (!path || !name || !data || !dims) return fail(LX_EINVAL, "lx_h5_write: null pointer");
     H5 *a;
     lx_status st = need_h5(&a);
     if (st) return st;
@@ -208,9 +209,13 @@ lx_status lx_h5_write(const char *path, double t, const double *data, const int6
     const hid_t f = append ? a->Fopen(path, kAccRdwr, kDefault) : a->Fcreate(path, kAccTrunc, kDefault, kDefault);
     if (f < 0) return fail(LX_EIO, "cannot %s %s", append ? "open" : "create", path);
     const hsize_t d[3] = {(hsize_t)dims[0], (hsize_t)dims[1], (hsize_t)dims[2]};
-    st = put(a, f, dataset_name(t), data, d);
+    st = put(a, f, name, data, d);
     a->Fclose(f);
     return st;
+}
+
+lx_status lx_h5_write(const char *path, double t, const double *data, const int64_t dims[3], int32_t append) {
+    return lx_h5_write_dataset(path, dataset_name(t).c_str(), data, dims, append);
 }
 
 lx_status lx_h5_read(const char *path, const char *dataset, double *dst, size_t n, int64_t dims[3]) {

@@ -197,12 +197,16 @@ def h5_dataset_name(t: float) -> str:
     return buf.value.decode()
 
 
-def h5_write(path: str, t: float, data: np.ndarray, append: bool = False) -> None:
+def h5_write(path: str, t, data: np.ndarray, append: bool = False) -> None:
+    """Write dataset "t=<t>" (t a number) or the dataset named t (a str)."""
     a = np.ascontiguousarray(data, dtype=np.float64)
     if a.ndim != 3:
         raise ValueError("datasets are 3D")
     dims = (C.c_int64 * 3)(*a.shape)
-    check(lib().lx_h5_write(path.encode(), float(t), a.ctypes.data, dims, int(append)))
+    if isinstance(t, str):
+        check(lib().lx_h5_write_dataset(path.encode(), t.encode(), a.ctypes.data, dims, int(append)))
+    else:
+        check(lib().lx_h5_write(path.encode(), float(t), a.ctypes.data, dims, int(append)))
 
 
 def h5_read(path: str, dataset: str) -> np.ndarray:

@@ -27,3 +27,18 @@ def rel_l2(a, b) -> float:
 def gpu_ctx():
     from luxum_amd.sim import Context
     return Context(0)
+
+
+def reference_yaml(base: str, dst, **over) -> str:
+    """Copy a reference-format parameters.yaml from tests/golden with some
+    "Key": value entries replaced (keys as the reference spells them)."""
+    import re
+    text = open(os.path.join(ROOT, "tests", "golden", base), encoding="utf-8-sig").read()
+    for key, val in over.items():
+        if isinstance(val, bool):
+            val = "true" if val else "false"
+        text, n = re.subn(r'("%s"\s*:\s*)[^,\n]+' % re.escape(key), r"\g<1>%s" % val, text)
+        assert n == 1, key
+    with open(dst, "w") as f:
+        f.write(text)
+    return str(dst)

@@ -34,3 +34,14 @@ def test_round_trip_append_and_errors(tmp_path):
     with pytest.raises(LuxumError):
         h5_read(path, "t=0")
     assert np.array_equal(h5_read(path, "t=6"), a)
+
+
+def test_named_dataset(tmp_path):
+    # material inputs of the 3D CPML solver carry one dataset named "data"
+    # (maxwell_three_dim_pml.cc:116-118)
+    path = str(tmp_path / "airplane_100_x.h5")
+    a = np.arange(24, dtype=np.float64).reshape(2, 3, 4) * 1e-6
+    h5_write(path, "data", a)
+    assert np.array_equal(h5_read(path, "data"), a)
+    with pytest.raises(LuxumError):
+        h5_read(path, "t=0")
