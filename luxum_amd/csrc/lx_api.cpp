@@ -433,9 +433,9 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
             gq.Fn = buf[(q + 1) & 1];
             gq.psin = pbuf[(q + 1) & 1];
             if (s->p.use_ps) launch_point_source(gq, s->ps, q, s->cs);
-            if (corr) launch_tfsf_hprep(gq, s->tfH, s->inc, s->aux, pr, l, s->tf_dh, s->tf_dp, s->cs);
+            if (corr) launch_tfsf_hpre(gq, s->tfH, s->inc, s->aux, pr, l, s->cs);
             if (ev) (void)hipEventRecord((*ev)[4 * q + 0], s->cs);
-            launch_step3(gq, s->nw, s->tfH, s->tf_dh, s->tf_dp, s->cs);
+            launch_step3(gq, s->nw, s->cs);
             if (ev) (void)hipEventRecord((*ev)[4 * q + 1], s->cs);
             Grid<T> gn = g;
             gn.F = gq.Fn;
@@ -541,8 +541,20 @@ size_t psi_total(const lx_sim *s) {
     return std::max<size_t>(total, 1);
 }
 
-// Second field/psi buffer set and the per-face-cell H corrections of the
-// fused 3D step.
+// The fused 3D step applies the H TF/SF corrections to the read buffer
+// before the step, which leaves out the CPML psi term: it needs every H face
+// outside the slabs along its face axis.
+bool fused_ok(const lx_sim *s) {
+    if (s->planar) return false;
+    for (int b = 0; b < s->tfH.n; ++b) {
+        const TfBox &bx = s->tfH.b[b];
+        const int a = bx.fa, c = bx.lo[a];
+        if (s->pml && (c < s->slo[a] || c >= s->shi[a])) return false;
+    }
+    return true;
+}
+
+// Second field/psi buffer set of the fused 3D step.
 lx_status ensure_fused_buffers(lx_sim *s) {
     if (s->F_alt) return LX_OK;
     const size_t fbytes = (size_t)s->fstride * 6 * s->esize;
@@ -553,12 +565,6 @@ lx_status ensure_fused_buffers(lx_sim *s) {
     if ((st = dalloc(s, &pp, psi_total(s) * sizeof(float)))) return st;
     launch_zero_bytes(pp, psi_total(s) * sizeof(float), s->cs);
     s->psi_alt = (float *)pp;
-    if (s->tfH.total > 0) {
-        if ((st = dalloc(s, &pp, sizeof(float) * (size_t)s->tfH.total))) return st;
-        s->tf_dh = (float *)pp;
-        if ((st = dalloc(s, &pp, sizeof(float) * (size_t)s->tfH.total))) return st;
-        s->tf_dp = (float *)pp;
-    }
     HIP_TRY(hipStreamSynchronize(s->cs));
     return LX_OK;
 }
@@ -578,11 +584,8 @@ void release_fused_buffers(lx_sim *s) {
     (void)hipStreamSynchronize(s->cs);
     dfree(s, s->F_alt);
     dfree(s, s->psi_alt);
-    dfree(s, s->tf_dh);
-    dfree(s, s->tf_dp);
     s->F_alt = nullptr;
     s->psi_alt = nullptr;
-    s->tf_dh = s->tf_dp = nullptr;
     s->fused = 0;
 }
 
@@ -709,9 +712,9 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
     if ((st = build_incident(s))) return bail(st);
     if (!s->planar) {
         // 3D: the half-step pair by default; LX_FUSED=1 selects the fused step
-        // (lx_sim_autotune times both and keeps the faster)
+        // where it applies (lx_sim_autotune times both and keeps the faster)
         const char *fz = std::getenv("LX_FUSED");
-        if (fz && *fz && *fz != '0') {
+        if (fz && *fz && *fz != '0' && fused_ok(s)) {
             if ((st = ensure_fused_buffers(s))) return bail(st);
             s->fused = 1;
         }
@@ -1001,7 +1004,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         if (it != cache.end()) {
             s->kz = it->second[0];
             s->nw = it->second[1];
-            if (it->second[2]) {
+            if (it->second[2] && fused_ok(s)) {
                 lx_status st = ensure_fused_buffers(s);
                 if (st) return st;
                 s->fused = 1;
@@ -1021,7 +1024,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         int best_kz = s->kz, best_nw = s->nw, best_fused = s->fused;
         // candidates: the half-step pair (nw = 0) and the fused step with 8 or
         // 4 wavefronts per block (its second buffer set is kept only if it wins)
-        const int can_fuse = ensure_fused_buffers(s) == LX_OK;
+        const int can_fuse = fused_ok(s) && ensure_fused_buffers(s) == LX_OK;
         for (int nw : {0, 8, 4}) {
             if (nw && !can_fuse) continue;
             s->fused = nw != 0;
@@ -1037,7 +1040,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
                             if (s->fused) {
                                 g.Fn = (__half *)s->F_alt;
                                 g.psin = s->psi_alt;
-                                launch_step3(g, nw, s->tfH, s->tf_dh, s->tf_dp, s->cs);
+                                launch_step3(g, nw, s->cs);
                             } else {
                                 launch_bulk(g, 0, 0, s->cs);
                                 launch_bulk(g, 0, 1, s->cs);
@@ -1047,7 +1050,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
                             if (s->fused) {
                                 g.Fn = (float *)s->F_alt;
                                 g.psin = s->psi_alt;
-                                launch_step3(g, nw, s->tfH, s->tf_dh, s->tf_dp, s->cs);
+                                launch_step3(g, nw, s->cs);
                             } else {
                                 launch_bulk(g, 0, 0, s->cs);
                                 launch_bulk(g, 0, 1, s->cs);

@@ -90,15 +90,6 @@ __device__ __forceinline__ double snap_value(const BSim &S, const TfBox &b, int 
     return win_s[j];
 }
 
-// DPP wavefront shifts: lane l receives the value of lane l+1 (next) or
-// lane l-1 (prev); the lane without a source receives 0.
-__device__ __forceinline__ float wave_next(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
-}
-__device__ __forceinline__ float wave_prev(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
-}
-
 __device__ __forceinline__ float rdlane(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }

@@ -31,6 +31,15 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
     return x * q + min(x, r) + l;
 }
 
+// Wavefront shifts (DPP wave_shl1 / wave_shr1): lane l receives lane l+1
+// (next) or l-1 (prev); lane 63 (next) and lane 0 (prev) receive 0.
+__device__ __forceinline__ float wave_next(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float wave_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+
 constexpr int kBX = 64;      // threads along x per block (one wavefront)
 constexpr int kBY = 4;       // rows per block
 constexpr int kMaxTf = 12;   // TF/SF correction boxes per half step
@@ -131,19 +140,21 @@ void launch_tfsf(const Grid<T> &g, const TfList &tf, const IncParams &inc, const
                  Launch l, int half, hipStream_t s);
 
 // Fused 3D timestep (H and E half steps in one launch, F/psi read, Fn/psin
-// written), nw wavefronts per block.  The H TF/SF corrections come
-// precomputed per face cell (dh, dp from launch_tfsf_hprep; may be NULL when
-// there are none); the E ones are applied by launch_tfsf on the written
-// buffer.
+// written), nw wavefronts per block.  Bulk update only: the H TF/SF
+// corrections are applied to the read buffer beforehand (launch_tfsf_hpre),
+// the E ones to the written buffer afterwards (launch_tfsf).  With CPML the
+// step is two launches: the interior box, then the boundary shell.
 template <typename T>
-void launch_step3(const Grid<T> &g, int nw, const TfList &tfH, const float *dh, const float *dp, hipStream_t s);
+void launch_step3(const Grid<T> &g, int nw, hipStream_t s);
 
-// Per H face cell of the step: dh = s*(kh + c'h)*delta (subtracted from H)
-// and dp = c'h*delta (added to psi); an extra block advances the aux line's
-// H phase and records the E probes of the previous step (g.F is read only).
+// H TF/SF corrections of the step applied to H of the read buffer before the
+// fused step: H -= s*kh*delta per face cell (the fused step then computes
+// (H - s*kh*delta) - curl).  Needs every H face outside the CPML slabs (no
+// psi term); an extra block advances the aux line's H phase and records the
+// E probes of the previous step.
 template <typename T>
-void launch_tfsf_hprep(const Grid<T> &g, const TfList &tf, const IncParams &inc, const AuxLine &aux,
-                       const ProbeArgs &pr, Launch l, float *dh, float *dp, hipStream_t s);
+void launch_tfsf_hpre(const Grid<T> &g, const TfList &tf, const IncParams &inc, const AuxLine &aux,
+                      const ProbeArgs &pr, Launch l, hipStream_t s);
 
 // End of a launch sequence of n steps: records the E probes of the last step
 // and advances counter[0] by n.

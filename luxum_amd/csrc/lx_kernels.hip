@@ -481,16 +481,16 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict
     }
 }
 
-// Fused 3D step: the H corrections of the step, per face cell, for k_step3
-// (which adds them to the cells it updates, halo cells included):
-//   dh = s*(kh_a + c'h_a)*delta,  dp = c'h_a*delta  (c' only on slab cells)
-// with the same delta and coefficients as k_tfsf<H>.  The extra block runs
-// the aux line's H phase (it writes h; the face cells read e) and records
-// the E probes of the previous step (E is read only in this launch).
+// Fused 3D step: the H corrections of the step applied to H of the buffer
+// the fused step reads, H -= s*kh*delta per face cell (same delta and
+// coefficient as k_tfsf<H>; the H face boxes of one component are disjoint,
+// so every cell has one owner thread).  Every face lies outside the CPML
+// slabs (checked on the host), so there is no psi term.  The extra block
+// runs the aux line's H phase (it writes h; the face cells read e) and
+// records the E probes of the previous step (E is not written here).
 template <typename T>
-__global__ __launch_bounds__(256) void k_tfsf_hprep(Grid<T> g, const TfBox *__restrict__ tb, int ntf, int total,
-                                                    IncParams inc, AuxLine aux, ProbeArgs pr, Launch l,
-                                                    float *__restrict__ dh, float *__restrict__ dp) {
+__global__ __launch_bounds__(256) void k_tfsf_hpre(Grid<T> g, const TfBox *__restrict__ tb, int ntf, int total,
+                                                   IncParams inc, AuxLine aux, ProbeArgs pr, Launch l) {
     const int t = g.counter[0] + l.toff;
     const int nb = (total + 255) / 256;
     if ((int)blockIdx.x >= nb) {
@@ -513,14 +513,10 @@ __global__ __launch_bounds__(256) void k_tfsf_hprep(Grid<T> g, const TfBox *__re
                                 t - 0.5, true);
     const float delta = (float)(bx.dir * gi);
     const int a = (bx.comp + 1 + bx.curl) % 3;
-    const int ia = ijk[a];
-    float coef = g.tab[(a * 6 + TK_KH) * g.tn + ia], cp = 0.f;
-    if (g.pml && (ia < g.slo[a] || ia >= g.shi[a])) {
-        cp = g.tab[(a * 6 + TK_CH) * g.tn + ia];
-        coef += cp;
-    }
-    dh[gid] = (bx.curl == 0 ? 1.f : -1.f) * coef * delta;
-    dp[gid] = cp * delta;
+    const float upd = (bx.curl == 0 ? 1.f : -1.f) * g.tab[(a * 6 + TK_KH) * g.tn + ijk[a]] * delta;
+    const long long o = (long long)ijk[2] * g.plane + (long long)ijk[1] * g.px + ijk[0];
+    T *H = g.F + (3 + bx.comp) * g.fstride;
+    st1(H + o, ld1(H + o) - upd);
 }
 
 // ---------------------------------------------------------------------------
@@ -675,12 +671,12 @@ void launch_tfsf(const Grid<T> &g, const TfList &tf, const IncParams &inc, const
 }
 
 template <typename T>
-void launch_tfsf_hprep(const Grid<T> &g, const TfList &tf, const IncParams &inc, const AuxLine &aux,
-                       const ProbeArgs &pr, Launch l, float *dh, float *dp, hipStream_t s) {
+void launch_tfsf_hpre(const Grid<T> &g, const TfList &tf, const IncParams &inc, const AuxLine &aux,
+                      const ProbeArgs &pr, Launch l, hipStream_t s) {
     const int nb = (tf.total + 255) / 256;
     const unsigned blocks = (unsigned)nb + ((l.aux || l.probes) ? 1u : 0u);
     if (blocks == 0) return;
-    hipLaunchKernelGGL(k_tfsf_hprep<T>, dim3(blocks), dim3(256), 0, s, g, tf.dev, tf.n, tf.total, inc, aux, pr, l, dh, dp);
+    hipLaunchKernelGGL(k_tfsf_hpre<T>, dim3(blocks), dim3(256), 0, s, g, tf.dev, tf.n, tf.total, inc, aux, pr, l);
 }
 
 template <typename T>
@@ -726,8 +722,8 @@ void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
     template void launch_tfsf<T>(const Grid<T> &, const TfList &, const IncParams &, const AuxLine &,        \
                                  const ProbeArgs &, Launch, int, hipStream_t);                               \
     template void launch_finish<T>(const Grid<T> &, const ProbeArgs &, int, hipStream_t);                    \
-    template void launch_tfsf_hprep<T>(const Grid<T> &, const TfList &, const IncParams &, const AuxLine &,  \
-                                       const ProbeArgs &, Launch, float *, float *, hipStream_t);           \
+    template void launch_tfsf_hpre<T>(const Grid<T> &, const TfList &, const IncParams &, const AuxLine &,   \
+                                      const ProbeArgs &, Launch, hipStream_t);                              \
     template void launch_point_source<T>(const Grid<T> &, const PointSrc &, int, hipStream_t);               \
     template void launch_gather<T>(const Grid<T> &, int, int, int, int, double, double *, hipStream_t);      \
     template void launch_monitor<T>(const Grid<T> &, int, double *, hipStream_t);                            \

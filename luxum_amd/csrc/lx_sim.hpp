@@ -47,7 +47,6 @@ struct lx_sim {
     int kz = 32;
     int fused = 0;              // 3D: one fused launch per step over ping-pong buffers (F/psi and F_alt/psi_alt)
     int nw = 8;                 // fused step: wavefronts per block
-    float *tf_dh = nullptr, *tf_dp = nullptr;   // fused step: H TF/SF corrections per face cell
     lx::IncParams inc{};
     lx::AuxLine aux{};
     int use_aux = 0;
@@ -85,6 +84,7 @@ lx_status dalloc(lx_sim *s, void **out, size_t bytes);
 size_t psi_total(const lx_sim *s);
 ProbeArgs probe_args(const lx_sim *s);
 void drop_graph(lx_sim *s);
+bool fused_ok(const lx_sim *s);
 lx_status ensure_fused_buffers(lx_sim *s);
 // enqueue on the compute stream: one component (what == LX_OUT_FIELD) or
 // |E|^2 (LX_OUT_E2) into dst_dev in the FP64 host layout of extents ext

@@ -1,6 +1,6 @@
-// lx_step3.hip -- fused 3D Yee timestep: the H half step and the E half step
-// of one timestep in a single z-march, reading one buffer set (fields, psi)
-// and writing the other.
+// lx_step3.hip -- fused 3D Yee timestep: the H and the E half step of one
+// timestep in one z-march, reading one buffer set (fields, psi) and writing
+// the other.
 //
 // The two half-step kernels read every E value twice per step (once by the
 // H update, once by the E update) and every H value twice; here each old
@@ -8,379 +8,438 @@
 // cell and step plus the slab psi, against 18*s + 1 for the half-step pair
 // (maxwell_three_dim_pml.cc:182-255 is the update order followed).
 //
-// Block: NW wavefronts, wave w owns grid row jr = j0 - 1 + w of its tile
-// (j0 = first of the tile's NW-1 output rows), every x cell of that row (in
-// chunks of 256, 4 cells per lane) and the planes [k0, k1).  Per plane k:
-//   H: every wave computes H_new(jr, k) from the old buffer: E_old at k and
-//      k+1 (a register queue along z), at jr+1 (load) and at x+1 (next lane).
-//      Wave 0's row j0-1 and the plane k0-1 are halo: computed, because the E
-//      update needs them, but not stored (their owner stores them).  TF/SF
-//      corrections of the H half step, precomputed per face cell by
-//      k_tfsf_hprep, are added to every computed cell.
-//   exchange: Hx and Hz of every wave's row go through LDS.
-//   E: waves 1..NW-1 compute E_new(jr, k) from H_new(jr, k), H_new(jr, k-1)
-//      (registers), H_new(jr-1, k) (wave w-1 through LDS) and x-1 (previous
-//      lane; lane 0 takes the last cell of the previous x chunk from LDS).
-// Every cell of a stored row is written, non-updated cells with their old
-// value, so both buffers always hold a complete state.  Neighbouring tiles
-// only read the old buffer, so there is no ordering between blocks.
+// Work unit: one wavefront per (64-cell x chunk, grid row j, block of kz
+// planes), one cell per lane, marching along z.  Units are numbered chunk
+// first, so the chunks of a row share a block (and an L2) and read each
+// other's edge cells while those are cached.  Per plane k:
+//   loads: E_old at (j, k+1) (the z queue), at (j+1, k) and (j-1, k), H_old
+//     at (j, k) and (j-1, k), the material id, the slab psi and the edge
+//     cells x0-1 and x0+64.  All loads of a plane are issued before its
+//     arithmetic and none sits in a branch that depends on loaded data, so a
+//     plane costs one memory round trip.
+//   H: H_new(j, k) of the three components; Hx, Hz of row j-1 (needed by Ex,
+//     Ez of row j) and Hy, Hz of cell x0-1 (needed by Ey, Ez of lane 0) are
+//     recomputed here and stored by their own wavefront.
+//   E: E_new(j, k) from H_new at (j, k), (j, k-1) (registers), (j-1, k) and
+//     x-1 (previous lane).
+// The plane below the block (k0-1) is recomputed for Hx and Hy only.  Cells
+// outside a component's update range are written with their old value, so
+// both buffers always hold a complete state.  Blocks only read the old
+// buffer: there is no ordering between blocks and no barrier after the
+// coefficient load.  The H TF/SF corrections are applied to H_old beforehand
+// (k_tfsf_hpre), the E ones to E_new afterwards (k_tfsf): this is the bulk
+// update only.
 #include "lx_device.hpp"
 
 namespace lx {
-
-constexpr int kMaxKz = 64;
 
 template <typename T>
 __device__ __forceinline__ float *psi_new(const Grid<T> &g, int a, int slot) {
     return g.psin + (psi_base(g, a, slot) - g.psi);
 }
 
-// H_new(row, k) of the three components for the 4 cells of a lane, with the
-// row's psi (returned in p*), from E_old at (row, k), (row, k+1), (row+1, k).
-struct HRow {
-    float4 hx, hy, hz;
-    float4 py0, pz1, pz0, px1, px0, py1;   // psi: y0 z1 (Hx), z0 x1 (Hy), x0 y1 (Hz)
+// Accesses at a wave-uniform base plus a 32-bit per-lane byte offset.  uo()
+// passes a per-plane element offset through readfirstlane, so that the
+// compiler does not fold the lane offset into one 64-bit induction variable
+// per access (the base pointer keeps its global address space).
+__device__ __forceinline__ long long uo(long long v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ float ldb(const T *base, unsigned boff) {
+    return ld1(reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff));
+}
+template <typename T>
+__device__ __forceinline__ void stb(T *base, unsigned boff, float v) {
+    st1(reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff), v);
+}
+__device__ __forceinline__ float lane0(float v) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+}
+
+// Split curl of one component, d1*k1 - d2*k2, with the CPML recursion of
+// each term on slab cells (p = b*p + c*d, added for term 1, subtracted for
+// term 2).  Every update of the kernel goes through it, so a recomputed
+// neighbour equals the value its owner stores.
+template <bool PML>
+__device__ __forceinline__ float curl2(float d1, float k1, bool s1, float &p1, float b1, float c1, float d2, float k2,
+                                       bool s2, float &p2, float b2, float c2) {
+    float acc = d1 * k1 - d2 * k2;
+    if (PML && s1) { p1 = p1 * b1 + d1 * c1; acc = acc + p1; }
+    if (PML && s2) { p2 = p2 * b2 + d2 * c2; acc = acc - p2; }
+    return acc;
+}
+
+// The units of one launch: a box of rows [jlo, jhi) x z blocks [tlo, thi)
+// (mode 0), or every unit outside that box (mode 1); all x chunks of each.
+struct UnitMap {
+    int mode, jlo, jhi, tlo, thi, ntz;
 };
 
-template <typename T, bool PML, int NW>
-__global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, const TfBox *__restrict__ tbH, int ntfH,
-                                                   const float *__restrict__ dh, const float *__restrict__ dp,
-                                                   int nunits) {
+// PM: CPML axes the launch handles (bit 0: x slab, 1: y, 2: z).  A CPML grid
+// runs two launches: the units whose rows and planes lie outside the y and
+// z slabs (PM = 1, x slab only, far fewer registers) and the rest (PM = 7).
+template <typename T, int PM, int NW>
+__global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nunits) {
+    constexpr bool PML = PM != 0;
     __shared__ float cbs[256];
-    __shared__ float xedge[NW][kMaxKz + 1][2];     // per wave and plane: Hy, Hz of the previous chunk's last cell
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int q = threadIdx.x; q < 256; q += 64 * NW) cbs[q] = g.cb[q];
     __syncthreads();
     const int unit = xcd_remap(blockIdx.x, (nunits + NW - 1) / NW) * NW + w;
     if (unit >= nunits) return;                    // whole wavefront; no barrier follows
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int jr = unit % ny, tz = unit / ny;
+    const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
+    const int nch = (nx + 63) >> 6;
+    const int ch = unit % nch;
+    int rest = unit / nch, jr, tz;
+    if (um.mode == 0) {
+        const int nr = um.jhi - um.jlo;
+        jr = um.jlo + rest % nr;
+        tz = um.tlo + rest / nr;
+    } else {
+        const int nro = um.jlo + (ny - um.jhi);    // rows outside the box: every z block
+        if (rest < nro * um.ntz) {
+            const int r = rest % nro;
+            jr = r < um.jlo ? r : um.jhi + (r - um.jlo);
+            tz = rest / nro;
+        } else {                                   // rows inside: z blocks outside the box
+            rest -= nro * um.ntz;
+            const int nr = um.jhi - um.jlo, t = rest / nr;
+            jr = um.jlo + rest % nr;
+            tz = t < um.tlo ? t : um.thi + (t - um.tlo);
+        }
+    }
     const int k0 = tz * g.kz, k1 = min(k0 + g.kz, nz);
-    const int ks = k0 > 0 ? k0 - 1 : 0;
-    const long long px = g.px, plane = g.plane, fs = g.fstride;
-    const T *Ex = g.F, *Ey = g.F + fs, *Ez = g.F + 2 * fs;
-    const T *Hx = g.F + 3 * fs, *Hy = g.F + 4 * fs, *Hz = g.F + 5 * fs;
-    T *Exn = g.Fn, *Eyn = g.Fn + fs, *Ezn = g.Fn + 2 * fs;
-    T *Hxn = g.Fn + 3 * fs, *Hyn = g.Fn + 4 * fs, *Hzn = g.Fn + 5 * fs;
+    const int x0 = ch * 64;
+    const long long plane = g.plane, fs = g.fstride;
     const int tn = g.tn;
     const float *tab = g.tab;
-    const bool has_m = jr >= 1;                    // row jr-1 exists (its Hx, Hz feed Ex, Ez)
+    const bool has_m = jr >= 1;                    // row j-1 exists (its Hx, Hz feed Ex, Ez)
     const int jm = has_m ? jr - 1 : 0;
     const int jp = min(jr + 1, ny - 1);
-    const bool rowx = jr < ny - 1;                 // Hx, Hz rows of jr; Ey rows
+    const bool rowx = jr < ny - 1;                 // Hx, Hz, Ey rows
     const bool jin = jr >= 1 && jr <= ny - 2;      // Ex, Ez rows
-    // row coefficients of jr and jr-1
     const float khy = tab[(1 * 6 + TK_KH) * tn + min(jr, ny - 2)], khym = tab[(1 * 6 + TK_KH) * tn + jm];
     const float key = tab[(1 * 6 + TK_KE) * tn + jr];
-    const bool sy = PML && (jr < g.slo[1] || jr >= g.shi[1]);
-    const bool sym = PML && has_m && (jm < g.slo[1] || jm >= g.shi[1]);
+    const bool sy = (PM & 2) && (jr < g.slo[1] || jr >= g.shi[1]);
+    const bool sym = (PM & 2) && has_m && (jm < g.slo[1] || jm >= g.shi[1]);
     const float bhy = sy ? tab[(1 * 6 + TK_BH) * tn + jr] : 0.f, chy = sy ? tab[(1 * 6 + TK_CH) * tn + jr] : 0.f;
     const float bhym = sym ? tab[(1 * 6 + TK_BH) * tn + jm] : 0.f, chym = sym ? tab[(1 * 6 + TK_CH) * tn + jm] : 0.f;
     const float bey = sy ? tab[(1 * 6 + TK_BE) * tn + jr] : 0.f, cey = sy ? tab[(1 * 6 + TK_CE) * tn + jr] : 0.f;
     const int siy = sy ? slab_index(jr, g.slo[1], g.shi[1]) : 0;
     const int siym = sym ? slab_index(jm, g.slo[1], g.shi[1]) : 0;
 
-    for (int x0 = 0; x0 < nx; x0 += 256) {
-        const int i0 = x0 + lane * 4;
-        const bool on = i0 < nx;
-        const bool onm = on && has_m;
-        const unsigned mx_all = on ? xmask(i0, 0, nx - 1) : 0u;
-        const unsigned mx_in = on ? xmask(i0, 0, nx - 2) : 0u;
-        const unsigned m_ex = on && jin ? xmask(i0, 0, nx - 2) : 0u;
-        const unsigned m_ey = on && rowx ? xmask(i0, 1, nx - 2) : 0u;
-        const unsigned m_ez = on && jin ? xmask(i0, 1, nx - 2) : 0u;
-        const int ic = min(i0, g.px - 4);
-        const float4 khx = *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_KH) * tn + ic);
-        const float4 kex = *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_KE) * tn + ic);
-        const bool sx = PML && on && (i0 < g.slo[0] || i0 >= g.shi[0]);
-        const int xs = sx ? (i0 < g.slo[0] ? i0 : g.slo[0] + (i0 - g.shi[0])) : 0;
-        const bool last_lane = (lane == 63) || (i0 + 4 >= nx);
-        const bool nxt = i0 + 4 < nx;
-        long long o = (long long)ks * plane + (long long)jr * px + i0;
-        const long long djp = (long long)(jp - jr) * px, djm = (long long)(jr - jm) * px;
-        // E_old queue along z: row jr all components, row jr-1 Ey
-        float4 ex0 = f4(0.f), ey0 = f4(0.f), ez0 = f4(0.f), eym0 = f4(0.f);
-        if (on) { ex0 = ld4(Ex + o); ey0 = ld4(Ey + o); ez0 = ld4(Ez + o); }
-        if (onm) eym0 = ld4(Ey + o - djm);
-        float4 hxm = f4(0.f), hym = f4(0.f);       // H_new(jr, k-1): Hx, Hy
-
-        for (int k = ks; k < k1; ++k, o += plane) {
-            const bool halo_plane = k < k0;
-            const bool kin = k < nz - 1;
-            float4 ex1 = f4(0.f), ey1 = f4(0.f), ez1 = f4(0.f), eym1 = f4(0.f);
-            if (on && kin) { ex1 = ld4(Ex + o + plane); ey1 = ld4(Ey + o + plane); ez1 = ld4(Ez + o + plane); }
-            if (onm && kin) eym1 = ld4(Ey + o - djm + plane);
-            float ezn = __shfl_down(ez0.x, 1), eyn = __shfl_down(ey0.x, 1);
-            if (last_lane) {
-                ezn = (on && nxt) ? ld1(Ez + o + 4) : 0.f;
-                eyn = (on && nxt) ? ld1(Ey + o + 4) : 0.f;
-            }
-            const bool sz = PML && (k < g.slo[2] || k >= g.shi[2]);
-            const long long zs = sz ? (long long)slab_index(k, g.slo[2], g.shi[2]) * ny : 0;
-            const long long zoff = (zs + jr) * px + i0, zoffm = (zs + jm) * px + i0;
-            const long long yoff = sy ? ((long long)k * g.sw[1] + siy) * px + i0 : 0;
-            const long long yoffm = sym ? ((long long)k * g.sw[1] + siym) * px + i0 : 0;
-            const long long xoff = sx ? ((long long)k * ny + jr) * g.sw[0] + xs : 0;
-            const long long xoffm = sx ? ((long long)k * ny + jm) * g.sw[0] + xs : 0;
-            const float khz = tab[(2 * 6 + TK_KH) * tn + min(k, nz - 2)];
-            const float bhz = sz ? tab[(2 * 6 + TK_BH) * tn + k] : 0.f, chz = sz ? tab[(2 * 6 + TK_CH) * tn + k] : 0.f;
-
-            // ---------------- H half step, row jr
-            float4 hx = f4(0.f), hy = f4(0.f), hz = f4(0.f);
-            if (on) { hx = ld4(Hx + o); hy = ld4(Hy + o); hz = ld4(Hz + o); }
-            float4 py0 = f4(0.f), pz1 = f4(0.f), pz0 = f4(0.f), px1 = f4(0.f), px0 = f4(0.f), py1 = f4(0.f);
-            const bool dhx = on && rowx && kin, dhy = on && kin, dhz = on && rowx;
-            if (dhx) {   // Hx: d1 = dEz/dy, d2 = dEy/dz
-                const float4 d1 = ld4(Ez + o + djp) - ez0, d2 = ey1 - ey0;
-                float4 acc = d1 * khy - d2 * khz;
-                if (sy) { py0 = ld4(psi_base(g, 1, 0) + yoff) * bhy + d1 * chy; acc = acc + py0; }
-                if (sz) { pz1 = ld4(psi_base(g, 2, 1) + zoff) * bhz + d2 * chz; acc = acc - pz1; }
-                hx = sel(mx_all, hx - acc, hx);
-            }
-            if (dhy) {   // Hy: d1 = dEx/dz, d2 = dEz/dx
-                const float4 ezi = make_float4(ez0.y, ez0.z, ez0.w, ezn);
-                const float4 d1 = ex1 - ex0, d2 = ezi - ez0;
-                float4 acc = d1 * khz - d2 * khx;
-                if (sz) { pz0 = ld4(psi_base(g, 2, 0) + zoff) * bhz + d1 * chz; acc = acc + pz0; }
-                if (sx) {
-                    px1 = ld4(psi_base(g, 0, 1) + xoff) * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_BH) * tn + i0) +
-                          d2 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_CH) * tn + i0);
-                    acc = acc - px1;
-                }
-                hy = sel(mx_in, hy - acc, hy);
-            }
-            if (dhz) {   // Hz: d1 = dEy/dx, d2 = dEx/dy
-                const float4 eyi = make_float4(ey0.y, ey0.z, ey0.w, eyn);
-                const float4 d1 = eyi - ey0, d2 = ld4(Ex + o + djp) - ex0;
-                float4 acc = d1 * khx - d2 * khy;
-                if (sx) {
-                    px0 = ld4(psi_base(g, 0, 0) + xoff) * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_BH) * tn + i0) +
-                          d1 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_CH) * tn + i0);
-                    acc = acc + px0;
-                }
-                if (sy) { py1 = ld4(psi_base(g, 1, 1) + yoff) * bhy + d2 * chy; acc = acc - py1; }
-                hz = sel(mx_in, hz - acc, hz);
-            }
-            // ---------------- Hx, Hz of row jr-1 (needed by Ex, Ez of row jr)
-            float4 hxj = f4(0.f), hzj = f4(0.f);
-            float4 qy0 = f4(0.f), qz1 = f4(0.f), qx0 = f4(0.f), qy1 = f4(0.f);
-            float eynm = __shfl_down(eym0.x, 1);
-            if (onm) {
-                hxj = ld4(Hx + o - djm);
-                hzj = ld4(Hz + o - djm);
-                const float4 ezm = ld4(Ez + o - djm), exm = ld4(Ex + o - djm);
-                if (kin) {   // Hx(jr-1): d1 = Ez(jr) - Ez(jr-1), d2 = Ey(jr-1, k+1) - Ey(jr-1, k)
-                    const float4 d1 = ez0 - ezm, d2 = eym1 - eym0;
-                    float4 acc = d1 * khym - d2 * khz;
-                    if (sym) { qy0 = ld4(psi_base(g, 1, 0) + yoffm) * bhym + d1 * chym; acc = acc + qy0; }
-                    if (sz) { qz1 = ld4(psi_base(g, 2, 1) + zoffm) * bhz + d2 * chz; acc = acc - qz1; }
-                    hxj = sel(mx_all, hxj - acc, hxj);
-                }
-                {            // Hz(jr-1): d1 = dEy(jr-1)/dx, d2 = Ex(jr) - Ex(jr-1)
-                    if (last_lane) eynm = nxt ? ld1(Ey + o - djm + 4) : 0.f;
-                    const float4 eyi = make_float4(eym0.y, eym0.z, eym0.w, eynm);
-                    const float4 d1 = eyi - eym0, d2 = ex0 - exm;
-                    float4 acc = d1 * khx - d2 * khym;
-                    if (sx) {
-                        qx0 = ld4(psi_base(g, 0, 0) + xoffm) * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_BH) * tn + i0) +
-                              d1 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_CH) * tn + i0);
-                        acc = acc + qx0;
-                    }
-                    if (sym) { qy1 = ld4(psi_base(g, 1, 1) + yoffm) * bhym + d2 * chym; acc = acc - qy1; }
-                    hzj = sel(mx_in, hzj - acc, hzj);
-                }
-            }
-            // TF/SF corrections of the H half step (maxwell.cc:512-594 face
-            // sets): H -= dh, psi += dp, with dh = s*(kh + c'h)*delta and
-            // dp = c'h*delta per face cell, computed by k_tfsf_hprep.  Rows jr
-            // (all components) and jr-1 (Hx, Hz) both get theirs.
-            for (int b = 0; b < ntfH; ++b) {
-                const TfBox &bx = tbH[b];
-                if (k < bx.lo[2] || k > bx.hi[2]) continue;   // wave-uniform
-                const bool on_j = jr >= bx.lo[1] && jr <= bx.hi[1];
-                const bool on_jm = has_m && bx.comp != 1 && jm >= bx.lo[1] && jm <= bx.hi[1];
-                if (!on_j && !on_jm) continue;
-                const int a = (bx.comp + 1 + bx.curl) % 3;
-                if (bx.fa == 0) {
-                    // x face: one cell per row, at a fixed lane and element;
-                    // its values are wave-uniform (scalar loads)
-                    const int fx = bx.lo[0] - x0;
-                    if (fx < 0 || fx >= 256) continue;
-                    const int fl = fx >> 2, fe = fx & 3;
-                    const int nu = bx.hi[1] - bx.lo[1] + 1;
-                    const int base = bx.start + nu * (k - bx.lo[2]) - bx.lo[1];
-                    const float vj = on_j ? dh[base + jr] : 0.f, vm = on_jm ? dh[base + jm] : 0.f;
-                    const float qj = (PML && on_j) ? dp[base + jr] : 0.f;
-                    if (lane == fl) {
-                        if (bx.comp == 0) {
-                            set(hx, fe, get(hx, fe) - vj);
-                            set(hxj, fe, get(hxj, fe) - vm);
-                        } else if (bx.comp == 1) {
-                            set(hy, fe, get(hy, fe) - vj);
-                        } else {
-                            set(hz, fe, get(hz, fe) - vj);
-                            set(hzj, fe, get(hzj, fe) - vm);
-                        }
-                        if (PML) {   // an x face carries the x-derivative correction (a = 0)
-                            if (bx.curl == 0) set(px0, fe, get(px0, fe) + qj);
-                            else set(px1, fe, get(px1, fe) + qj);
-                        }
-                    }
-                    continue;
-                }
-                const unsigned m = on ? xmask(i0, bx.lo[0], bx.hi[0]) : 0u;
-                if (!m) continue;
-                const int fa = bx.fa, u = fa == 2 ? 0 : fa + 1, v = fa == 0 ? 2 : fa - 1;
-                const int nu = bx.hi[u] - bx.lo[u] + 1;
-                for (int e = 0; e < 4; ++e) {
-                    if (!(m & (1u << e))) continue;
-                    for (int rr = 0; rr < 2; ++rr) {
-                        if (rr == 0 ? !on_j : !on_jm) continue;
-                        const int jj = rr == 0 ? jr : jm;
-                        const int cu = u == 0 ? i0 + e : (u == 1 ? jj : k);
-                        const int cv = v == 0 ? i0 + e : (v == 1 ? jj : k);
-                        const int idx = bx.start + (cu - bx.lo[u]) + nu * (cv - bx.lo[v]);
-                        const float vh = dh[idx];
-                        const float q = PML ? dp[idx] : 0.f;
-                        if (rr == 0) {
-                            if (bx.comp == 0) set(hx, e, get(hx, e) - vh);
-                            else if (bx.comp == 1) set(hy, e, get(hy, e) - vh);
-                            else set(hz, e, get(hz, e) - vh);
-                            if (PML) {
-                                if (a == 0) {
-                                    if (bx.curl == 0) set(px0, e, get(px0, e) + q);
-                                    else set(px1, e, get(px1, e) + q);
-                                } else if (a == 1) {
-                                    if (bx.curl == 0) set(py0, e, get(py0, e) + q);
-                                    else set(py1, e, get(py1, e) + q);
-                                } else {
-                                    if (bx.curl == 0) set(pz0, e, get(pz0, e) + q);
-                                    else set(pz1, e, get(pz1, e) + q);
-                                }
-                            }
-                        } else {
-                            // halo row: only the field matters (its psi is not stored)
-                            if (bx.comp == 0) set(hxj, e, get(hxj, e) - vh);
-                            else set(hzj, e, get(hzj, e) - vh);
-                        }
-                    }
-                }
-            }
-            (void)qy0; (void)qz1; (void)qx0; (void)qy1;
-            if (!halo_plane && on) {
-                st4(Hxn + o, hx);
-                st4(Hyn + o, hy);
-                st4(Hzn + o, hz);
-                if (sy && dhx) st4(psi_new(g, 1, 0) + yoff, py0);
-                if (sz && dhx) st4(psi_new(g, 2, 1) + zoff, pz1);
-                if (sz && dhy) st4(psi_new(g, 2, 0) + zoff, pz0);
-                if (sx && dhy) st4(psi_new(g, 0, 1) + xoff, px1);
-                if (sx && dhz) st4(psi_new(g, 0, 0) + xoff, px0);
-                if (sy && dhz) st4(psi_new(g, 1, 1) + yoff, py1);
-            }
-
-            // ---------------- E half step, row jr
-            float hzp = __shfl_up(hz.w, 1), hyp = __shfl_up(hy.w, 1);
-            if (lane == 0) {
-                hyp = x0 > 0 ? xedge[w][k - ks][0] : 0.f;
-                hzp = x0 > 0 ? xedge[w][k - ks][1] : 0.f;
-            }
-            if (!halo_plane && on) {
-                const bool kine = k >= 1 && k <= nz - 2;
-                const float kez = tab[(2 * 6 + TK_KE) * tn + k];
-                const float bez = sz ? tab[(2 * 6 + TK_BE) * tn + k] : 0.f, cez = sz ? tab[(2 * 6 + TK_CE) * tn + k] : 0.f;
-                float4 ex = ex0, ey = ey0, ez = ez0;
-                const uchar4 id = *reinterpret_cast<const uchar4 *>(g.mat + o);
-                const float4 cb = make_float4(cbs[id.x], cbs[id.y], cbs[id.z], cbs[id.w]);
-                if (kine && m_ex) {   // Ex: d1 = dHz/dy, d2 = dHy/dz
-                    const float4 d1 = hz - hzj, d2 = hy - hym;
-                    float4 acc = d1 * key - d2 * kez;
-                    if (sy) {
-                        const float4 p = ld4(psi_base(g, 1, 2) + yoff) * bey + d1 * cey;
-                        st4(psi_new(g, 1, 2) + yoff, p);
-                        acc = acc + p;
-                    }
-                    if (sz) {
-                        const float4 p = ld4(psi_base(g, 2, 3) + zoff) * bez + d2 * cez;
-                        st4(psi_new(g, 2, 3) + zoff, p);
-                        acc = acc - p;
-                    }
-                    ex = sel(m_ex, ex0 + cb * acc, ex0);
-                }
-                if (kine && m_ey) {   // Ey: d1 = dHx/dz, d2 = dHz/dx
-                    const float4 hzi = make_float4(hzp, hz.x, hz.y, hz.z);
-                    const float4 d1 = hx - hxm, d2 = hz - hzi;
-                    float4 acc = d1 * kez - d2 * kex;
-                    if (sz) {
-                        const float4 p = ld4(psi_base(g, 2, 2) + zoff) * bez + d1 * cez;
-                        st4(psi_new(g, 2, 2) + zoff, p);
-                        acc = acc + p;
-                    }
-                    if (sx) {
-                        const float4 p = ld4(psi_base(g, 0, 3) + xoff) *
-                                             *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_BE) * tn + i0) +
-                                         d2 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_CE) * tn + i0);
-                        st4(psi_new(g, 0, 3) + xoff, p);
-                        acc = acc - p;
-                    }
-                    ey = sel(m_ey, ey0 + cb * acc, ey0);
-                }
-                if (kin && m_ez) {   // Ez: d1 = dHy/dx, d2 = dHx/dy
-                    const float4 hyi = make_float4(hyp, hy.x, hy.y, hy.z);
-                    const float4 d1 = hy - hyi, d2 = hx - hxj;
-                    float4 acc = d1 * kex - d2 * key;
-                    if (sx) {
-                        const float4 p = ld4(psi_base(g, 0, 2) + xoff) *
-                                             *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_BE) * tn + i0) +
-                                         d1 * *reinterpret_cast<const float4 *>(tab + (0 * 6 + TK_CE) * tn + i0);
-                        st4(psi_new(g, 0, 2) + xoff, p);
-                        acc = acc + p;
-                    }
-                    if (sy) {
-                        const float4 p = ld4(psi_base(g, 1, 3) + yoff) * bey + d2 * cey;
-                        st4(psi_new(g, 1, 3) + yoff, p);
-                        acc = acc - p;
-                    }
-                    ez = sel(m_ez, ez0 + cb * acc, ez0);
-                }
-                st4(Exn + o, ex);
-                st4(Eyn + o, ey);
-                st4(Ezn + o, ez);
-            }
-            // the last cell of this chunk, for lane 0 of the next chunk (same wave)
-            if (lane == 63) {
-                xedge[w][k - ks][0] = hy.w;
-                xedge[w][k - ks][1] = hz.w;
-            }
-            hxm = hx;
-            hym = hy;
-            ex0 = ex1;
-            ey0 = ey1;
-            ez0 = ez1;
-            eym0 = eym1;
-        }
+    // every lane is inside the padded row (x0 + 63 < px); lanes past the grid
+    // read the zero padding and write it back unchanged
+    const int i = x0 + lane;
+    const bool nxt = x0 + 64 < nx;                 // x+1 of lane 63 lies in the next chunk
+    const bool lft = x0 > 0;                       // x-1 of lane 0 lies in the previous chunk
+    const bool m_hx = i < nx, m_in = i < nx - 1;
+    const bool m_ex = jin && i < nx - 1;
+    const bool m_ey = rowx && i >= 1 && i < nx - 1;
+    const bool m_ez = jin && i >= 1 && i < nx - 1;
+    const float khx = tab[(0 * 6 + TK_KH) * tn + i], kex = tab[(0 * 6 + TK_KE) * tn + i];
+    const bool sxw = (PM & 1) && (x0 < g.slo[0] || x0 + 64 > g.shi[0]);   // chunk meets the x slab
+    const bool sx = sxw && (i < g.slo[0] || i >= g.shi[0]);
+    const int xs = sx ? (i < g.slo[0] ? i : g.slo[0] + (i - g.shi[0])) : 0;
+    float bhx = 0.f, chx = 0.f, bex = 0.f, cex = 0.f;
+    if (sx) {
+        bhx = tab[(0 * 6 + TK_BH) * tn + i];
+        chx = tab[(0 * 6 + TK_CH) * tn + i];
+        bex = tab[(0 * 6 + TK_BE) * tn + i];
+        cex = tab[(0 * 6 + TK_CE) * tn + i];
     }
+    // cell x0-1 (wave-uniform): its x coefficients and slab
+    const int xl = lft ? x0 - 1 : 0;
+    const float khxl = tab[(0 * 6 + TK_KH) * tn + xl];
+    const bool sxl = (PM & 1) && lft && (xl < g.slo[0] || xl >= g.shi[0]);
+    const int xsl = sxl ? (xl < g.slo[0] ? xl : g.slo[0] + (xl - g.shi[0])) : 0;
+    const float bhxl = sxl ? tab[(0 * 6 + TK_BH) * tn + xl] : 0.f, chxl = sxl ? tab[(0 * 6 + TK_CH) * tn + xl] : 0.f;
+    // per-lane byte offsets inside a plane (rows j, j-1, j+1) and inside a psi
+    // row (y and z slabs: x index; x slab: slab x index)
+    constexpr unsigned S = sizeof(T);
+    const unsigned rj = (unsigned)(jr * px + i) * S, rjm = (unsigned)(jm * px + i) * S;
+    const unsigned rjp = (unsigned)(jp * px + i) * S, ri = (unsigned)i * 4u, rx = (unsigned)xs * 4u;
+    const unsigned re = (unsigned)(jr * px + i);   // material id (one byte per cell)
+    // byte offsets of the edge cells x0+64 (rows j, j-1) and x0-1 (rows j,
+    // j+1; psi rows): equal in every lane, kept in vector registers like the
+    // others
+    const unsigned bn = rj + (unsigned)(64 - lane) * S, bnm = rjm + (unsigned)(64 - lane) * S;
+    const unsigned bl = rj - (unsigned)(lane + 1) * S, blp = rjp - (unsigned)(lane + 1) * S;
+    const unsigned ril = ri - (unsigned)(lane + 1) * 4u;
+    const unsigned rxl = (unsigned)xsl * 4u + (unsigned)(lane - lane);
+
+    // z queue at k0 (row j: Ex, Ey, Ez; row j-1: Ey; cell x0-1: Ex) and H_new(j, k0-1)
+    float ex0, ey0, ez0, eym0, exl0 = 0.f;
+    float hxm = 0.f, hym = 0.f;
+    if (k0 > 0) {
+        const int kh = k0 - 1;
+        const T *Fk = g.F + kh * plane;
+        const float exh = ldb(Fk, rj), eyh = ldb(Fk + fs, rj), ezh = ldb(Fk + 2 * fs, rj);
+        ex0 = ldb(Fk + plane, rj);
+        ey0 = ldb(Fk + plane + fs, rj);
+        ez0 = ldb(Fk + plane + 2 * fs, rj);
+        eym0 = ldb(Fk + plane + fs, rjm);
+        if (lft) exl0 = ldb(Fk + plane, bl);
+        const float ezph = ldb(Fk + 2 * fs, rjp);
+        const float ezNh = nxt ? ldb(Fk + 2 * fs, bn) : 0.f;
+        const float hxh = ldb(Fk + 3 * fs, rj), hyh = ldb(Fk + 4 * fs, rj);
+        const bool sz = (PM & 4) && (kh < g.slo[2] || kh >= g.shi[2]);
+        float py0 = 0.f, pz1 = 0.f, pz0 = 0.f, px1 = 0.f;
+        if (sy) py0 = ldb(psi_base(g, 1, 0) + ((long long)kh * g.sw[1] + siy) * px, ri);
+        if (sz) {
+            const long long zb = ((long long)slab_index(kh, g.slo[2], g.shi[2]) * ny + jr) * px;
+            pz1 = ldb(psi_base(g, 2, 1) + zb, ri);
+            pz0 = ldb(psi_base(g, 2, 0) + zb, ri);
+        }
+        if (sx) px1 = ldb(psi_base(g, 0, 1) + ((long long)kh * ny + jr) * g.sw[0], rx);
+        const float khz = tab[(2 * 6 + TK_KH) * tn + kh];
+        const float bhz = sz ? tab[(2 * 6 + TK_BH) * tn + kh] : 0.f, chz = sz ? tab[(2 * 6 + TK_CH) * tn + kh] : 0.f;
+        float ezi = wave_next(ezh);
+        if (lane == 63) ezi = ezNh;
+        hxm = hxh;
+        if (rowx) {   // Hx: d1 = dEz/dy, d2 = dEy/dz
+            const float acc = curl2<PML>(ezph - ezh, khy, sy, py0, bhy, chy, ey0 - eyh, khz, sz, pz1, bhz, chz);
+            hxm = m_hx ? hxh - acc : hxh;
+        }
+        {             // Hy: d1 = dEx/dz, d2 = dEz/dx
+            const float acc = curl2<PML>(ex0 - exh, khz, sz, pz0, bhz, chz, ezi - ezh, khx, sx, px1, bhx, chx);
+            hym = m_in ? hyh - acc : hyh;
+        }
+    } else {
+        ex0 = ldb(g.F, rj);
+        ey0 = ldb(g.F + fs, rj);
+        ez0 = ldb(g.F + 2 * fs, rj);
+        eym0 = ldb(g.F + fs, rjm);
+        if (lft) exl0 = ldb(g.F, bl);
+    }
+
+    for (int k = k0; k < k1; ++k) {
+        const bool kin = k < nz - 1;
+        const bool kine = k >= 1 && kin;
+        const long long kp = uo(k * plane);
+        const T *Fk = g.F + kp;
+        T *Fnk = g.Fn + kp;
+        // ---------------- loads of the plane
+        float ex1 = 0.f, ey1 = 0.f, ez1 = 0.f, eym1 = 0.f, exl1 = 0.f;
+        if (kin) {
+            ex1 = ldb(Fk + plane, rj);
+            ey1 = ldb(Fk + plane + fs, rj);
+            ez1 = ldb(Fk + plane + 2 * fs, rj);
+            eym1 = ldb(Fk + plane + fs, rjm);
+        }
+        const float ezp = ldb(Fk + 2 * fs, rjp), exq = ldb(Fk, rjp);
+        float hx = ldb(Fk + 3 * fs, rj), hy = ldb(Fk + 4 * fs, rj), hz = ldb(Fk + 5 * fs, rj);
+        float hxj = ldb(Fk + 3 * fs, rjm), hzj = ldb(Fk + 5 * fs, rjm);
+        const float ezm = ldb(Fk + 2 * fs, rjm), exm = ldb(Fk, rjm);
+        const unsigned id = g.mat[kp + re];
+        float ezN = 0.f, eyN = 0.f, eymN = 0.f;
+        if (nxt) {
+            ezN = ldb(Fk + 2 * fs, bn);
+            eyN = ldb(Fk + fs, bn);
+            eymN = ldb(Fk + fs, bnm);
+        }
+        float hyl = 0.f, hzl = 0.f, ezl = 0.f, eyl = 0.f, exlp = 0.f;
+        if (lft) {
+            if (kin) exl1 = ldb(Fk + plane, bl);
+            hyl = ldb(Fk + 4 * fs, bl);
+            hzl = ldb(Fk + 5 * fs, bl);
+            ezl = ldb(Fk + 2 * fs, bl);
+            eyl = ldb(Fk + fs, bl);
+            exlp = ldb(Fk, blp);
+        }
+        const bool sz = (PM & 4) && (k < g.slo[2] || k >= g.shi[2]);
+        // psi: H of row j (p*), H of row j-1 (q*), E of row j (e*), H of cell
+        // x0-1 (l*); row bases of the read (P*) and written (Q*) arrays, slot s
+        // of axis a lying s*scells[a] further on
+        float py0 = 0.f, py1 = 0.f, pz0 = 0.f, pz1 = 0.f, px0 = 0.f, px1 = 0.f;
+        float qy0 = 0.f, qy1 = 0.f, qz1 = 0.f, qx0 = 0.f;
+        float ey2 = 0.f, ey3 = 0.f, ez2 = 0.f, ez3 = 0.f, ex2 = 0.f, ex3 = 0.f;
+        float ly1 = 0.f, lz0 = 0.f, lx0 = 0.f, lx1 = 0.f;
+        const float *PY = nullptr, *PZ = nullptr, *PX = nullptr;
+        float *QY = nullptr, *QZ = nullptr, *QX = nullptr;
+        const long long cy = g.scells[1], cz = g.scells[2], cx = g.scells[0];
+        if (sy) {
+            const long long yb = uo(((long long)k * g.sw[1] + siy) * px);
+            PY = psi_base(g, 1, 0) + yb;
+            QY = psi_new(g, 1, 0) + yb;
+            py0 = ldb(PY, ri);
+            py1 = ldb(PY + cy, ri);
+            ey2 = ldb(PY + 2 * cy, ri);
+            ey3 = ldb(PY + 3 * cy, ri);
+            if (lft) ly1 = ldb(PY + cy, ril);
+        }
+        if (sym) {
+            const float *PYm = psi_base(g, 1, 0) + uo(((long long)k * g.sw[1] + siym) * px);
+            qy0 = ldb(PYm, ri);
+            qy1 = ldb(PYm + cy, ri);
+        }
+        if (sz) {
+            const long long zs = (long long)slab_index(k, g.slo[2], g.shi[2]) * ny;
+            const long long zb = uo((zs + jr) * px);
+            PZ = psi_base(g, 2, 0) + zb;
+            QZ = psi_new(g, 2, 0) + zb;
+            pz0 = ldb(PZ, ri);
+            pz1 = ldb(PZ + cz, ri);
+            ez2 = ldb(PZ + 2 * cz, ri);
+            ez3 = ldb(PZ + 3 * cz, ri);
+            qz1 = ldb(psi_base(g, 2, 1) + uo((zs + jm) * px), ri);
+            if (lft) lz0 = ldb(PZ, ril);
+        }
+        if (sxw || sxl) {
+            const long long xb = uo(((long long)k * ny + jr) * g.sw[0]);
+            PX = psi_base(g, 0, 0) + xb;
+            QX = psi_new(g, 0, 0) + xb;
+            if (sx) {
+                px0 = ldb(PX, rx);
+                px1 = ldb(PX + cx, rx);
+                ex2 = ldb(PX + 2 * cx, rx);
+                ex3 = ldb(PX + 3 * cx, rx);
+                qx0 = ldb(psi_base(g, 0, 0) + uo(((long long)k * ny + jm) * g.sw[0]), rx);
+            }
+            if (sxl) {
+                lx0 = ldb(PX, rxl);
+                lx1 = ldb(PX + cx, rxl);
+            }
+        }
+        const float khz = tab[(2 * 6 + TK_KH) * tn + min(k, nz - 2)];
+        const float bhz = sz ? tab[(2 * 6 + TK_BH) * tn + k] : 0.f, chz = sz ? tab[(2 * 6 + TK_CH) * tn + k] : 0.f;
+        const float kez = tab[(2 * 6 + TK_KE) * tn + k];
+        const float bez = sz ? tab[(2 * 6 + TK_BE) * tn + k] : 0.f, cez = sz ? tab[(2 * 6 + TK_CE) * tn + k] : 0.f;
+        float ezi = wave_next(ez0), eyi = wave_next(ey0), eymi = wave_next(eym0);
+        if (lane == 63) {
+            ezi = ezN;
+            eyi = eyN;
+            eymi = eymN;
+        }
+
+        // ---------------- H half step, row j
+        if (rowx && kin) {   // Hx: d1 = dEz/dy, d2 = dEy/dz
+            const float acc = curl2<PML>(ezp - ez0, khy, sy, py0, bhy, chy, ey1 - ey0, khz, sz, pz1, bhz, chz);
+            hx = m_hx ? hx - acc : hx;
+        }
+        if (kin) {           // Hy: d1 = dEx/dz, d2 = dEz/dx
+            const float acc = curl2<PML>(ex1 - ex0, khz, sz, pz0, bhz, chz, ezi - ez0, khx, sx, px1, bhx, chx);
+            hy = m_in ? hy - acc : hy;
+        }
+        if (rowx) {          // Hz: d1 = dEy/dx, d2 = dEx/dy
+            const float acc = curl2<PML>(eyi - ey0, khx, sx, px0, bhx, chx, exq - ex0, khy, sy, py1, bhy, chy);
+            hz = m_in ? hz - acc : hz;
+        }
+        // ---------------- Hx, Hz of row j-1 (only Ex, Ez of row j >= 1 use them)
+        if (has_m) {
+            if (kin) {       // Hx(j-1): d1 = Ez(j) - Ez(j-1), d2 = dEy(j-1)/dz
+                const float acc = curl2<PML>(ez0 - ezm, khym, sym, qy0, bhym, chym, eym1 - eym0, khz, sz, qz1, bhz, chz);
+                hxj = m_hx ? hxj - acc : hxj;
+            }
+            {                // Hz(j-1): d1 = dEy(j-1)/dx, d2 = Ex(j) - Ex(j-1)
+                const float acc = curl2<PML>(eymi - eym0, khx, sx, qx0, bhx, chx, ex0 - exm, khym, sym, qy1, bhym, chym);
+                hzj = m_in ? hzj - acc : hzj;
+            }
+        }
+        // ---------------- Hy, Hz of cell x0-1 (only lane 0's Ey, Ez use them)
+        if (lft) {
+            if (kin) {       // Hy(x0-1): d1 = dEx/dz, d2 = Ez(x0) - Ez(x0-1)
+                const float acc = curl2<PML>(exl1 - exl0, khz, sz, lz0, bhz, chz, lane0(ez0) - ezl, khxl, sxl, lx1, bhxl, chxl);
+                hyl = hyl - acc;
+            }
+            if (rowx) {      // Hz(x0-1): d1 = Ey(x0) - Ey(x0-1), d2 = dEx/dy
+                const float acc = curl2<PML>(lane0(ey0) - eyl, khxl, sxl, lx0, bhxl, chxl, exlp - exl0, khy, sy, ly1, bhy, chy);
+                hzl = hzl - acc;
+            }
+        }
+        stb(Fnk + 3 * fs, rj, hx);
+        stb(Fnk + 4 * fs, rj, hy);
+        stb(Fnk + 5 * fs, rj, hz);
+        if (PML) {
+            if (sy && rowx && kin) stb(QY, ri, py0);
+            if (sz && rowx && kin) stb(QZ + cz, ri, pz1);
+            if (sz && kin) stb(QZ, ri, pz0);
+            if (sx && kin) stb(QX + cx, rx, px1);
+            if (sx && rowx) stb(QX, rx, px0);
+            if (sy && rowx) stb(QY + cy, ri, py1);
+        }
+
+        // ---------------- E half step, row j
+        float hyp = wave_prev(hy), hzp = wave_prev(hz);
+        if (lane == 0) {
+            hyp = hyl;
+            hzp = hzl;
+        }
+        const float cb = cbs[id];
+        float ex = ex0, ey = ey0, ez = ez0;
+        if (kine) {
+            {                // Ex: d1 = dHz/dy, d2 = dHy/dz
+                const float acc = curl2<PML>(hz - hzj, key, sy, ey2, bey, cey, hy - hym, kez, sz, ez3, bez, cez);
+                ex = m_ex ? ex0 + cb * acc : ex0;
+            }
+            {                // Ey: d1 = dHx/dz, d2 = dHz/dx
+                const float acc = curl2<PML>(hx - hxm, kez, sz, ez2, bez, cez, hz - hzp, kex, sx, ex3, bex, cex);
+                ey = m_ey ? ey0 + cb * acc : ey0;
+            }
+        }
+        if (kin) {           // Ez: d1 = dHy/dx, d2 = dHx/dy
+            const float acc = curl2<PML>(hy - hyp, kex, sx, ex2, bex, cex, hx - hxj, key, sy, ey3, bey, cey);
+            ez = m_ez ? ez0 + cb * acc : ez0;
+        }
+        stb(Fnk, rj, ex);
+        stb(Fnk + fs, rj, ey);
+        stb(Fnk + 2 * fs, rj, ez);
+        if (PML) {
+            if (sy && kine && jin) stb(QY + 2 * cy, ri, ey2);
+            if (sz && kine && jin) stb(QZ + 3 * cz, ri, ez3);
+            if (sz && kine && rowx) stb(QZ + 2 * cz, ri, ez2);
+            if (sx && kine && rowx) stb(QX + 3 * cx, rx, ex3);
+            if (sx && kin && jin) stb(QX + 2 * cx, rx, ex2);
+            if (sy && kin && jin) stb(QY + 3 * cy, ri, ey3);
+        }
+        hxm = hx;
+        hym = hy;
+        ex0 = ex1;
+        ey0 = ey1;
+        ez0 = ez1;
+        eym0 = eym1;
+        exl0 = exl1;
+    }
+}
+
+template <typename T, int PM>
+static void launch_units(const Grid<T> &g, int nw, const UnitMap &um, int nunits, hipStream_t s) {
+    if (nunits <= 0) return;
+    if (nw == 8)
+        hipLaunchKernelGGL((k_step3<T, PM, 8>), dim3((nunits + 7) / 8), dim3(512), 0, s, g, um, nunits);
+    else
+        hipLaunchKernelGGL((k_step3<T, PM, 4>), dim3((nunits + 3) / 4), dim3(256), 0, s, g, um, nunits);
 }
 
 template <typename T>
-void launch_step3(const Grid<T> &g, int nw, const TfList &tfH, const float *dh, const float *dp, hipStream_t s) {
+void launch_step3(const Grid<T> &g, int nw, hipStream_t s) {
     const int ntz = (g.nz + g.kz - 1) / g.kz;
-    const int nunits = g.ny * ntz;    // one wavefront per (row, z block)
-    const int ntf = dh ? tfH.n : 0;
-    if (nw == 8) {
-        const unsigned blocks = (unsigned)((nunits + 7) / 8);
-        if (g.pml) hipLaunchKernelGGL((k_step3<T, true, 8>), dim3(blocks), dim3(512), 0, s, g, tfH.dev, ntf, dh, dp, nunits);
-        else hipLaunchKernelGGL((k_step3<T, false, 8>), dim3(blocks), dim3(512), 0, s, g, tfH.dev, ntf, dh, dp, nunits);
-    } else {
-        const unsigned blocks = (unsigned)((nunits + 3) / 4);
-        if (g.pml) hipLaunchKernelGGL((k_step3<T, true, 4>), dim3(blocks), dim3(256), 0, s, g, tfH.dev, ntf, dh, dp, nunits);
-        else hipLaunchKernelGGL((k_step3<T, false, 4>), dim3(blocks), dim3(256), 0, s, g, tfH.dev, ntf, dh, dp, nunits);
+    const int nch = (g.nx + 63) / 64;              // one wavefront per (x chunk, row, z block)
+    if (!g.pml) {
+        launch_units<T, 0>(g, nw, UnitMap{0, 0, g.ny, 0, ntz, ntz}, nch * g.ny * ntz, s);
+        return;
     }
+    // box: rows j with j and j-1 outside the y slabs, z blocks whose planes
+    // (k0-1 included) lie outside the z slabs
+    UnitMap um{0, 0, 0, 0, 0, ntz};
+    um.jlo = g.slo[1] > 0 ? g.slo[1] + 1 : 0;
+    um.jhi = std::max(um.jlo, g.shi[1]);
+    um.tlo = g.slo[2] > 0 ? (g.slo[2] + 1 + g.kz - 1) / g.kz : 0;
+    um.thi = g.shi[2] >= g.nz ? ntz : g.shi[2] / g.kz;
+    um.tlo = std::min(um.tlo, ntz);
+    um.thi = std::max(um.thi, um.tlo);
+    const int nbox = nch * (um.jhi - um.jlo) * (um.thi - um.tlo);
+    UnitMap sh = um;
+    sh.mode = 1;
+    launch_units<T, 1>(g, nw, um, nbox, s);
+    launch_units<T, 7>(g, nw, sh, nch * g.ny * ntz - nbox, s);
 }
 
-template void launch_step3<float>(const Grid<float> &, int, const TfList &, const float *, const float *, hipStream_t);
-template void launch_step3<__half>(const Grid<__half> &, int, const TfList &, const float *, const float *,
-                                   hipStream_t);
+template void launch_step3<float>(const Grid<float> &, int, hipStream_t);
+template void launch_step3<__half>(const Grid<__half> &, int, hipStream_t);
 
 }  // namespace lx
