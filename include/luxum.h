@@ -123,8 +123,10 @@ typedef struct lx_sim_info {
     int64_t bytes_per_step;     /* algorithmic HBM bytes per full timestep */
     int32_t step;               /* timesteps taken so far */
     int32_t slab_lo[3], slab_hi[3];  /* CPML slab: [0,lo) and [hi,n) per axis */
-    int32_t fused_step;         /* 3D: one fused launch per timestep (lx_sim_time_kernels then reports it as ms_h) */
+    int32_t fused_step;         /* 3D: fused H+E step over two buffer sets (see lx_sim_time_kernels) */
     int32_t kz, waves;          /* z planes and wavefronts per block of the 3D kernels (autotuned) */
+    int64_t core_bytes;         /* fused step on a CPML grid: algorithmic bytes of its interior-box launch
+                                   (rows and planes outside the y and z slabs); else bytes_per_step */
 } lx_sim_info;
 
 /* ---- parameters ------------------------------------------------------- */
@@ -201,7 +203,9 @@ void *lx_sim_stream(lx_sim *sim);
 
 /* Average device time per launch, in ms, of the H-update and E-update
  * kernels over `nsteps` extra timesteps, timed with events on the compute
- * stream around every launch (the steps are real steps: state advances). */
+ * stream around every launch (the steps are real steps: state advances).
+ * With the fused 3D step, ms_h is its interior-box launch and ms_e its CPML
+ * shell launch (0 without CPML). */
 lx_status lx_sim_time_kernels(lx_sim *sim, int32_t nsteps, double *ms_h, double *ms_e,
                               double *ms_step);
 

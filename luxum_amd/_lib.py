@@ -56,6 +56,7 @@ class SimInfo(C.Structure):
         ("fused_step", C.c_int32),
         ("kz", C.c_int32),
         ("waves", C.c_int32),
+        ("core_bytes", C.c_int64),
     ]
 
 

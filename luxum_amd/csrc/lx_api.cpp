@@ -434,15 +434,17 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
             gq.psin = pbuf[(q + 1) & 1];
             if (s->p.use_ps) launch_point_source(gq, s->ps, q, s->cs);
             if (corr) launch_tfsf_hpre(gq, s->tfH, s->inc, s->aux, pr, l, s->cs);
+            // timed parts: [0,1] interior box, [2,3] CPML shell
             if (ev) (void)hipEventRecord((*ev)[4 * q + 0], s->cs);
-            launch_step3(gq, s->nw, s->cs);
+            launch_step3(gq, s->nw, 0, s->cs);
             if (ev) (void)hipEventRecord((*ev)[4 * q + 1], s->cs);
+            if (ev) (void)hipEventRecord((*ev)[4 * q + 2], s->cs);
+            launch_step3(gq, s->nw, 1, s->cs);
+            if (ev) (void)hipEventRecord((*ev)[4 * q + 3], s->cs);
             Grid<T> gn = g;
             gn.F = gq.Fn;
             gn.psi = gq.psin;
-            if (ev) (void)hipEventRecord((*ev)[4 * q + 2], s->cs);
             if (corr) launch_tfsf(gn, s->tfE, s->inc, s->aux, pr, l, 1, s->cs);
-            if (ev) (void)hipEventRecord((*ev)[4 * q + 3], s->cs);
         }
         Grid<T> gf = g;
         gf.F = buf[n & 1];
@@ -968,6 +970,13 @@ lx_status lx_sim_get_info(lx_sim *s, lx_sim_info *info) {
     info->fused_step = s->fused;
     info->kz = s->kz;
     info->waves = s->fused ? s->nw : 4;
+    info->core_bytes = info->bytes_per_step;
+    if (s->fused && s->pml) {
+        int rows = 0, planes = 0;
+        step3_core_extent(s->N[1], s->N[2], s->kz, s->slo, s->shi, &rows, &planes);
+        const int64_t box = (int64_t)rows * planes;
+        info->core_bytes = (int64_t)s->N[0] * box * per + (int64_t)s->sw[0] * box * 32;
+    }
     return LX_OK;
 }
 
@@ -1040,7 +1049,8 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
                             if (s->fused) {
                                 g.Fn = (__half *)s->F_alt;
                                 g.psin = s->psi_alt;
-                                launch_step3(g, nw, s->cs);
+                                launch_step3(g, nw, 0, s->cs);
+                                launch_step3(g, nw, 1, s->cs);
                             } else {
                                 launch_bulk(g, 0, 0, s->cs);
                                 launch_bulk(g, 0, 1, s->cs);
@@ -1050,7 +1060,8 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
                             if (s->fused) {
                                 g.Fn = (float *)s->F_alt;
                                 g.psin = s->psi_alt;
-                                launch_step3(g, nw, s->cs);
+                                launch_step3(g, nw, 0, s->cs);
+                                launch_step3(g, nw, 1, s->cs);
                             } else {
                                 launch_bulk(g, 0, 0, s->cs);
                                 launch_bulk(g, 0, 1, s->cs);

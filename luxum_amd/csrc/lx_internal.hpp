@@ -139,13 +139,16 @@ template <typename T>
 void launch_tfsf(const Grid<T> &g, const TfList &tf, const IncParams &inc, const AuxLine &aux, const ProbeArgs &pr,
                  Launch l, int half, hipStream_t s);
 
-// Fused 3D timestep (H and E half steps in one launch, F/psi read, Fn/psin
-// written), nw wavefronts per block.  Bulk update only: the H TF/SF
-// corrections are applied to the read buffer beforehand (launch_tfsf_hpre),
-// the E ones to the written buffer afterwards (launch_tfsf).  With CPML the
-// step is two launches: the interior box, then the boundary shell.
+// Fused 3D timestep (H and E half steps, F/psi read, Fn/psin written), nw
+// wavefronts per block.  Bulk update only: the H TF/SF corrections are
+// applied to the read buffer beforehand (launch_tfsf_hpre), the E ones to the
+// written buffer afterwards (launch_tfsf).  Two parts: part 0 is the whole
+// grid without CPML, else the interior box (rows and planes outside the y and
+// z slabs); part 1 is the CPML shell around it (nothing without CPML).
 template <typename T>
-void launch_step3(const Grid<T> &g, int nw, hipStream_t s);
+void launch_step3(const Grid<T> &g, int nw, int part, hipStream_t s);
+// Rows and planes of the part-0 box of a CPML grid.
+void step3_core_extent(int ny, int nz, int kz, const int slo[3], const int shi[3], int *rows, int *planes);
 
 // H TF/SF corrections of the step applied to H of the read buffer before the
 // fused step: H -= s*kh*delta per face cell (the fused step then computes

@@ -414,6 +414,17 @@ __global__ __launch_bounds__(256) void k_e2(Grid<T> g) {
 // component share the cells on the TF box edges (the reference applies both
 // corrections there), so the first-curl thread applies both and the
 // second-curl thread skips those cells.
+// Copy the box table to LDS (the per-thread box search and the edge checks
+// then read LDS instead of chaining dependent global loads).  Called by every
+// thread of a face-cell block before any of them returns.
+__device__ __forceinline__ void stage_boxes(TfBox *sb, const TfBox *__restrict__ tb, int ntf) {
+    constexpr int W = (int)(sizeof(TfBox) / sizeof(int));
+    const int *src = reinterpret_cast<const int *>(tb);
+    int *dst = reinterpret_cast<int *>(sb);
+    for (int q = threadIdx.x; q < ntf * W; q += blockDim.x) dst[q] = src[q];
+    __syncthreads();
+}
+
 __device__ __forceinline__ bool box_has(const TfBox &b, const int ijk[3]) {
     return ijk[0] >= b.lo[0] && ijk[0] <= b.hi[0] && ijk[1] >= b.lo[1] && ijk[1] <= b.hi[1] && ijk[2] >= b.lo[2] &&
            ijk[2] <= b.hi[2];
@@ -439,7 +450,7 @@ __device__ __forceinline__ float tf_term(const Grid<T> &g, const TfBox &bx, cons
 }
 
 template <typename T, bool IS_H>
-__global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict__ tb, int ntf, int total,
+__global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict__ tb_g, int ntf, int total,
                                               IncParams inc, AuxLine aux, ProbeArgs pr, Launch l) {
     const int t = g.counter[0] + l.toff;
     const int nb = (total + 255) / 256;
@@ -447,6 +458,9 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict
         extra_block<T, IS_H>(g, aux, pr, l, t);
         return;
     }
+    __shared__ TfBox sb[kMaxTf];
+    stage_boxes(sb, tb_g, ntf);
+    const TfBox *tb = sb;
     const int gid = blockIdx.x * 256 + threadIdx.x;
     if (gid >= total) return;
     int b = 0;
@@ -489,7 +503,7 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict
 // runs the aux line's H phase (it writes h; the face cells read e) and
 // records the E probes of the previous step (E is not written here).
 template <typename T>
-__global__ __launch_bounds__(256) void k_tfsf_hpre(Grid<T> g, const TfBox *__restrict__ tb, int ntf, int total,
+__global__ __launch_bounds__(256) void k_tfsf_hpre(Grid<T> g, const TfBox *__restrict__ tb_g, int ntf, int total,
                                                    IncParams inc, AuxLine aux, ProbeArgs pr, Launch l) {
     const int t = g.counter[0] + l.toff;
     const int nb = (total + 255) / 256;
@@ -497,6 +511,9 @@ __global__ __launch_bounds__(256) void k_tfsf_hpre(Grid<T> g, const TfBox *__res
         extra_block<T, true>(g, aux, pr, l, t);
         return;
     }
+    __shared__ TfBox sb[kMaxTf];
+    stage_boxes(sb, tb_g, ntf);
+    const TfBox *tb = sb;
     const int gid = blockIdx.x * 256 + threadIdx.x;
     if (gid >= total) return;
     int b = 0;

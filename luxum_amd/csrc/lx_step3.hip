@@ -415,31 +415,44 @@ static void launch_units(const Grid<T> &g, int nw, const UnitMap &um, int nunits
         hipLaunchKernelGGL((k_step3<T, PM, 4>), dim3((nunits + 3) / 4), dim3(256), 0, s, g, um, nunits);
 }
 
+// Rows [jlo, jhi) and z blocks [tlo, thi) of the interior box: rows j with j
+// and j-1 outside the y slabs, z blocks whose planes (k0-1 included) lie
+// outside the z slabs.
+static UnitMap core_box(int ny, int nz, int kz, const int slo[3], const int shi[3]) {
+    const int ntz = (nz + kz - 1) / kz;
+    UnitMap um{0, 0, 0, 0, 0, ntz};
+    um.jlo = slo[1] > 0 ? slo[1] + 1 : 0;
+    um.jhi = std::max(um.jlo, shi[1]);
+    um.tlo = std::min(slo[2] > 0 ? (slo[2] + 1 + kz - 1) / kz : 0, ntz);
+    um.thi = std::max(shi[2] >= nz ? ntz : shi[2] / kz, um.tlo);
+    return um;
+}
+
+void step3_core_extent(int ny, int nz, int kz, const int slo[3], const int shi[3], int *rows, int *planes) {
+    const UnitMap um = core_box(ny, nz, kz, slo, shi);
+    *rows = um.jhi - um.jlo;
+    *planes = std::min(um.thi * kz, nz) - um.tlo * kz;
+}
+
 template <typename T>
-void launch_step3(const Grid<T> &g, int nw, hipStream_t s) {
+void launch_step3(const Grid<T> &g, int nw, int part, hipStream_t s) {
     const int ntz = (g.nz + g.kz - 1) / g.kz;
     const int nch = (g.nx + 63) / 64;              // one wavefront per (x chunk, row, z block)
     if (!g.pml) {
-        launch_units<T, 0>(g, nw, UnitMap{0, 0, g.ny, 0, ntz, ntz}, nch * g.ny * ntz, s);
+        if (part == 0) launch_units<T, 0>(g, nw, UnitMap{0, 0, g.ny, 0, ntz, ntz}, nch * g.ny * ntz, s);
         return;
     }
-    // box: rows j with j and j-1 outside the y slabs, z blocks whose planes
-    // (k0-1 included) lie outside the z slabs
-    UnitMap um{0, 0, 0, 0, 0, ntz};
-    um.jlo = g.slo[1] > 0 ? g.slo[1] + 1 : 0;
-    um.jhi = std::max(um.jlo, g.shi[1]);
-    um.tlo = g.slo[2] > 0 ? (g.slo[2] + 1 + g.kz - 1) / g.kz : 0;
-    um.thi = g.shi[2] >= g.nz ? ntz : g.shi[2] / g.kz;
-    um.tlo = std::min(um.tlo, ntz);
-    um.thi = std::max(um.thi, um.tlo);
+    UnitMap um = core_box(g.ny, g.nz, g.kz, g.slo, g.shi);
     const int nbox = nch * (um.jhi - um.jlo) * (um.thi - um.tlo);
-    UnitMap sh = um;
-    sh.mode = 1;
-    launch_units<T, 1>(g, nw, um, nbox, s);
-    launch_units<T, 7>(g, nw, sh, nch * g.ny * ntz - nbox, s);
+    if (part == 0) {
+        launch_units<T, 1>(g, nw, um, nbox, s);
+    } else {
+        um.mode = 1;
+        launch_units<T, 7>(g, nw, um, nch * g.ny * ntz - nbox, s);
+    }
 }
 
-template void launch_step3<float>(const Grid<float> &, int, hipStream_t);
-template void launch_step3<__half>(const Grid<__half> &, int, hipStream_t);
+template void launch_step3<float>(const Grid<float> &, int, int, hipStream_t);
+template void launch_step3<__half>(const Grid<__half> &, int, int, hipStream_t);
 
 }  // namespace lx
