@@ -22,7 +22,10 @@ Two more objects ride on the same line:
   "serving": a seeded Poisson request stream (80% 2D 256^2-2048^2, 20% 3D
      64^3-320^3 dielectric scatterers, 1000-5000 steps, FP32/FP16 storage,
      200 GB admission cap) through the continuous-batching server and through
-     the plain per-request server: solves/s, p50/p99 latency, Gcell-updates/s.
+     the plain per-request server: solves/s, p50/p99 latency, Gcell-updates/s;
+     and the same for a stream of 2D requests alone arriving faster than
+     either server clears them ("stream_2d": the batched 2D path's
+     throughput against one request at a time).
 """
 import argparse
 import json
@@ -123,11 +126,37 @@ def batch_section(args):
     return out
 
 
-def serve_section(args):
-    """Served request stream: continuous batching against the plain server."""
+def _serve_stream(ctx, stream):
+    """Run one request stream through the continuous-batching server and the
+    plain per-request server."""
     import numpy as np
-    from luxum_amd import configs
     from luxum_amd.server import Server
+    out = {}
+    for mode in ("batched", "plain"):
+        srv = Server(ctx, hbm_cap=200 * 10 ** 9, plain=(mode == "plain"))
+        ids = [srv.submit(w.p, w.ids, w.eps_r, steps, arr, w.probes) for w, steps, arr in stream]
+        srv.run()
+        st = srv.stats()
+        lat = np.array([srv.result(i).latency for i in ids])
+        res = {"solves_per_s": round(st.completed / st.makespan, 3),
+               "p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
+               "makespan_s": round(st.makespan, 3), "gcell_per_s": round(st.cell_updates / st.makespan / 1e9, 2),
+               "completed": st.completed, "failed": st.failed, "peak_active": st.peak_active,
+               "peak_reserved_GB": round(st.peak_bytes / 1e9, 2), "ticks": st.ticks}
+        eff = st.alg_bytes / st.makespan / 1e9
+        res["effective_GBps"] = round(eff, 1)
+        res["effective_roofline_frac"] = round(eff / HBM_PEAK_GBS, 4)
+        out[mode] = res
+        print(f"bench: serving {mode}: {res}", file=sys.stderr, flush=True)
+        srv.close()
+    out["speedup_solves_per_s"] = round(out["batched"]["solves_per_s"] / out["plain"]["solves_per_s"], 2)
+    return out
+
+
+def serve_section(args):
+    """Served request streams: continuous batching against the plain server,
+    for the mixed 2D/3D stream and for its 2D requests alone."""
+    from luxum_amd import configs
     from luxum_amd.sim import Context
     ctx = Context(0)
     stream = configs.serve_stream(args.serve_requests, args.serve_rate, seed=11)
@@ -140,25 +169,17 @@ def serve_section(args):
                        f"200 GB admission cap",
            "requests": len(stream), "requests_3d": n3, "rate_per_s": args.serve_rate,
            "cell_updates": cells_steps}
-    for mode in ("batched", "plain"):
-        srv = Server(ctx, hbm_cap=200 * 10 ** 9, plain=(mode == "plain"))
-        ids = [srv.submit(w.p, w.ids, w.eps_r, steps, arr, w.probes) for w, steps, arr in stream]
-        srv.run()
-        st = srv.stats()
-        lat = np.array([srv.result(i).latency for i in ids])
-        res = {"solves_per_s": round(st.completed / st.makespan, 3),
-               "p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
-               "makespan_s": round(st.makespan, 3), "gcell_per_s": round(st.cell_updates / st.makespan / 1e9, 2),
-               "completed": st.completed, "failed": st.failed, "peak_active": st.peak_active,
-               "peak_reserved_GB": round(st.peak_bytes / 1e9, 2), "ticks": st.ticks}
-        if mode == "batched":
-            eff = st.alg_bytes / st.makespan / 1e9
-            res["effective_GBps"] = round(eff, 1)
-            res["effective_roofline_frac"] = round(eff / HBM_PEAK_GBS, 4)
-        out[mode] = res
-        print(f"bench: serving {mode}: {res}", file=sys.stderr, flush=True)
-        srv.close()
-    out["speedup_solves_per_s"] = round(out["batched"]["solves_per_s"] / out["plain"]["solves_per_s"], 2)
+    out.update(_serve_stream(ctx, stream))
+    # the 2D requests alone (same generator, no 3D), arriving faster than
+    # either server clears them: the throughput of the batched 2D path
+    s2 = configs.serve_stream(args.serve2d_requests, args.serve2d_rate, seed=13, frac3d=0.0)
+    print(f"bench: serving 2D stream, {len(s2)} requests, rate {args.serve2d_rate}/s", file=sys.stderr, flush=True)
+    d2 = {"workload": f"{len(s2)} seeded 2D requests, Poisson {args.serve2d_rate}/s, 256^2-2048^2 dielectric "
+                      f"scatterers, 1000-5000 steps, FP32/FP16 storage, 200 GB admission cap",
+          "requests": len(s2), "rate_per_s": args.serve2d_rate,
+          "cell_updates": sum(w.cells * steps for w, steps, _ in s2)}
+    d2.update(_serve_stream(ctx, s2))
+    out["stream_2d"] = d2
     return out
 
 
@@ -182,6 +203,8 @@ def main():
     ap.add_argument("--no-single", action="store_true", help="skip the one-at-a-time baseline of the batch section")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve2d-requests", type=int, default=64)
+    ap.add_argument("--serve2d-rate", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (PMC traffic calibration)")

@@ -87,16 +87,55 @@ Grid<T> make_grid(const lx_sim *s) {
 
 lx_status dalloc(lx_sim *s, void **out, size_t bytes) {
     void *p = nullptr;
-    size_t b = bytes ? (bytes + 255) / 256 * 256 : 256;
-    hipError_t e = hipMalloc(&p, b);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(LX_ENOMEM, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+    const size_t b = bytes ? (bytes + 255) / 256 * 256 : 256;
+    {
+        lx_ctx *c = s->ctx;
+        std::lock_guard<std::mutex> lk(c->mu);
+        auto it = c->blocks.find(b);
+        if (it != c->blocks.end()) {
+            p = it->second;
+            c->blocks.erase(it);
+            c->cached -= (int64_t)b;
+        }
     }
-    s->allocs.push_back(p);
+    if (!p) {
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            // blocks cached for other sizes may hold the memory: free them and retry once
+            (void)hipGetLastError();
+            std::multimap<size_t, void *> drop;
+            {
+                lx_ctx *c = s->ctx;
+                std::lock_guard<std::mutex> lk(c->mu);
+                drop.swap(c->blocks);
+                c->cached = 0;
+            }
+            for (const auto &d : drop) (void)hipFree(d.second);
+            e = drop.empty() ? e : hipMalloc(&p, b);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(LX_ENOMEM, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+        }
+    }
+    s->allocs.emplace_back(p, b);
     s->bytes += (int64_t)b;
     *out = p;
     return LX_OK;
+}
+
+// Give a block back to the context's cache (or free it when the cache is
+// full).  The caller has synchronised every stream that used the block.
+static void release_block(lx_ctx *c, void *p, size_t b) {
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (c->cached + (int64_t)b <= c->cache_limit) {
+            c->blocks.emplace(b, p);
+            c->cached += (int64_t)b;
+            return;
+        }
+    }
+    (void)hipFree(p);
 }
 
 // ---- CPML profiles (maxwell.cc:997-1280), FP64 ---------------------------
@@ -573,13 +612,12 @@ lx_status ensure_fused_buffers(lx_sim *s) {
 
 static void dfree(lx_sim *s, void *p) {
     if (!p) return;
-    auto it = std::find(s->allocs.begin(), s->allocs.end(), p);
+    auto it = std::find_if(s->allocs.begin(), s->allocs.end(), [&](const auto &a) { return a.first == p; });
     if (it == s->allocs.end()) return;
-    size_t bytes = 0;
-    (void)hipMemPtrGetInfo(p, &bytes);
-    (void)hipFree(p);
+    const size_t bytes = it->second;
     s->allocs.erase(it);
     s->bytes -= (int64_t)bytes;
+    release_block(s->ctx, p, bytes);
 }
 
 void release_fused_buffers(lx_sim *s) {
@@ -604,6 +642,58 @@ extern "C" {
 const char *lx_last_error(void) { return lx::error_buffer(); }
 const char *lx_version(void) { return "luxum-mi355x 0.1 (gfx950)"; }
 
+}  // extern "C"
+
+// Streams of destroyed simulations go back to their context for reuse
+// (creating a stream costs milliseconds; a server creates two per request).
+static lx_status take_stream(lx_ctx *c, hipStream_t *out) {
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (!c->streams.empty()) {
+            *out = c->streams.back();
+            c->streams.pop_back();
+            return LX_OK;
+        }
+    }
+    const hipError_t he = hipStreamCreateWithFlags(out, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        *out = nullptr;
+        return fail(LX_EDEVICE, "stream: %s", hipGetErrorString(he));
+    }
+    return LX_OK;
+}
+
+static void give_stream(lx_ctx *c, hipStream_t st);
+
+namespace lx {
+lx_status ensure_side_stream(lx_sim *s) { return s->ss ? LX_OK : take_stream(s->ctx, &s->ss); }
+
+void prewarm_streams(lx_ctx *c, int n) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    while ((int)c->streams.size() < n) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            return;
+        }
+        c->streams.push_back(st);
+    }
+}
+}  // namespace lx
+
+static void give_stream(lx_ctx *c, hipStream_t st) {
+    if (!st) return;
+    if (!c) {
+        (void)hipStreamDestroy(st);
+        return;
+    }
+    (void)hipStreamSynchronize(st);
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->streams.push_back(st);
+}
+
+extern "C" {
+
 lx_status lx_ctx_create(int32_t device, lx_ctx **out) {
     if (!out) return fail(LX_EINVAL, "lx_ctx_create: null pointer");
     int n = 0;
@@ -616,11 +706,18 @@ lx_status lx_ctx_create(int32_t device, lx_ctx **out) {
     size_t fr = 0, tot = 0;
     HIP_TRY(hipMemGetInfo(&fr, &tot));
     c->total_mem = (int64_t)tot;
+    c->cache_limit = std::min<int64_t>(16LL << 30, c->total_mem / 16);
     *out = c;
     return LX_OK;
 }
 
-void lx_ctx_destroy(lx_ctx *ctx) { delete ctx; }
+void lx_ctx_destroy(lx_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    for (hipStream_t st : ctx->streams) (void)hipStreamDestroy(st);
+    for (const auto &b : ctx->blocks) (void)hipFree(b.second);
+    delete ctx;
+}
 
 lx_status lx_ctx_mem_info(lx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes) {
     if (!ctx) return fail(LX_EINVAL, "null context");
@@ -639,7 +736,7 @@ void lx_sim_destroy(lx_sim *s) {
     if (s->cs) (void)hipStreamSynchronize(s->cs);
     if (s->ss) (void)hipStreamSynchronize(s->ss);
     drop_graph(s);
-    for (void *p : s->allocs) (void)hipFree(p);
+    for (const auto &a : s->allocs) release_block(s->ctx, a.first, a.second);
     if (s->prec) (void)hipFree(s->prec);
     for (int q = 0; q < 2; ++q) {
         if (s->stage[q]) (void)hipFree(s->stage[q]);
@@ -647,8 +744,8 @@ void lx_sim_destroy(lx_sim *s) {
         if (s->done[q]) (void)hipEventDestroy(s->done[q]);
     }
     if (s->rd_stage) (void)hipFree(s->rd_stage);
-    if (s->cs) (void)hipStreamDestroy(s->cs);
-    if (s->ss) (void)hipStreamDestroy(s->ss);
+    give_stream(s->ctx, s->cs);
+    give_stream(s->ctx, s->ss);
     delete s;
 }
 
@@ -681,10 +778,8 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
         s->sigma_max = sopt * p->sigma_ratio;
     }
     auto bail = [&](lx_status e) { lx_sim_destroy(s); return e; };
-    hipError_t he = hipStreamCreateWithFlags(&s->cs, hipStreamNonBlocking);
-    if (he != hipSuccess) return bail(fail(LX_EDEVICE, "stream: %s", hipGetErrorString(he)));
-    he = hipStreamCreateWithFlags(&s->ss, hipStreamNonBlocking);
-    if (he != hipSuccess) return bail(fail(LX_EDEVICE, "stream: %s", hipGetErrorString(he)));
+    if ((st = take_stream(ctx, &s->cs))) return bail(st);   // the side stream is taken on first use
+    hipError_t he = hipSuccess;
     for (int q = 0; q < 2; ++q) {
         if (hipEventCreateWithFlags(&s->ready[q], hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&s->done[q], hipEventDisableTiming) != hipSuccess)
@@ -838,12 +933,22 @@ lx_status lx_sim_step(lx_sim *s, int32_t nsteps) {
     return LX_OK;
 }
 
+// Wait for a stream unless it is already idle (a query is far cheaper than
+// a synchronisation call; batches sync every member after each step).
+static lx_status drain(hipStream_t st) {
+    const hipError_t q = hipStreamQuery(st);
+    if (q == hipSuccess) return LX_OK;
+    if (q != hipErrorNotReady) HIP_TRY(q);
+    HIP_TRY(hipStreamSynchronize(st));
+    return LX_OK;
+}
+
 lx_status lx_sim_sync(lx_sim *s) {
     if (!s) return fail(LX_EINVAL, "null simulation");
     HIP_TRY(hipSetDevice(s->ctx->device));
-    HIP_TRY(hipStreamSynchronize(s->cs));
-    HIP_TRY(hipStreamSynchronize(s->ss));
-    return LX_OK;
+    lx_status st = drain(s->cs);
+    if (st || !s->ss) return st;
+    return drain(s->ss);
 }
 
 static lx_status gather(lx_sim *s, int comp, double *dst_dev, int ext[3]) {
@@ -896,6 +1001,8 @@ lx_status lx_sim_snapshot_async(lx_sim *s, int32_t comp, double *dst, size_t n) 
     const size_t m = (size_t)ext[0] * ext[1] * ext[2];
     if (n != m) return fail(LX_EINVAL, "buffer holds %zu values, component has %zu", n, m);
     HIP_TRY(hipSetDevice(s->ctx->device));
+    lx_status sst = ensure_side_stream(s);
+    if (sst) return sst;
     const int slot = s->snap_next;
     s->snap_next ^= 1;
     if (s->stage_n[slot] < m) {

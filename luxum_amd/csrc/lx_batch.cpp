@@ -39,6 +39,7 @@ struct Group {
     int nouter = 0, ninner = 0, naux = 0;
     int par = 0;                              // buffer index holding the current state
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // two chunks starting at parity 0 / 1
+    int stable = 0;                           // stepped at least once since the last rebuild
 };
 
 }  // namespace
@@ -51,7 +52,11 @@ struct lx_batch {
     bool dirty = true;
     Group grp[2];                   // FP32 and FP16 storage
     std::vector<lx_sim *> others;
-    std::vector<void *> allocs;     // descriptor arrays and snapshot buffers
+    std::vector<void *> allocs;     // snapshot buffers
+    // descriptor arrays per group (sims, jobs, aux lines), kept across
+    // rebuilds and grown when a membership needs more
+    void *desc[2][3] = {};
+    size_t desc_cap[2][3] = {};
     int graphs = 1;
 };
 
@@ -80,6 +85,7 @@ void free_group(Group &g) {
         x = nullptr;
     }
     g.par = 0;
+    g.stable = 0;
     g.sims.clear();
     g.d_sims = nullptr;
     g.d_jobs = nullptr;
@@ -92,14 +98,24 @@ void free_allocs(lx_batch *b) {
     b->allocs.clear();
 }
 
-lx_status balloc(lx_batch *b, void **out, size_t bytes) {
-    void *p = nullptr;
-    hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(LX_ENOMEM, "batch allocation of %zu bytes failed", bytes);
+// Descriptor array `kind` of group `gi`, at least `bytes` long (the batch
+// stream is idle when this runs: rebuild synchronises it first).
+lx_status desc_buf(lx_batch *b, int gi, int kind, size_t bytes, void **out) {
+    void *&p = b->desc[gi][kind];
+    size_t &cap = b->desc_cap[gi][kind];
+    if (cap < bytes) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes + bytes / 2, 4096);
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return fail(LX_ENOMEM, "batch allocation of %zu bytes failed", want);
+        }
+        cap = want;
     }
-    b->allocs.push_back(p);
     *out = p;
     return LX_OK;
 }
@@ -223,16 +239,17 @@ lx_status rebuild(lx_batch *b) {
         ojobs.insert(ojobs.end(), jobs.begin(), jobs.end());
         jobs.swap(ojobs);
         void *p = nullptr;
-        lx_status st = balloc(b, &p, sizeof(BSim) * hs.size());
+        const int gi = g.fp16;
+        lx_status st = desc_buf(b, gi, 0, sizeof(BSim) * hs.size(), &p);
         if (st) return st;
         g.d_sims = (BSim *)p;
         HIP_TRY(hipMemcpy(g.d_sims, hs.data(), sizeof(BSim) * hs.size(), hipMemcpyHostToDevice));
-        st = balloc(b, &p, sizeof(BJob) * jobs.size());
+        st = desc_buf(b, gi, 1, sizeof(BJob) * jobs.size(), &p);
         if (st) return st;
         g.d_jobs = (BJob *)p;
         HIP_TRY(hipMemcpy(g.d_jobs, jobs.data(), sizeof(BJob) * jobs.size(), hipMemcpyHostToDevice));
         if (!aux.empty()) {
-            st = balloc(b, &p, sizeof(BAux) * aux.size());
+            st = desc_buf(b, gi, 2, sizeof(BAux) * aux.size(), &p);
             if (st) return st;
             g.d_aux = (BAux *)p;
             HIP_TRY(hipMemcpy(g.d_aux, aux.data(), sizeof(BAux) * aux.size(), hipMemcpyHostToDevice));
@@ -318,6 +335,9 @@ void lx_batch_destroy(lx_batch *b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (Group &g : b->grp) free_group(g);
     free_allocs(b);
+    for (auto &row : b->desc)
+        for (void *p : row)
+            if (p) (void)hipFree(p);
     for (lx_sim *s : b->sims) s->batch = nullptr;
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->pre) (void)hipEventDestroy(b->pre);
@@ -364,9 +384,13 @@ lx_status lx_batch_step(lx_batch *b, int32_t nsteps) {
         lx_status st = rebuild(b);
         if (st) return st;
     }
-    // order after each member's pending work
+    // order after each member's pending work (members whose stream is idle
+    // need no event: a served batch steps right after a sync)
     for (Group &g : b->grp)
         for (lx_sim *s : g.sims) {
+            const hipError_t q = hipStreamQuery(s->cs);
+            if (q == hipSuccess) continue;
+            if (q != hipErrorNotReady) HIP_TRY(q);
             HIP_TRY(hipEventRecord(b->pre, s->cs));
             HIP_TRY(hipStreamWaitEvent(b->stream, b->pre, 0));
         }
@@ -375,7 +399,10 @@ lx_status lx_batch_step(lx_batch *b, int32_t nsteps) {
         int par = g.par;
         int full = nsteps / kTT;
         const int rem = nsteps % kTT;
-        if (full >= 2) {
+        // a graph pays for its capture only when it is replayed: capture once
+        // the membership has survived a step, or for a long first step (a
+        // served batch changes members at most ticks and launches directly)
+        if (full >= 2 && (g.stable || full >= 32)) {
             lx_status st = ensure_pair_graph(b, g, par);
             if (st) return st;
             if (g.gexec[par])
@@ -401,6 +428,7 @@ lx_status lx_batch_step(lx_batch *b, int32_t nsteps) {
             g.par = par;
         }
         for (lx_sim *s : g.sims) s->steps += nsteps;
+        g.stable = 1;
     }
     HIP_TRY(hipEventRecord(b->done, b->stream));
     for (Group &g : b->grp)

@@ -319,6 +319,7 @@ lx_status lx_writer_write(lx_writer *w, double t) {
         w->next ^= 1;
     }
     lx_status st = hipSetDevice(s->ctx->device) == hipSuccess ? LX_OK : fail(LX_EDEVICE, "hipSetDevice failed");
+    if (!st) st = lx::ensure_side_stream(s);
     // gather on the compute stream (ordered with the steps), copy out on the side stream
     if (!st) st = lx::gather_output(s, w->what, w->comp, w->dstage[slot], w->ext);
     if (!st && (hipEventRecord(s->ready[0], s->cs) != hipSuccess || hipStreamWaitEvent(s->ss, s->ready[0], 0) != hipSuccess ||

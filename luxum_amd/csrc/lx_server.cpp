@@ -13,7 +13,10 @@
 // at kTick: every request ends exactly on a tick boundary.  Batched mode
 // steps one lx_batch (planar requests in one tiled launch per chunk, the
 // others on their own streams, concurrently); plain mode runs one request at
-// a time, start to end, with the single-simulation kernels.
+// a time, start to end, with the single-simulation kernels.  Requests that
+// arrived are set up (allocation, tables, material upload) while the tick
+// runs on the device, and join the next one.  The server fills the
+// context's stream pool once, so admitting a request creates no stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +36,7 @@ using lx::fail;
 namespace {
 
 constexpr int kTick = 128;
+constexpr int kPrewarm = 64;   // pooled streams created with the server
 
 struct Req {
     lx_params p{};
@@ -67,6 +71,7 @@ struct lx_server {
     std::vector<Req *> reqs;      // by id
     std::deque<Req *> queue;      // not yet admitted, arrival order
     std::vector<Req *> active;
+    std::vector<Req *> retiring;  // finished on the device, results not yet read back
     int64_t used = 0;
     lx_server_stats st{};
     Clock::time_point t0;
@@ -171,12 +176,27 @@ void wait_for_arrival(lx_server *s) {
     if (dt > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(dt, 0.5)));
 }
 
+lx_status retire_pending(lx_server *s) {
+    for (Req *r : s->retiring) {
+        lx_status st = retire(s, r);
+        if (st) return st;
+    }
+    s->retiring.clear();
+    return LX_OK;
+}
+
 lx_status tick(lx_server *s) {
     int32_t k = kTick;
     for (Req *r : s->active) k = std::min(k, r->steps - r->done);
+    const std::vector<Req *> stepped = s->active;   // the members of this tick
     lx_status st;
     if (s->mode == LX_SERVE_BATCHED) {
         if ((st = lx_batch_step(s->batch, k))) return st;
+        // while the tick runs on the device: read back the requests that
+        // finished in the previous tick (their memory returns to the pool)
+        // and set up the requests that arrived meanwhile
+        if ((st = retire_pending(s))) return st;
+        if ((st = admit_arrived(s))) return st;
         if ((st = lx_batch_sync(s->batch))) return st;
     } else {
         Req *r = s->active.front();
@@ -186,14 +206,19 @@ lx_status tick(lx_server *s) {
     }
     s->st.ticks++;
     std::vector<Req *> keep;
-    for (Req *r : s->active) {
+    for (Req *r : stepped) {
         r->done += k;
-        if (r->done >= r->steps) {
-            if ((st = retire(s, r))) return st;
-        } else {
+        if (r->done < r->steps) {
             keep.push_back(r);
+        } else if (s->mode == LX_SERVE_BATCHED) {
+            // leave the batch now (it is idle); results are read during the next tick
+            if ((st = lx_batch_remove(s->batch, r->sim))) return st;
+            s->retiring.push_back(r);
+        } else if ((st = retire(s, r))) {
+            return st;
         }
     }
+    for (size_t q = stepped.size(); q < s->active.size(); ++q) keep.push_back(s->active[q]);   // admitted during the tick
     s->active.swap(keep);
     return LX_OK;
 }
@@ -212,10 +237,15 @@ lx_status lx_server_create(lx_ctx *ctx, int32_t mode, int64_t hbm_cap_bytes, int
     s->mode = mode;
     s->cap = hbm_cap_bytes;
     s->max_active = mode == LX_SERVE_PLAIN ? 1 : max_active;
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        delete s;
+        return fail(LX_EDEVICE, "hipSetDevice(%d) failed", ctx->device);
+    }
+    lx::prewarm_streams(ctx, std::min(s->max_active, kPrewarm));
     if (mode == LX_SERVE_BATCHED) {
         lx_status st = lx_batch_create(ctx, &s->batch);
         if (st) {
-            delete s;
+            lx_server_destroy(s);
             return st;
         }
     }
@@ -278,11 +308,12 @@ lx_status lx_server_submit(lx_server *s, const lx_request *q, int64_t *id) {
 lx_status lx_server_run(lx_server *s) {
     if (!s) return fail(LX_EINVAL, "null server");
     s->t0 = Clock::now();
-    while (!s->queue.empty() || !s->active.empty()) {
-        lx_status st = admit_arrived(s);
-        if (st) return st;
+    while (!s->queue.empty() || !s->active.empty() || !s->retiring.empty()) {
+        lx_status st;
+        if (s->active.empty() && (st = retire_pending(s))) return st;
+        if ((st = admit_arrived(s))) return st;
         if (s->active.empty()) {
-            wait_for_arrival(s);
+            if (!s->queue.empty()) wait_for_arrival(s);
             continue;
         }
         if ((st = tick(s))) return st;

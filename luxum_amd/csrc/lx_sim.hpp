@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/luxum.h"
@@ -11,6 +13,12 @@
 struct lx_ctx {
     int device;
     int64_t total_mem;
+    std::mutex mu;
+    std::vector<hipStream_t> streams;   // idle streams of destroyed simulations
+    // device blocks of destroyed simulations, by size, reused by exact size
+    // (a server creates and destroys simulations of a few grid sizes)
+    std::multimap<size_t, void *> blocks;
+    int64_t cached = 0, cache_limit = 0;
 };
 
 struct lx_sim {
@@ -30,7 +38,7 @@ struct lx_sim {
     // constants (maxwell.cc:63-84)
     double eps0 = 0, mu0 = 0, c0 = 0, dt = 0, eta0 = 0, sigma_max = 0;
     hipStream_t cs = nullptr, ss = nullptr;
-    std::vector<void *> allocs;
+    std::vector<std::pair<void *, size_t>> allocs;   // device blocks and their sizes
     int64_t bytes = 0;
     void *F = nullptr;          // six components, component c at F + c*fstride elements
     long long fstride = 0;
@@ -85,6 +93,12 @@ size_t psi_total(const lx_sim *s);
 ProbeArgs probe_args(const lx_sim *s);
 void drop_graph(lx_sim *s);
 bool fused_ok(const lx_sim *s);
+// The side stream (snapshot copies) is taken from the context's pool on
+// first use.
+lx_status ensure_side_stream(lx_sim *s);
+// Fill the context's stream pool up to n idle streams (servers call it once,
+// so that admitting a request does not create streams).
+void prewarm_streams(lx_ctx *c, int n);
 lx_status ensure_fused_buffers(lx_sim *s);
 // enqueue on the compute stream: one component (what == LX_OUT_FIELD) or
 // |E|^2 (LX_OUT_E2) into dst_dev in the FP64 host layout of extents ext
