@@ -850,25 +850,28 @@ lx_status lx_sim_set_materials(lx_sim *s, const uint8_t *ids, const double *eps_
         if (!(eps_r[q] > 0)) return fail(LX_EINVAL, "eps_r must be positive");
     HIP_TRY(hipSetDevice(s->ctx->device));
     const int Nx = s->N[0], Ny = s->N[1], Nz = s->N[2];
-    // transpose (i,j,k) C order -> x-fastest device layout
-    std::vector<uint8_t> dev((size_t)s->plane * Nz, 0);
-    for (int i = 0; i < Nx; ++i)
-        for (int j = 0; j < Ny; ++j) {
-            const uint8_t *src = ids + ((size_t)i * Ny + j) * Nz;
-            for (int k = 0; k < Nz; ++k) {
-                if (src[k] >= n_ids) return fail(LX_EINVAL, "material id %d >= n_ids", (int)src[k]);
-                dev[(size_t)k * s->plane + (size_t)j * s->px + i] = src[k];
-            }
-        }
+    const size_t n = (size_t)Nx * Ny * Nz;
+    uint8_t top = 0;
+    for (size_t q = 0; q < n; ++q) top = std::max(top, ids[q]);
+    if (top >= n_ids) return fail(LX_EINVAL, "material id %d >= n_ids", (int)top);
     for (int q = 0; q < 256; ++q) {
         const double er = q < n_ids ? eps_r[q] : 1.0;
         const double Cb = s->dt / (s->eps0 * er);             // maxwell_three_dim_pml.cc:122-128
         s->cb_host[q] = (float)(Cb / (s->eta0 * s->p.dx));
     }
+    // upload in the caller's C order and transpose on the device into the
+    // x-fastest layout (the padding columns keep id 0)
+    void *raw = nullptr;
+    lx_status st = dalloc(s, &raw, n);
+    if (st) return st;
     HIP_TRY(hipStreamSynchronize(s->cs));
-    HIP_TRY(hipMemcpy(s->mat, dev.data(), dev.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(raw, ids, n, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->cb, s->cb_host, sizeof(s->cb_host), hipMemcpyHostToDevice));
-    return LX_OK;
+    launch_ids_to_device((const uint8_t *)raw, s->mat, Nx, Ny, Nz, s->px, s->plane, s->cs);
+    st = check_launch();
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    dfree(s, raw);
+    return st;
 }
 
 lx_status lx_sim_set_probes(lx_sim *s, const int32_t *cijk, int32_t n) {

@@ -222,18 +222,22 @@ lx_status rebuild(lx_batch *b) {
             d.prec = s->prec;
             d.pcap = s->pcap;
             d.hscale = 1.0 / s->eta0;
+        }
+        // large simulations first: long jobs start early, small ones fill the
+        // tail (members ordered by size, each member's tiles in order)
+        std::vector<int> order(g.sims.size());
+        for (size_t q = 0; q < order.size(); ++q) order[q] = (int)q;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return (int64_t)g.sims[x]->N[0] * g.sims[x]->N[2] > (int64_t)g.sims[y]->N[0] * g.sims[y]->N[2];
+        });
+        for (int q : order) {
+            const lx_sim *s = g.sims[q];
             for (int tz = 0; tz < (s->N[2] + kTH - 1) / kTH; ++tz)
                 for (int tx = 0; tx < (s->N[0] + kTW - 1) / kTW; ++tx) {
-                    const BJob j{(int)q, tx * kTW, tz * kTH};
+                    const BJob j{q, tx * kTW, tz * kTH};
                     (job_inner(s, j.i0, j.k0) ? jobs : ojobs).push_back(j);
                 }
         }
-        // large simulations first: long jobs start early, small ones fill the tail
-        auto larger = [&](const BJob &x, const BJob &y) {
-            return g.sims[x.sim]->N[0] * g.sims[x.sim]->N[2] > g.sims[y.sim]->N[0] * g.sims[y.sim]->N[2];
-        };
-        std::stable_sort(ojobs.begin(), ojobs.end(), larger);
-        std::stable_sort(jobs.begin(), jobs.end(), larger);
         g.nouter = (int)ojobs.size();
         g.ninner = (int)jobs.size();
         ojobs.insert(ojobs.end(), jobs.begin(), jobs.end());

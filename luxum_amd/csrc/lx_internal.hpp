@@ -189,6 +189,9 @@ template <typename T>
 void launch_monitor(const Grid<T> &g, int planar, double *out, hipStream_t s);
 
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
+// material ids, host C order (i, j, k) -> device layout (k, j, x)
+void launch_ids_to_device(const uint8_t *src, uint8_t *dst, int nx, int ny, int nz, int px, long long plane,
+                          hipStream_t s);
 void launch_zero_bytes(void *p, size_t n, hipStream_t s);
 
 }  // namespace lx

@@ -642,6 +642,17 @@ __global__ __launch_bounds__(256) void k_monitor(Grid<T> g, int planar, double *
     }
 }
 
+// Material ids from the host's C order (i, j, k; k fastest) to the device
+// layout (k, j, x; x fastest, rows px long).  One thread per (i, j, k),
+// threads along i so that the stores are contiguous.
+__global__ void k_ids_to_device(const uint8_t *__restrict__ src, uint8_t *__restrict__ dst, int nx, int ny, int nz,
+                                int px, long long plane) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, k = blockIdx.z;
+    if (i >= nx) return;
+    dst[(long long)k * plane + (long long)j * px + i] = src[((long long)i * ny + j) * nz + k];
+}
+
 __global__ void k_fill_u8(uint8_t *p, uint8_t v, size_t n) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; q < n; q += (size_t)gridDim.x * blockDim.x) p[q] = v;
@@ -719,6 +730,10 @@ template <typename T>
 void launch_monitor(const Grid<T> &g, int planar, double *out, hipStream_t s) {
     dim3 grd((g.nx + 255) / 256, g.ny, g.nz);
     hipLaunchKernelGGL(k_monitor<T>, grd, dim3(256), 0, s, g, planar, out);
+}
+void launch_ids_to_device(const uint8_t *src, uint8_t *dst, int nx, int ny, int nz, int px, long long plane,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_ids_to_device, dim3((nx + 255) / 256, ny, nz), dim3(256), 0, s, src, dst, nx, ny, nz, px, plane);
 }
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s) {
     size_t blocks = (n + 255) / 256;
