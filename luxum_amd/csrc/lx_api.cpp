@@ -668,6 +668,14 @@ static void give_stream(lx_ctx *c, hipStream_t st);
 namespace lx {
 lx_status ensure_side_stream(lx_sim *s) { return s->ss ? LX_OK : take_stream(s->ctx, &s->ss); }
 
+void member_enter(lx_sim *s) {
+    if (s->batch_wait) {
+        (void)hipStreamWaitEvent(s->cs, s->batch_wait, 0);
+        s->batch_wait = nullptr;
+    }
+    s->cs_dirty = true;
+}
+
 void prewarm_streams(lx_ctx *c, int n) {
     std::lock_guard<std::mutex> lk(c->mu);
     while ((int)c->streams.size() < n) {
@@ -849,6 +857,7 @@ lx_status lx_sim_set_materials(lx_sim *s, const uint8_t *ids, const double *eps_
     for (int q = 0; q < n_ids; ++q)
         if (!(eps_r[q] > 0)) return fail(LX_EINVAL, "eps_r must be positive");
     HIP_TRY(hipSetDevice(s->ctx->device));
+    member_enter(s);
     const int Nx = s->N[0], Ny = s->N[1], Nz = s->N[2];
     const size_t n = (size_t)Nx * Ny * Nz;
     uint8_t top = 0;
@@ -877,6 +886,7 @@ lx_status lx_sim_set_materials(lx_sim *s, const uint8_t *ids, const double *eps_
 lx_status lx_sim_set_probes(lx_sim *s, const int32_t *cijk, int32_t n) {
     if (!s || (n > 0 && !cijk) || n < 0) return fail(LX_EINVAL, "lx_sim_set_probes: bad arguments");
     HIP_TRY(hipSetDevice(s->ctx->device));
+    member_enter(s);
     std::vector<long long> idx(n);
     std::vector<int> comp(n);
     for (int q = 0; q < n; ++q) {
@@ -909,6 +919,7 @@ lx_status lx_sim_step(lx_sim *s, int32_t nsteps) {
     if (!s || nsteps < 0) return fail(LX_EINVAL, "lx_sim_step: bad arguments");
     if (nsteps == 0) return LX_OK;
     HIP_TRY(hipSetDevice(s->ctx->device));
+    member_enter(s);
     lx_status st = ensure_probe_cap(s, s->steps + nsteps);
     if (st) return st;
     int left = nsteps;
@@ -949,9 +960,14 @@ static lx_status drain(hipStream_t st) {
 lx_status lx_sim_sync(lx_sim *s) {
     if (!s) return fail(LX_EINVAL, "null simulation");
     HIP_TRY(hipSetDevice(s->ctx->device));
+    if (s->batch_wait) {
+        HIP_TRY(hipEventSynchronize(s->batch_wait));
+        s->batch_wait = nullptr;
+    }
     lx_status st = drain(s->cs);
-    if (st || !s->ss) return st;
-    return drain(s->ss);
+    if (st) return st;
+    s->cs_dirty = false;
+    return s->ss ? drain(s->ss) : LX_OK;
 }
 
 static lx_status gather(lx_sim *s, int comp, double *dst_dev, int ext[3]) {
@@ -984,6 +1000,7 @@ lx_status lx_sim_read_field(lx_sim *s, int32_t comp, double *dst, size_t n) {
     const size_t m = (size_t)ext[0] * ext[1] * ext[2];
     if (n != m) return fail(LX_EINVAL, "buffer holds %zu values, component has %zu", n, m);
     HIP_TRY(hipSetDevice(s->ctx->device));
+    member_enter(s);
     if (s->rd_n < m) {
         if (s->rd_stage) (void)hipFree(s->rd_stage);
         s->rd_stage = nullptr;
@@ -1004,6 +1021,7 @@ lx_status lx_sim_snapshot_async(lx_sim *s, int32_t comp, double *dst, size_t n) 
     const size_t m = (size_t)ext[0] * ext[1] * ext[2];
     if (n != m) return fail(LX_EINVAL, "buffer holds %zu values, component has %zu", n, m);
     HIP_TRY(hipSetDevice(s->ctx->device));
+    member_enter(s);
     lx_status sst = ensure_side_stream(s);
     if (sst) return sst;
     const int slot = s->snap_next;
@@ -1029,6 +1047,7 @@ lx_status lx_sim_snapshot_async(lx_sim *s, int32_t comp, double *dst, size_t n) 
 lx_status lx_sim_read_probes(lx_sim *s, double *dst, size_t n, int32_t *n_steps) {
     if (!s || !n_steps) return fail(LX_EINVAL, "lx_sim_read_probes: null pointer");
     HIP_TRY(hipSetDevice(s->ctx->device));
+    member_enter(s);
     HIP_TRY(hipStreamSynchronize(s->cs));
     const int rec = std::min(s->steps, s->pcap);
     const size_t m = (size_t)rec * s->np;
@@ -1042,6 +1061,7 @@ lx_status lx_sim_read_probes(lx_sim *s, double *dst, size_t n, int32_t *n_steps)
 lx_status lx_sim_monitor(lx_sim *s, double *max_e2, double *energy) {
     if (!s) return fail(LX_EINVAL, "null simulation");
     HIP_TRY(hipSetDevice(s->ctx->device));
+    member_enter(s);
     launch_zero_bytes(s->mon, 32, s->cs);
     if (s->fp16) launch_monitor(make_grid<__half>(s), s->planar, s->mon, s->cs);
     else launch_monitor(make_grid<float>(s), s->planar, s->mon, s->cs);
@@ -1113,6 +1133,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
     if (!s) return fail(LX_EINVAL, "null simulation");
     if (s->steps != 0) return fail(LX_EINVAL, "lx_sim_autotune must run before the first step");
     HIP_TRY(hipSetDevice(s->ctx->device));
+    member_enter(s);
     using Key = std::tuple<int, int, int, int, int, int, int>;
     static std::mutex mu;
     static std::map<Key, std::array<int, 3>> cache;   // (kz, nw, fused)
@@ -1215,6 +1236,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
 lx_status lx_sim_time_kernels(lx_sim *s, int32_t nsteps, double *ms_h, double *ms_e, double *ms_step) {
     if (!s || nsteps < 1) return fail(LX_EINVAL, "lx_sim_time_kernels: bad arguments");
     HIP_TRY(hipSetDevice(s->ctx->device));
+    member_enter(s);
     lx_status st = ensure_probe_cap(s, s->steps + nsteps);
     if (st) return st;
     std::vector<hipEvent_t> ev(4 * (size_t)nsteps + 2);

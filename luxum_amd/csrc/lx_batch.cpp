@@ -392,6 +392,8 @@ lx_status lx_batch_step(lx_batch *b, int32_t nsteps) {
     // need no event: a served batch steps right after a sync)
     for (Group &g : b->grp)
         for (lx_sim *s : g.sims) {
+            if (!s->cs_dirty) continue;
+            s->cs_dirty = false;
             const hipError_t q = hipStreamQuery(s->cs);
             if (q == hipSuccess) continue;
             if (q != hipErrorNotReady) HIP_TRY(q);
@@ -436,7 +438,7 @@ lx_status lx_batch_step(lx_batch *b, int32_t nsteps) {
     }
     HIP_TRY(hipEventRecord(b->done, b->stream));
     for (Group &g : b->grp)
-        for (lx_sim *s : g.sims) HIP_TRY(hipStreamWaitEvent(s->cs, b->done, 0));
+        for (lx_sim *s : g.sims) s->batch_wait = b->done;   // the member's own next work waits for this step
     for (lx_sim *s : b->others) {
         lx_status st = lx_sim_step(s, nsteps);
         if (st) return st;
@@ -449,6 +451,8 @@ lx_status lx_batch_sync(lx_batch *b) {
     HIP_TRY(hipSetDevice(b->ctx->device));
     HIP_TRY(hipStreamSynchronize(b->stream));
     for (lx_sim *s : b->sims) {
+        s->batch_wait = nullptr;   // that step is complete
+        if (!s->cs_dirty && !s->ss) continue;
         lx_status st = lx_sim_sync(s);
         if (st) return st;
     }

@@ -320,6 +320,7 @@ lx_status lx_writer_write(lx_writer *w, double t) {
     }
     lx_status st = hipSetDevice(s->ctx->device) == hipSuccess ? LX_OK : fail(LX_EDEVICE, "hipSetDevice failed");
     if (!st) st = lx::ensure_side_stream(s);
+    if (!st) lx::member_enter(s);
     // gather on the compute stream (ordered with the steps), copy out on the side stream
     if (!st) st = lx::gather_output(s, w->what, w->comp, w->dstage[slot], w->ext);
     if (!st && (hipEventRecord(s->ready[0], s->cs) != hipSuccess || hipStreamWaitEvent(s->ss, s->ready[0], 0) != hipSuccess ||

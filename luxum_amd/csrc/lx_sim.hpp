@@ -28,6 +28,12 @@ struct lx_sim {
     double *bsnap = nullptr;    // incident line snapshots of one chunk: E then H, [8][win] each
     int bsnap_win = 0;
     lx_batch *batch = nullptr;
+    // ordering with a batch without per-member stream operations: the batch
+    // waits for a member's stream only if the member enqueued work of its own
+    // (cs_dirty); a member's own work waits for the batch's last step
+    // (batch_wait, set by the batch) only when there is such work
+    hipEvent_t batch_wait = nullptr;
+    bool cs_dirty = false;
     lx_ctx *ctx = nullptr;
     lx_params p{};
     int planar = 0, fp16 = 0, pml = 0;
@@ -96,6 +102,8 @@ bool fused_ok(const lx_sim *s);
 // The side stream (snapshot copies) is taken from the context's pool on
 // first use.
 lx_status ensure_side_stream(lx_sim *s);
+// Called by every simulation entry point before it enqueues work on s->cs.
+void member_enter(lx_sim *s);
 // Fill the context's stream pool up to n idle streams (servers call it once,
 // so that admitting a request does not create streams).
 void prewarm_streams(lx_ctx *c, int n);
