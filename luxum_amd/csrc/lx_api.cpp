@@ -1168,7 +1168,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         for (int nw : {0, 8, 4}) {
             if (nw && !can_fuse) continue;
             s->fused = nw != 0;
-            for (int kz : {4, 8, 16, 32, 64}) {
+            for (int kz : {4, 5, 6, 8, 12, 16, 32, 64}) {
                 if (kz > s->N[2] && kz != 4) continue;
                 s->kz = kz;
                 if (nw) s->nw = nw;
