@@ -57,6 +57,8 @@ class SimInfo(C.Structure):
         ("kz", C.c_int32),
         ("waves", C.c_int32),
         ("core_bytes", C.c_int64),
+        ("cells_per_lane", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -475,10 +475,10 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
             if (corr) launch_tfsf_hpre(gq, s->tfH, s->inc, s->aux, pr, l, s->cs);
             // timed parts: [0,1] interior box, [2,3] CPML shell
             if (ev) (void)hipEventRecord((*ev)[4 * q + 0], s->cs);
-            launch_step3(gq, s->nw, 0, s->cs);
+            launch_step3(gq, s->nw, s->vec, 0, s->cs);
             if (ev) (void)hipEventRecord((*ev)[4 * q + 1], s->cs);
             if (ev) (void)hipEventRecord((*ev)[4 * q + 2], s->cs);
-            launch_step3(gq, s->nw, 1, s->cs);
+            launch_step3(gq, s->nw, s->vec, 1, s->cs);
             if (ev) (void)hipEventRecord((*ev)[4 * q + 3], s->cs);
             Grid<T> gn = g;
             gn.F = gq.Fn;
@@ -817,11 +817,14 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
     if ((st = build_incident(s))) return bail(st);
     if (!s->planar) {
         // 3D: the half-step pair by default; LX_FUSED=1 selects the fused step
-        // where it applies (lx_sim_autotune times both and keeps the faster)
+        // where it applies, LX_FUSED=2 the fused step with two cells per lane
+        // where the row pitch allows it (lx_sim_autotune times them all and
+        // keeps the fastest)
         const char *fz = std::getenv("LX_FUSED");
         if (fz && *fz && *fz != '0' && fused_ok(s)) {
             if ((st = ensure_fused_buffers(s))) return bail(st);
             s->fused = 1;
+            if (*fz == '2' && (s->N[0] + 127) / 128 * 128 <= s->px) s->vec = 2;
         }
     }
     if (p->use_ps) {
@@ -1101,6 +1104,8 @@ lx_status lx_sim_get_info(lx_sim *s, lx_sim_info *info) {
     info->kz = s->kz;
     info->waves = s->fused ? s->nw : 4;
     info->core_bytes = info->bytes_per_step;
+    info->cells_per_lane = s->fused ? s->vec : 1;
+    info->reserved = 0;
     if (s->fused && s->pml) {
         int rows = 0, planes = 0;
         step3_core_extent(s->N[1], s->N[2], s->kz, s->slo, s->shi, &rows, &planes);
@@ -1136,7 +1141,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
     member_enter(s);
     using Key = std::tuple<int, int, int, int, int, int, int>;
     static std::mutex mu;
-    static std::map<Key, std::array<int, 3>> cache;   // (kz, nw, fused)
+    static std::map<Key, std::array<int, 4>> cache;   // (kz, nw, fused, vec)
     const Key key{s->N[0], s->N[1], s->N[2], s->fp16, s->pml, s->planar, 0};
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -1144,6 +1149,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         if (it != cache.end()) {
             s->kz = it->second[0];
             s->nw = it->second[1];
+            s->vec = it->second[3];
             if (it->second[2] && fused_ok(s)) {
                 lx_status st = ensure_fused_buffers(s);
                 if (st) return st;
@@ -1161,12 +1167,17 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         float best = 1e30f;
-        int best_kz = s->kz, best_nw = s->nw, best_fused = s->fused;
+        int best_kz = s->kz, best_nw = s->nw, best_fused = s->fused, best_vec = s->vec;
         // candidates: the half-step pair (nw = 0) and the fused step with 8 or
-        // 4 wavefronts per block (its second buffer set is kept only if it wins)
+        // 4 wavefronts per block and 1 or 2 cells per lane (its second buffer
+        // set is kept only if it wins)
         const int can_fuse = fused_ok(s) && ensure_fused_buffers(s) == LX_OK;
-        for (int nw : {0, 8, 4}) {
+        const bool vec2 = (s->N[0] + 127) / 128 * 128 <= s->px;
+        struct Cand { int nw, vec; };
+        for (Cand c : {Cand{0, 1}, Cand{8, 1}, Cand{4, 1}, Cand{4, 2}, Cand{8, 2}}) {
+            const int nw = c.nw;
             if (nw && !can_fuse) continue;
+            if (c.vec == 2 && !vec2) continue;
             s->fused = nw != 0;
             for (int kz : {4, 5, 6, 8, 12, 16, 32, 64}) {
                 if (kz > s->N[2] && kz != 4) continue;
@@ -1180,8 +1191,8 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
                             if (s->fused) {
                                 g.Fn = (__half *)s->F_alt;
                                 g.psin = s->psi_alt;
-                                launch_step3(g, nw, 0, s->cs);
-                                launch_step3(g, nw, 1, s->cs);
+                                launch_step3(g, nw, c.vec, 0, s->cs);
+                                launch_step3(g, nw, c.vec, 1, s->cs);
                             } else {
                                 launch_bulk(g, 0, 0, s->cs);
                                 launch_bulk(g, 0, 1, s->cs);
@@ -1191,8 +1202,8 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
                             if (s->fused) {
                                 g.Fn = (float *)s->F_alt;
                                 g.psin = s->psi_alt;
-                                launch_step3(g, nw, 0, s->cs);
-                                launch_step3(g, nw, 1, s->cs);
+                                launch_step3(g, nw, c.vec, 0, s->cs);
+                                launch_step3(g, nw, c.vec, 1, s->cs);
                             } else {
                                 launch_bulk(g, 0, 0, s->cs);
                                 launch_bulk(g, 0, 1, s->cs);
@@ -1208,10 +1219,11 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
                         best_kz = kz;
                         best_fused = s->fused;
                         if (nw) best_nw = nw;
+                        best_vec = c.vec;
                     }
                     if (rep == 1 && std::getenv("LX_VERBOSE"))
-                        std::fprintf(stderr, "lx_sim_autotune: %s nw=%d kz=%d  %.4f ms per step\n",
-                                     s->fused ? "fused" : "half-step", nw, kz, ms / 4);
+                        std::fprintf(stderr, "lx_sim_autotune: %s nw=%d vec=%d kz=%d  %.4f ms per step\n",
+                                     s->fused ? "fused" : "half-step", nw, c.vec, kz, ms / 4);
                 }
             }
         }
@@ -1219,6 +1231,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         (void)hipEventDestroy(e1);
         s->kz = best_kz;
         s->nw = best_nw;
+        s->vec = best_vec;
         s->fused = best_fused;
         if (!s->fused) release_fused_buffers(s);
     }
@@ -1227,7 +1240,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
     if (st) return st;
     {
         std::lock_guard<std::mutex> lk(mu);
-        cache[key] = {s->kz, s->nw, s->fused};
+        cache[key] = {s->kz, s->nw, s->fused, s->vec};
     }
     if (kz_out) *kz_out = s->kz;
     return LX_OK;

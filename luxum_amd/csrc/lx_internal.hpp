@@ -140,13 +140,14 @@ void launch_tfsf(const Grid<T> &g, const TfList &tf, const IncParams &inc, const
                  Launch l, int half, hipStream_t s);
 
 // Fused 3D timestep (H and E half steps, F/psi read, Fn/psin written), nw
-// wavefronts per block.  Bulk update only: the H TF/SF corrections are
+// wavefronts per block, vec (1 or 2) x cells per lane; vec 2 needs the
+// padded row pitch to hold every 128-cell chunk (px >= nx rounded up to 128).  Bulk update only: the H TF/SF corrections are
 // applied to the read buffer beforehand (launch_tfsf_hpre), the E ones to the
 // written buffer afterwards (launch_tfsf).  Two parts: part 0 is the whole
 // grid without CPML, else the interior box (rows and planes outside the y and
 // z slabs); part 1 is the CPML shell around it (nothing without CPML).
 template <typename T>
-void launch_step3(const Grid<T> &g, int nw, int part, hipStream_t s);
+void launch_step3(const Grid<T> &g, int nw, int vec, int part, hipStream_t s);
 // Rows and planes of the part-0 box of a CPML grid.
 void step3_core_extent(int ny, int nz, int kz, const int slo[3], const int shi[3], int *rows, int *planes);
 

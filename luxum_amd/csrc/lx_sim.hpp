@@ -61,6 +61,7 @@ struct lx_sim {
     int kz = 32;
     int fused = 0;              // 3D: one fused launch per step over ping-pong buffers (F/psi and F_alt/psi_alt)
     int nw = 8;                 // fused step: wavefronts per block
+    int vec = 1;                // fused step: x cells per lane (1 or 2)
     lx::IncParams inc{};
     lx::AuxLine aux{};
     int use_aux = 0;

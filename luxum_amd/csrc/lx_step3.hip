@@ -8,27 +8,28 @@
 // cell and step plus the slab psi, against 18*s + 1 for the half-step pair
 // (maxwell_three_dim_pml.cc:182-255 is the update order followed).
 //
-// Work unit: one wavefront per (64-cell x chunk, grid row j, block of kz
-// planes), one cell per lane, marching along z.  Units are numbered chunk
-// first, so the chunks of a row share a block (and an L2) and read each
-// other's edge cells while those are cached.  Per plane k:
+// Work unit: one wavefront per (x chunk of 64*V cells, grid row j, block of
+// kz planes), V consecutive cells per lane, marching along z.  Units are
+// numbered chunk first, so the chunks of a row share a block (and an L2) and
+// read each other's edge cells while those are cached.  Per plane k:
 //   loads: E_old at (j, k+1) (the z queue), at (j+1, k) and (j-1, k), H_old
-//     at (j, k) and (j-1, k), the material id, the slab psi and the edge
-//     cells x0-1 and x0+64.  All loads of a plane are issued before its
+//     at (j, k) and (j-1, k), the material ids, the slab psi and the edge
+//     cells x0-1 and x0+64V.  All loads of a plane are issued before its
 //     arithmetic and none sits in a branch that depends on loaded data, so a
 //     plane costs one memory round trip.
 //   H: H_new(j, k) of the three components; Hx, Hz of row j-1 (needed by Ex,
-//     Ez of row j) and Hy, Hz of cell x0-1 (needed by Ey, Ez of lane 0) are
-//     recomputed here and stored by their own wavefront.
+//     Ez of row j) and Hy, Hz of cell x0-1 (needed by Ey, Ez of the chunk's
+//     first cell) are recomputed here and stored by their own wavefront.
 //   E: E_new(j, k) from H_new at (j, k), (j, k-1) (registers), (j-1, k) and
-//     x-1 (previous lane).
+//     x-1 (previous element or lane).
 // The plane below the block (k0-1) is recomputed for Hx and Hy only.  Cells
 // outside a component's update range are written with their old value, so
 // both buffers always hold a complete state.  Blocks only read the old
 // buffer: there is no ordering between blocks and no barrier after the
 // coefficient load.  The H TF/SF corrections are applied to H_old beforehand
 // (k_tfsf_hpre), the E ones to E_new afterwards (k_tfsf): this is the bulk
-// update only.
+// update only.  V = 2 (two cells per lane, 8- or 4-byte loads) halves the
+// memory instructions and address arithmetic per cell.
 #include "lx_device.hpp"
 
 namespace lx {
@@ -48,15 +49,117 @@ __device__ __forceinline__ long long uo(long long v) {
     return (long long)(((unsigned long long)hi << 32) | lo);
 }
 template <typename T>
-__device__ __forceinline__ float ldb(const T *base, unsigned boff) {
-    return ld1(reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff));
+__device__ __forceinline__ const T *at(const T *base, unsigned boff) {
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
 }
 template <typename T>
-__device__ __forceinline__ void stb(T *base, unsigned boff, float v) {
-    st1(reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff), v);
+__device__ __forceinline__ T *at(T *base, unsigned boff) {
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff);
+}
+template <typename T>
+__device__ __forceinline__ float ldb(const T *base, unsigned boff) {
+    return ld1(at(base, boff));
 }
 __device__ __forceinline__ float lane0(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+}
+
+// V consecutive x cells of one lane
+template <int V>
+struct vf {
+    float v[V];
+};
+template <int V>
+__device__ __forceinline__ vf<V> vs(float a) {
+    vf<V> r;
+#pragma unroll
+    for (int e = 0; e < V; ++e) r.v[e] = a;
+    return r;
+}
+#define LX_VOP(op)                                                                         \
+    template <int V>                                                                       \
+    __device__ __forceinline__ vf<V> operator op(const vf<V> &a, const vf<V> &b) {         \
+        vf<V> r;                                                                           \
+        _Pragma("unroll") for (int e = 0; e < V; ++e) r.v[e] = a.v[e] op b.v[e];           \
+        return r;                                                                          \
+    }
+LX_VOP(+)
+LX_VOP(-)
+LX_VOP(*)
+#undef LX_VOP
+// keep the old value where element e is outside the update range (bit e of m clear)
+template <int V>
+__device__ __forceinline__ vf<V> vsel(unsigned m, const vf<V> &nw, const vf<V> &od) {
+    vf<V> r;
+#pragma unroll
+    for (int e = 0; e < V; ++e) r.v[e] = (m >> e) & 1u ? nw.v[e] : od.v[e];
+    return r;
+}
+// bit e: lo <= i0 + e <= hi
+template <int V>
+__device__ __forceinline__ unsigned vmask(int i0, int lo, int hi) {
+    unsigned m = 0;
+#pragma unroll
+    for (int e = 0; e < V; ++e) m |= (i0 + e >= lo && i0 + e <= hi) ? 1u << e : 0u;
+    return m;
+}
+
+// loads and stores of V consecutive values at a byte offset
+template <typename T, int V>
+__device__ __forceinline__ vf<V> ldv(const T *base, unsigned boff);
+template <>
+__device__ __forceinline__ vf<1> ldv<float, 1>(const float *b, unsigned o) { return vf<1>{{*at(b, o)}}; }
+template <>
+__device__ __forceinline__ vf<2> ldv<float, 2>(const float *b, unsigned o) {
+    const float2 x = *reinterpret_cast<const float2 *>(at(b, o));
+    return vf<2>{{x.x, x.y}};
+}
+template <>
+__device__ __forceinline__ vf<1> ldv<__half, 1>(const __half *b, unsigned o) { return vf<1>{{__half2float(*at(b, o))}}; }
+template <>
+__device__ __forceinline__ vf<2> ldv<__half, 2>(const __half *b, unsigned o) {
+    const float2 x = __half22float2(*reinterpret_cast<const __half2 *>(at(b, o)));
+    return vf<2>{{x.x, x.y}};
+}
+template <typename T, int V>
+__device__ __forceinline__ void stv(T *base, unsigned boff, const vf<V> &v);
+template <>
+__device__ __forceinline__ void stv<float, 1>(float *b, unsigned o, const vf<1> &v) { *at(b, o) = v.v[0]; }
+template <>
+__device__ __forceinline__ void stv<float, 2>(float *b, unsigned o, const vf<2> &v) {
+    *reinterpret_cast<float2 *>(at(b, o)) = make_float2(v.v[0], v.v[1]);
+}
+template <>
+__device__ __forceinline__ void stv<__half, 1>(__half *b, unsigned o, const vf<1> &v) { *at(b, o) = __float2half(v.v[0]); }
+template <>
+__device__ __forceinline__ void stv<__half, 2>(__half *b, unsigned o, const vf<2> &v) {
+    *reinterpret_cast<__half2 *>(at(b, o)) = __floats2half2_rn(v.v[0], v.v[1]);
+}
+// coefficient row of the x axis (float table) at x index i0
+template <int V>
+__device__ __forceinline__ vf<V> ldtab(const float *row, int i0) {
+    return ldv<float, V>(row, (unsigned)i0 * 4u);
+}
+
+// x+1 neighbour of every element; the last element of lane 63 takes `edge`
+template <int V>
+__device__ __forceinline__ vf<V> xnext(const vf<V> &a, float edge, int lane) {
+    vf<V> r;
+#pragma unroll
+    for (int e = 0; e + 1 < V; ++e) r.v[e] = a.v[e + 1];
+    const float n = wave_next(a.v[0]);
+    r.v[V - 1] = lane == 63 ? edge : n;
+    return r;
+}
+// x-1 neighbour of every element; the first element of lane 0 takes `edge`
+template <int V>
+__device__ __forceinline__ vf<V> xprev(const vf<V> &a, float edge, int lane) {
+    vf<V> r;
+    const float p = wave_prev(a.v[V - 1]);
+    r.v[0] = lane == 0 ? edge : p;
+#pragma unroll
+    for (int e = 1; e < V; ++e) r.v[e] = a.v[e - 1];
+    return r;
 }
 
 // Split curl of one component, d1*k1 - d2*k2, with the CPML recursion of
@@ -71,9 +174,20 @@ __device__ __forceinline__ float curl2(float d1, float k1, bool s1, float &p1, f
     if (PML && s2) { p2 = p2 * b2 + d2 * c2; acc = acc - p2; }
     return acc;
 }
+template <bool PML, int V>
+__device__ __forceinline__ vf<V> curl2(const vf<V> &d1, const vf<V> &k1, bool s1, vf<V> &p1, const vf<V> &b1,
+                                       const vf<V> &c1, const vf<V> &d2, const vf<V> &k2, bool s2, vf<V> &p2,
+                                       const vf<V> &b2, const vf<V> &c2) {
+    vf<V> r;
+#pragma unroll
+    for (int e = 0; e < V; ++e)
+        r.v[e] = curl2<PML>(d1.v[e], k1.v[e], s1, p1.v[e], b1.v[e], c1.v[e], d2.v[e], k2.v[e], s2, p2.v[e], b2.v[e],
+                            c2.v[e]);
+    return r;
+}
 
-// The units of one launch: a box of rows [jlo, jhi) x z blocks [tlo, thi)
-// (mode 0), or every unit outside that box (mode 1); all x chunks of each.
+// The units of one launch, all x chunks of each: the rows [jlo, jhi) x the z
+// blocks [tlo, thi) (mode 0), or every unit outside that box (mode 1).
 struct UnitMap {
     int mode, jlo, jhi, tlo, thi, ntz;
 };
@@ -81,9 +195,10 @@ struct UnitMap {
 // PM: CPML axes the launch handles (bit 0: x slab, 1: y, 2: z).  A CPML grid
 // runs two launches: the units whose rows and planes lie outside the y and
 // z slabs (PM = 1, x slab only, far fewer registers) and the rest (PM = 7).
-template <typename T, int PM, int NW>
+template <typename T, int PM, int NW, int V>
 __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nunits) {
     constexpr bool PML = PM != 0;
+    using F = vf<V>;
     __shared__ float cbs[256];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -92,7 +207,7 @@ __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nu
     const int unit = xcd_remap(blockIdx.x, (nunits + NW - 1) / NW) * NW + w;
     if (unit >= nunits) return;                    // whole wavefront; no barrier follows
     const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
-    const int nch = (nx + 63) >> 6;
+    const int nch = (nx + 64 * V - 1) / (64 * V);
     const int ch = unit % nch;
     int rest = unit / nch, jr, tz;
     if (um.mode == 0) {
@@ -113,7 +228,7 @@ __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nu
         }
     }
     const int k0 = tz * g.kz, k1 = min(k0 + g.kz, nz);
-    const int x0 = ch * 64;
+    const int x0 = ch * 64 * V;
     const long long plane = g.plane, fs = g.fstride;
     const int tn = g.tn;
     const float *tab = g.tab;
@@ -131,26 +246,30 @@ __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nu
     const float bey = sy ? tab[(1 * 6 + TK_BE) * tn + jr] : 0.f, cey = sy ? tab[(1 * 6 + TK_CE) * tn + jr] : 0.f;
     const int siy = sy ? slab_index(jr, g.slo[1], g.shi[1]) : 0;
     const int siym = sym ? slab_index(jm, g.slo[1], g.shi[1]) : 0;
+    const F vkhy = vs<V>(khy), vkhym = vs<V>(khym), vkey = vs<V>(key);
+    const F vbhy = vs<V>(bhy), vchy = vs<V>(chy), vbhym = vs<V>(bhym), vchym = vs<V>(chym);
+    const F vbey = vs<V>(bey), vcey = vs<V>(cey);
 
-    // every lane is inside the padded row (x0 + 63 < px); lanes past the grid
-    // read the zero padding and write it back unchanged
-    const int i = x0 + lane;
-    const bool nxt = x0 + 64 < nx;                 // x+1 of lane 63 lies in the next chunk
-    const bool lft = x0 > 0;                       // x-1 of lane 0 lies in the previous chunk
-    const bool m_hx = i < nx, m_in = i < nx - 1;
-    const bool m_ex = jin && i < nx - 1;
-    const bool m_ey = rowx && i >= 1 && i < nx - 1;
-    const bool m_ez = jin && i >= 1 && i < nx - 1;
-    const float khx = tab[(0 * 6 + TK_KH) * tn + i], kex = tab[(0 * 6 + TK_KE) * tn + i];
-    const bool sxw = (PM & 1) && (x0 < g.slo[0] || x0 + 64 > g.shi[0]);   // chunk meets the x slab
-    const bool sx = sxw && (i < g.slo[0] || i >= g.shi[0]);
-    const int xs = sx ? (i < g.slo[0] ? i : g.slo[0] + (i - g.shi[0])) : 0;
-    float bhx = 0.f, chx = 0.f, bex = 0.f, cex = 0.f;
+    // every lane is inside the padded row (x0 + 64V - 1 < px); lanes past the
+    // grid read the zero padding and write it back unchanged
+    const int i0 = x0 + lane * V;                  // the lane's first cell
+    const bool nxt = x0 + 64 * V < nx;             // x+1 of the chunk's last cell lies in the next chunk
+    const bool lft = x0 > 0;                       // x-1 of the chunk's first cell lies in the previous chunk
+    const unsigned m_hx = vmask<V>(i0, 0, nx - 1), m_in = vmask<V>(i0, 0, nx - 2);
+    const unsigned m_ex = jin ? vmask<V>(i0, 0, nx - 2) : 0u;
+    const unsigned m_ey = rowx ? vmask<V>(i0, 1, nx - 2) : 0u;
+    const unsigned m_ez = jin ? vmask<V>(i0, 1, nx - 2) : 0u;
+    const F khx = ldtab<V>(tab + (0 * 6 + TK_KH) * tn, i0), kex = ldtab<V>(tab + (0 * 6 + TK_KE) * tn, i0);
+    // the x slab bounds are multiples of 4, so a lane's V cells are wholly in or out
+    const bool sxw = (PM & 1) && (x0 < g.slo[0] || x0 + 64 * V > g.shi[0]);   // chunk meets the x slab
+    const bool sx = sxw && (i0 < g.slo[0] || i0 >= g.shi[0]);
+    const int xs = sx ? (i0 < g.slo[0] ? i0 : g.slo[0] + (i0 - g.shi[0])) : 0;
+    F bhx = vs<V>(0.f), chx = vs<V>(0.f), bex = vs<V>(0.f), cex = vs<V>(0.f);
     if (sx) {
-        bhx = tab[(0 * 6 + TK_BH) * tn + i];
-        chx = tab[(0 * 6 + TK_CH) * tn + i];
-        bex = tab[(0 * 6 + TK_BE) * tn + i];
-        cex = tab[(0 * 6 + TK_CE) * tn + i];
+        bhx = ldtab<V>(tab + (0 * 6 + TK_BH) * tn, i0);
+        chx = ldtab<V>(tab + (0 * 6 + TK_CH) * tn, i0);
+        bex = ldtab<V>(tab + (0 * 6 + TK_BE) * tn, i0);
+        cex = ldtab<V>(tab + (0 * 6 + TK_CE) * tn, i0);
     }
     // cell x0-1 (wave-uniform): its x coefficients and slab
     const int xl = lft ? x0 - 1 : 0;
@@ -161,59 +280,60 @@ __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nu
     // per-lane byte offsets inside a plane (rows j, j-1, j+1) and inside a psi
     // row (y and z slabs: x index; x slab: slab x index)
     constexpr unsigned S = sizeof(T);
-    const unsigned rj = (unsigned)(jr * px + i) * S, rjm = (unsigned)(jm * px + i) * S;
-    const unsigned rjp = (unsigned)(jp * px + i) * S, ri = (unsigned)i * 4u, rx = (unsigned)xs * 4u;
-    const unsigned re = (unsigned)(jr * px + i);   // material id (one byte per cell)
-    // byte offsets of the edge cells x0+64 (rows j, j-1) and x0-1 (rows j,
+    const unsigned rj = (unsigned)(jr * px + i0) * S, rjm = (unsigned)(jm * px + i0) * S;
+    const unsigned rjp = (unsigned)(jp * px + i0) * S, ri = (unsigned)i0 * 4u, rx = (unsigned)xs * 4u;
+    const unsigned re = (unsigned)(jr * px + i0);  // material ids (one byte per cell)
+    // byte offsets of the edge cells x0+64V (rows j, j-1) and x0-1 (rows j,
     // j+1; psi rows): equal in every lane, kept in vector registers like the
     // others
-    const unsigned bn = rj + (unsigned)(64 - lane) * S, bnm = rjm + (unsigned)(64 - lane) * S;
-    const unsigned bl = rj - (unsigned)(lane + 1) * S, blp = rjp - (unsigned)(lane + 1) * S;
-    const unsigned ril = ri - (unsigned)(lane + 1) * 4u;
+    const unsigned bn = rj + (unsigned)(64 * V - lane * V) * S, bnm = rjm + (unsigned)(64 * V - lane * V) * S;
+    const unsigned bl = rj - (unsigned)(lane * V + 1) * S, blp = rjp - (unsigned)(lane * V + 1) * S;
+    const unsigned ril = ri - (unsigned)(lane * V + 1) * 4u;
     const unsigned rxl = (unsigned)xsl * 4u + (unsigned)(lane - lane);
 
     // z queue at k0 (row j: Ex, Ey, Ez; row j-1: Ey; cell x0-1: Ex) and H_new(j, k0-1)
-    float ex0, ey0, ez0, eym0, exl0 = 0.f;
-    float hxm = 0.f, hym = 0.f;
+    F ex0, ey0, ez0, eym0;
+    float exl0 = 0.f;
+    F hxm = vs<V>(0.f), hym = vs<V>(0.f);
     if (k0 > 0) {
         const int kh = k0 - 1;
         const T *Fk = g.F + kh * plane;
-        const float exh = ldb(Fk, rj), eyh = ldb(Fk + fs, rj), ezh = ldb(Fk + 2 * fs, rj);
-        ex0 = ldb(Fk + plane, rj);
-        ey0 = ldb(Fk + plane + fs, rj);
-        ez0 = ldb(Fk + plane + 2 * fs, rj);
-        eym0 = ldb(Fk + plane + fs, rjm);
+        const F exh = ldv<T, V>(Fk, rj), eyh = ldv<T, V>(Fk + fs, rj), ezh = ldv<T, V>(Fk + 2 * fs, rj);
+        ex0 = ldv<T, V>(Fk + plane, rj);
+        ey0 = ldv<T, V>(Fk + plane + fs, rj);
+        ez0 = ldv<T, V>(Fk + plane + 2 * fs, rj);
+        eym0 = ldv<T, V>(Fk + plane + fs, rjm);
         if (lft) exl0 = ldb(Fk + plane, bl);
-        const float ezph = ldb(Fk + 2 * fs, rjp);
+        const F ezph = ldv<T, V>(Fk + 2 * fs, rjp);
         const float ezNh = nxt ? ldb(Fk + 2 * fs, bn) : 0.f;
-        const float hxh = ldb(Fk + 3 * fs, rj), hyh = ldb(Fk + 4 * fs, rj);
+        const F hxh = ldv<T, V>(Fk + 3 * fs, rj), hyh = ldv<T, V>(Fk + 4 * fs, rj);
         const bool sz = (PM & 4) && (kh < g.slo[2] || kh >= g.shi[2]);
-        float py0 = 0.f, pz1 = 0.f, pz0 = 0.f, px1 = 0.f;
-        if (sy) py0 = ldb(psi_base(g, 1, 0) + ((long long)kh * g.sw[1] + siy) * px, ri);
+        F py0 = vs<V>(0.f), pz1 = vs<V>(0.f), pz0 = vs<V>(0.f), px1 = vs<V>(0.f);
+        if (sy) py0 = ldv<float, V>(psi_base(g, 1, 0) + ((long long)kh * g.sw[1] + siy) * px, ri);
         if (sz) {
             const long long zb = ((long long)slab_index(kh, g.slo[2], g.shi[2]) * ny + jr) * px;
-            pz1 = ldb(psi_base(g, 2, 1) + zb, ri);
-            pz0 = ldb(psi_base(g, 2, 0) + zb, ri);
+            pz1 = ldv<float, V>(psi_base(g, 2, 1) + zb, ri);
+            pz0 = ldv<float, V>(psi_base(g, 2, 0) + zb, ri);
         }
-        if (sx) px1 = ldb(psi_base(g, 0, 1) + ((long long)kh * ny + jr) * g.sw[0], rx);
+        if (sx) px1 = ldv<float, V>(psi_base(g, 0, 1) + ((long long)kh * ny + jr) * g.sw[0], rx);
         const float khz = tab[(2 * 6 + TK_KH) * tn + kh];
         const float bhz = sz ? tab[(2 * 6 + TK_BH) * tn + kh] : 0.f, chz = sz ? tab[(2 * 6 + TK_CH) * tn + kh] : 0.f;
-        float ezi = wave_next(ezh);
-        if (lane == 63) ezi = ezNh;
+        const F vkhz = vs<V>(khz), vbhz = vs<V>(bhz), vchz = vs<V>(chz);
+        const F ezi = xnext<V>(ezh, ezNh, lane);
         hxm = hxh;
         if (rowx) {   // Hx: d1 = dEz/dy, d2 = dEy/dz
-            const float acc = curl2<PML>(ezph - ezh, khy, sy, py0, bhy, chy, ey0 - eyh, khz, sz, pz1, bhz, chz);
-            hxm = m_hx ? hxh - acc : hxh;
+            const F acc = curl2<PML, V>(ezph - ezh, vkhy, sy, py0, vbhy, vchy, ey0 - eyh, vkhz, sz, pz1, vbhz, vchz);
+            hxm = vsel<V>(m_hx, hxh - acc, hxh);
         }
         {             // Hy: d1 = dEx/dz, d2 = dEz/dx
-            const float acc = curl2<PML>(ex0 - exh, khz, sz, pz0, bhz, chz, ezi - ezh, khx, sx, px1, bhx, chx);
-            hym = m_in ? hyh - acc : hyh;
+            const F acc = curl2<PML, V>(ex0 - exh, vkhz, sz, pz0, vbhz, vchz, ezi - ezh, khx, sx, px1, bhx, chx);
+            hym = vsel<V>(m_in, hyh - acc, hyh);
         }
     } else {
-        ex0 = ldb(g.F, rj);
-        ey0 = ldb(g.F + fs, rj);
-        ez0 = ldb(g.F + 2 * fs, rj);
-        eym0 = ldb(g.F + fs, rjm);
+        ex0 = ldv<T, V>(g.F, rj);
+        ey0 = ldv<T, V>(g.F + fs, rj);
+        ez0 = ldv<T, V>(g.F + 2 * fs, rj);
+        eym0 = ldv<T, V>(g.F + fs, rjm);
         if (lft) exl0 = ldb(g.F, bl);
     }
 
@@ -224,18 +344,26 @@ __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nu
         const T *Fk = g.F + kp;
         T *Fnk = g.Fn + kp;
         // ---------------- loads of the plane
-        float ex1 = 0.f, ey1 = 0.f, ez1 = 0.f, eym1 = 0.f, exl1 = 0.f;
+        F ex1 = vs<V>(0.f), ey1 = vs<V>(0.f), ez1 = vs<V>(0.f), eym1 = vs<V>(0.f);
+        float exl1 = 0.f;
         if (kin) {
-            ex1 = ldb(Fk + plane, rj);
-            ey1 = ldb(Fk + plane + fs, rj);
-            ez1 = ldb(Fk + plane + 2 * fs, rj);
-            eym1 = ldb(Fk + plane + fs, rjm);
+            ex1 = ldv<T, V>(Fk + plane, rj);
+            ey1 = ldv<T, V>(Fk + plane + fs, rj);
+            ez1 = ldv<T, V>(Fk + plane + 2 * fs, rj);
+            eym1 = ldv<T, V>(Fk + plane + fs, rjm);
         }
-        const float ezp = ldb(Fk + 2 * fs, rjp), exq = ldb(Fk, rjp);
-        float hx = ldb(Fk + 3 * fs, rj), hy = ldb(Fk + 4 * fs, rj), hz = ldb(Fk + 5 * fs, rj);
-        float hxj = ldb(Fk + 3 * fs, rjm), hzj = ldb(Fk + 5 * fs, rjm);
-        const float ezm = ldb(Fk + 2 * fs, rjm), exm = ldb(Fk, rjm);
-        const unsigned id = g.mat[kp + re];
+        const F ezp = ldv<T, V>(Fk + 2 * fs, rjp), exq = ldv<T, V>(Fk, rjp);
+        F hx = ldv<T, V>(Fk + 3 * fs, rj), hy = ldv<T, V>(Fk + 4 * fs, rj), hz = ldv<T, V>(Fk + 5 * fs, rj);
+        F hxj = ldv<T, V>(Fk + 3 * fs, rjm), hzj = ldv<T, V>(Fk + 5 * fs, rjm);
+        const F ezm = ldv<T, V>(Fk + 2 * fs, rjm), exm = ldv<T, V>(Fk, rjm);
+        unsigned id[V];
+        if constexpr (V == 1) {
+            id[0] = g.mat[kp + re];
+        } else {
+            const uchar2 u = *reinterpret_cast<const uchar2 *>(g.mat + kp + re);
+            id[0] = u.x;
+            id[1] = u.y;
+        }
         float ezN = 0.f, eyN = 0.f, eymN = 0.f;
         if (nxt) {
             ezN = ldb(Fk + 2 * fs, bn);
@@ -255,9 +383,9 @@ __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nu
         // psi: H of row j (p*), H of row j-1 (q*), E of row j (e*), H of cell
         // x0-1 (l*); row bases of the read (P*) and written (Q*) arrays, slot s
         // of axis a lying s*scells[a] further on
-        float py0 = 0.f, py1 = 0.f, pz0 = 0.f, pz1 = 0.f, px0 = 0.f, px1 = 0.f;
-        float qy0 = 0.f, qy1 = 0.f, qz1 = 0.f, qx0 = 0.f;
-        float ey2 = 0.f, ey3 = 0.f, ez2 = 0.f, ez3 = 0.f, ex2 = 0.f, ex3 = 0.f;
+        F py0 = vs<V>(0.f), py1 = vs<V>(0.f), pz0 = vs<V>(0.f), pz1 = vs<V>(0.f), px0 = vs<V>(0.f), px1 = vs<V>(0.f);
+        F qy0 = vs<V>(0.f), qy1 = vs<V>(0.f), qz1 = vs<V>(0.f), qx0 = vs<V>(0.f);
+        F ey2 = vs<V>(0.f), ey3 = vs<V>(0.f), ez2 = vs<V>(0.f), ez3 = vs<V>(0.f), ex2 = vs<V>(0.f), ex3 = vs<V>(0.f);
         float ly1 = 0.f, lz0 = 0.f, lx0 = 0.f, lx1 = 0.f;
         const float *PY = nullptr, *PZ = nullptr, *PX = nullptr;
         float *QY = nullptr, *QZ = nullptr, *QX = nullptr;
@@ -266,27 +394,27 @@ __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nu
             const long long yb = uo(((long long)k * g.sw[1] + siy) * px);
             PY = psi_base(g, 1, 0) + yb;
             QY = psi_new(g, 1, 0) + yb;
-            py0 = ldb(PY, ri);
-            py1 = ldb(PY + cy, ri);
-            ey2 = ldb(PY + 2 * cy, ri);
-            ey3 = ldb(PY + 3 * cy, ri);
+            py0 = ldv<float, V>(PY, ri);
+            py1 = ldv<float, V>(PY + cy, ri);
+            ey2 = ldv<float, V>(PY + 2 * cy, ri);
+            ey3 = ldv<float, V>(PY + 3 * cy, ri);
             if (lft) ly1 = ldb(PY + cy, ril);
         }
         if (sym) {
             const float *PYm = psi_base(g, 1, 0) + uo(((long long)k * g.sw[1] + siym) * px);
-            qy0 = ldb(PYm, ri);
-            qy1 = ldb(PYm + cy, ri);
+            qy0 = ldv<float, V>(PYm, ri);
+            qy1 = ldv<float, V>(PYm + cy, ri);
         }
         if (sz) {
             const long long zs = (long long)slab_index(k, g.slo[2], g.shi[2]) * ny;
             const long long zb = uo((zs + jr) * px);
             PZ = psi_base(g, 2, 0) + zb;
             QZ = psi_new(g, 2, 0) + zb;
-            pz0 = ldb(PZ, ri);
-            pz1 = ldb(PZ + cz, ri);
-            ez2 = ldb(PZ + 2 * cz, ri);
-            ez3 = ldb(PZ + 3 * cz, ri);
-            qz1 = ldb(psi_base(g, 2, 1) + uo((zs + jm) * px), ri);
+            pz0 = ldv<float, V>(PZ, ri);
+            pz1 = ldv<float, V>(PZ + cz, ri);
+            ez2 = ldv<float, V>(PZ + 2 * cz, ri);
+            ez3 = ldv<float, V>(PZ + 3 * cz, ri);
+            qz1 = ldv<float, V>(psi_base(g, 2, 1) + uo((zs + jm) * px), ri);
             if (lft) lz0 = ldb(PZ, ril);
         }
         if (sxw || sxl) {
@@ -294,11 +422,11 @@ __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nu
             PX = psi_base(g, 0, 0) + xb;
             QX = psi_new(g, 0, 0) + xb;
             if (sx) {
-                px0 = ldb(PX, rx);
-                px1 = ldb(PX + cx, rx);
-                ex2 = ldb(PX + 2 * cx, rx);
-                ex3 = ldb(PX + 3 * cx, rx);
-                qx0 = ldb(psi_base(g, 0, 0) + uo(((long long)k * ny + jm) * g.sw[0]), rx);
+                px0 = ldv<float, V>(PX, rx);
+                px1 = ldv<float, V>(PX + cx, rx);
+                ex2 = ldv<float, V>(PX + 2 * cx, rx);
+                ex3 = ldv<float, V>(PX + 3 * cx, rx);
+                qx0 = ldv<float, V>(psi_base(g, 0, 0) + uo(((long long)k * ny + jm) * g.sw[0]), rx);
             }
             if (sxl) {
                 lx0 = ldb(PX, rxl);
@@ -309,92 +437,91 @@ __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nu
         const float bhz = sz ? tab[(2 * 6 + TK_BH) * tn + k] : 0.f, chz = sz ? tab[(2 * 6 + TK_CH) * tn + k] : 0.f;
         const float kez = tab[(2 * 6 + TK_KE) * tn + k];
         const float bez = sz ? tab[(2 * 6 + TK_BE) * tn + k] : 0.f, cez = sz ? tab[(2 * 6 + TK_CE) * tn + k] : 0.f;
-        float ezi = wave_next(ez0), eyi = wave_next(ey0), eymi = wave_next(eym0);
-        if (lane == 63) {
-            ezi = ezN;
-            eyi = eyN;
-            eymi = eymN;
-        }
+        const F vkhz = vs<V>(khz), vbhz = vs<V>(bhz), vchz = vs<V>(chz);
+        const F vkez = vs<V>(kez), vbez = vs<V>(bez), vcez = vs<V>(cez);
+        const F ezi = xnext<V>(ez0, ezN, lane), eyi = xnext<V>(ey0, eyN, lane), eymi = xnext<V>(eym0, eymN, lane);
 
         // ---------------- H half step, row j
         if (rowx && kin) {   // Hx: d1 = dEz/dy, d2 = dEy/dz
-            const float acc = curl2<PML>(ezp - ez0, khy, sy, py0, bhy, chy, ey1 - ey0, khz, sz, pz1, bhz, chz);
-            hx = m_hx ? hx - acc : hx;
+            const F acc = curl2<PML, V>(ezp - ez0, vkhy, sy, py0, vbhy, vchy, ey1 - ey0, vkhz, sz, pz1, vbhz, vchz);
+            hx = vsel<V>(m_hx, hx - acc, hx);
         }
         if (kin) {           // Hy: d1 = dEx/dz, d2 = dEz/dx
-            const float acc = curl2<PML>(ex1 - ex0, khz, sz, pz0, bhz, chz, ezi - ez0, khx, sx, px1, bhx, chx);
-            hy = m_in ? hy - acc : hy;
+            const F acc = curl2<PML, V>(ex1 - ex0, vkhz, sz, pz0, vbhz, vchz, ezi - ez0, khx, sx, px1, bhx, chx);
+            hy = vsel<V>(m_in, hy - acc, hy);
         }
         if (rowx) {          // Hz: d1 = dEy/dx, d2 = dEx/dy
-            const float acc = curl2<PML>(eyi - ey0, khx, sx, px0, bhx, chx, exq - ex0, khy, sy, py1, bhy, chy);
-            hz = m_in ? hz - acc : hz;
+            const F acc = curl2<PML, V>(eyi - ey0, khx, sx, px0, bhx, chx, exq - ex0, vkhy, sy, py1, vbhy, vchy);
+            hz = vsel<V>(m_in, hz - acc, hz);
         }
         // ---------------- Hx, Hz of row j-1 (only Ex, Ez of row j >= 1 use them)
         if (has_m) {
             if (kin) {       // Hx(j-1): d1 = Ez(j) - Ez(j-1), d2 = dEy(j-1)/dz
-                const float acc = curl2<PML>(ez0 - ezm, khym, sym, qy0, bhym, chym, eym1 - eym0, khz, sz, qz1, bhz, chz);
-                hxj = m_hx ? hxj - acc : hxj;
+                const F acc =
+                    curl2<PML, V>(ez0 - ezm, vkhym, sym, qy0, vbhym, vchym, eym1 - eym0, vkhz, sz, qz1, vbhz, vchz);
+                hxj = vsel<V>(m_hx, hxj - acc, hxj);
             }
             {                // Hz(j-1): d1 = dEy(j-1)/dx, d2 = Ex(j) - Ex(j-1)
-                const float acc = curl2<PML>(eymi - eym0, khx, sx, qx0, bhx, chx, ex0 - exm, khym, sym, qy1, bhym, chym);
-                hzj = m_in ? hzj - acc : hzj;
+                const F acc =
+                    curl2<PML, V>(eymi - eym0, khx, sx, qx0, bhx, chx, ex0 - exm, vkhym, sym, qy1, vbhym, vchym);
+                hzj = vsel<V>(m_in, hzj - acc, hzj);
             }
         }
-        // ---------------- Hy, Hz of cell x0-1 (only lane 0's Ey, Ez use them)
+        // ---------------- Hy, Hz of cell x0-1 (only the chunk's first Ey, Ez use them)
         if (lft) {
             if (kin) {       // Hy(x0-1): d1 = dEx/dz, d2 = Ez(x0) - Ez(x0-1)
-                const float acc = curl2<PML>(exl1 - exl0, khz, sz, lz0, bhz, chz, lane0(ez0) - ezl, khxl, sxl, lx1, bhxl, chxl);
+                const float acc = curl2<PML>(exl1 - exl0, khz, sz, lz0, bhz, chz, lane0(ez0.v[0]) - ezl, khxl, sxl, lx1,
+                                             bhxl, chxl);
                 hyl = hyl - acc;
             }
             if (rowx) {      // Hz(x0-1): d1 = Ey(x0) - Ey(x0-1), d2 = dEx/dy
-                const float acc = curl2<PML>(lane0(ey0) - eyl, khxl, sxl, lx0, bhxl, chxl, exlp - exl0, khy, sy, ly1, bhy, chy);
+                const float acc = curl2<PML>(lane0(ey0.v[0]) - eyl, khxl, sxl, lx0, bhxl, chxl, exlp - exl0, khy, sy, ly1,
+                                             bhy, chy);
                 hzl = hzl - acc;
             }
         }
-        stb(Fnk + 3 * fs, rj, hx);
-        stb(Fnk + 4 * fs, rj, hy);
-        stb(Fnk + 5 * fs, rj, hz);
+        stv<T, V>(Fnk + 3 * fs, rj, hx);
+        stv<T, V>(Fnk + 4 * fs, rj, hy);
+        stv<T, V>(Fnk + 5 * fs, rj, hz);
         if (PML) {
-            if (sy && rowx && kin) stb(QY, ri, py0);
-            if (sz && rowx && kin) stb(QZ + cz, ri, pz1);
-            if (sz && kin) stb(QZ, ri, pz0);
-            if (sx && kin) stb(QX + cx, rx, px1);
-            if (sx && rowx) stb(QX, rx, px0);
-            if (sy && rowx) stb(QY + cy, ri, py1);
+            if (sy && rowx && kin) stv<float, V>(QY, ri, py0);
+            if (sz && rowx && kin) stv<float, V>(QZ + cz, ri, pz1);
+            if (sz && kin) stv<float, V>(QZ, ri, pz0);
+            if (sx && kin) stv<float, V>(QX + cx, rx, px1);
+            if (sx && rowx) stv<float, V>(QX, rx, px0);
+            if (sy && rowx) stv<float, V>(QY + cy, ri, py1);
         }
 
         // ---------------- E half step, row j
-        float hyp = wave_prev(hy), hzp = wave_prev(hz);
-        if (lane == 0) {
-            hyp = hyl;
-            hzp = hzl;
-        }
-        const float cb = cbs[id];
-        float ex = ex0, ey = ey0, ez = ez0;
+        const F hyp = xprev<V>(hy, hyl, lane), hzp = xprev<V>(hz, hzl, lane);
+        F cb;
+#pragma unroll
+        for (int e = 0; e < V; ++e) cb.v[e] = cbs[id[e]];
+        F ex = ex0, ey = ey0, ez = ez0;
         if (kine) {
             {                // Ex: d1 = dHz/dy, d2 = dHy/dz
-                const float acc = curl2<PML>(hz - hzj, key, sy, ey2, bey, cey, hy - hym, kez, sz, ez3, bez, cez);
-                ex = m_ex ? ex0 + cb * acc : ex0;
+                const F acc = curl2<PML, V>(hz - hzj, vkey, sy, ey2, vbey, vcey, hy - hym, vkez, sz, ez3, vbez, vcez);
+                ex = vsel<V>(m_ex, ex0 + cb * acc, ex0);
             }
             {                // Ey: d1 = dHx/dz, d2 = dHz/dx
-                const float acc = curl2<PML>(hx - hxm, kez, sz, ez2, bez, cez, hz - hzp, kex, sx, ex3, bex, cex);
-                ey = m_ey ? ey0 + cb * acc : ey0;
+                const F acc = curl2<PML, V>(hx - hxm, vkez, sz, ez2, vbez, vcez, hz - hzp, kex, sx, ex3, bex, cex);
+                ey = vsel<V>(m_ey, ey0 + cb * acc, ey0);
             }
         }
         if (kin) {           // Ez: d1 = dHy/dx, d2 = dHx/dy
-            const float acc = curl2<PML>(hy - hyp, kex, sx, ex2, bex, cex, hx - hxj, key, sy, ey3, bey, cey);
-            ez = m_ez ? ez0 + cb * acc : ez0;
+            const F acc = curl2<PML, V>(hy - hyp, kex, sx, ex2, bex, cex, hx - hxj, vkey, sy, ey3, vbey, vcey);
+            ez = vsel<V>(m_ez, ez0 + cb * acc, ez0);
         }
-        stb(Fnk, rj, ex);
-        stb(Fnk + fs, rj, ey);
-        stb(Fnk + 2 * fs, rj, ez);
+        stv<T, V>(Fnk, rj, ex);
+        stv<T, V>(Fnk + fs, rj, ey);
+        stv<T, V>(Fnk + 2 * fs, rj, ez);
         if (PML) {
-            if (sy && kine && jin) stb(QY + 2 * cy, ri, ey2);
-            if (sz && kine && jin) stb(QZ + 3 * cz, ri, ez3);
-            if (sz && kine && rowx) stb(QZ + 2 * cz, ri, ez2);
-            if (sx && kine && rowx) stb(QX + 3 * cx, rx, ex3);
-            if (sx && kin && jin) stb(QX + 2 * cx, rx, ex2);
-            if (sy && kin && jin) stb(QY + 3 * cy, ri, ey3);
+            if (sy && kine && jin) stv<float, V>(QY + 2 * cy, ri, ey2);
+            if (sz && kine && jin) stv<float, V>(QZ + 3 * cz, ri, ez3);
+            if (sz && kine && rowx) stv<float, V>(QZ + 2 * cz, ri, ez2);
+            if (sx && kine && rowx) stv<float, V>(QX + 3 * cx, rx, ex3);
+            if (sx && kin && jin) stv<float, V>(QX + 2 * cx, rx, ex2);
+            if (sy && kin && jin) stv<float, V>(QY + 3 * cy, ri, ey3);
         }
         hxm = hx;
         hym = hy;
@@ -406,13 +533,13 @@ __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nu
     }
 }
 
-template <typename T, int PM>
+template <typename T, int PM, int V>
 static void launch_units(const Grid<T> &g, int nw, const UnitMap &um, int nunits, hipStream_t s) {
     if (nunits <= 0) return;
     if (nw == 8)
-        hipLaunchKernelGGL((k_step3<T, PM, 8>), dim3((nunits + 7) / 8), dim3(512), 0, s, g, um, nunits);
+        hipLaunchKernelGGL((k_step3<T, PM, 8, V>), dim3((nunits + 7) / 8), dim3(512), 0, s, g, um, nunits);
     else
-        hipLaunchKernelGGL((k_step3<T, PM, 4>), dim3((nunits + 3) / 4), dim3(256), 0, s, g, um, nunits);
+        hipLaunchKernelGGL((k_step3<T, PM, 4, V>), dim3((nunits + 3) / 4), dim3(256), 0, s, g, um, nunits);
 }
 
 // Rows [jlo, jhi) and z blocks [tlo, thi) of the interior box: rows j with j
@@ -434,25 +561,31 @@ void step3_core_extent(int ny, int nz, int kz, const int slo[3], const int shi[3
     *planes = std::min(um.thi * kz, nz) - um.tlo * kz;
 }
 
-template <typename T>
-void launch_step3(const Grid<T> &g, int nw, int part, hipStream_t s) {
+template <typename T, int V>
+static void launch_parts(const Grid<T> &g, int nw, int part, hipStream_t s) {
     const int ntz = (g.nz + g.kz - 1) / g.kz;
-    const int nch = (g.nx + 63) / 64;              // one wavefront per (x chunk, row, z block)
+    const int nch = (g.nx + 64 * V - 1) / (64 * V);   // one wavefront per (x chunk, row, z block)
     if (!g.pml) {
-        if (part == 0) launch_units<T, 0>(g, nw, UnitMap{0, 0, g.ny, 0, ntz, ntz}, nch * g.ny * ntz, s);
+        if (part == 0) launch_units<T, 0, V>(g, nw, UnitMap{0, 0, g.ny, 0, ntz, ntz}, nch * g.ny * ntz, s);
         return;
     }
     UnitMap um = core_box(g.ny, g.nz, g.kz, g.slo, g.shi);
     const int nbox = nch * (um.jhi - um.jlo) * (um.thi - um.tlo);
     if (part == 0) {
-        launch_units<T, 1>(g, nw, um, nbox, s);
+        launch_units<T, 1, V>(g, nw, um, nbox, s);
     } else {
         um.mode = 1;
-        launch_units<T, 7>(g, nw, um, nch * g.ny * ntz - nbox, s);
+        launch_units<T, 7, V>(g, nw, um, nch * g.ny * ntz - nbox, s);
     }
 }
 
-template void launch_step3<float>(const Grid<float> &, int, int, hipStream_t);
-template void launch_step3<__half>(const Grid<__half> &, int, int, hipStream_t);
+template <typename T>
+void launch_step3(const Grid<T> &g, int nw, int vec, int part, hipStream_t s) {
+    if (vec == 2) launch_parts<T, 2>(g, nw, part, s);
+    else launch_parts<T, 1>(g, nw, part, s);
+}
+
+template void launch_step3<float>(const Grid<float> &, int, int, int, hipStream_t);
+template void launch_step3<__half>(const Grid<__half> &, int, int, int, hipStream_t);
 
 }  // namespace lx

@@ -215,3 +215,87 @@ def test_fused_step_matches_half_steps(gpu_ctx, monkeypatch):
     e = (num / den) ** 0.5
     say(f"fused vs half-step (plain, oblique analytic): fields {e:.2e}")
     assert e <= 1e-6
+
+
+def _slab_box(nx, ny, nz, pml, margin, precision=0):
+    """CPML grid of uneven extents, aux-1D plane wave along +x, a dielectric
+    block off the centre and 4 Ez probes."""
+    p = default_params(is_planar=0, nx=nx, ny=ny, nz=nz, dx=configs.DX, dy=configs.DX, dz=configs.DX,
+                       courant=0.5, precision=precision)
+    configs._cpml(p, pml)
+    p.use_pw = 1
+    p.incident = LX_INC_AUX1D
+    p.k_pw[:] = (1.0, 0.0, 0.0)
+    p.e_pw[:] = (0.0, 0.0, 1.0)
+    lo = pml + margin
+    p.tf_lo[:] = (lo, lo, lo)
+    p.tf_hi[:] = (nx - 1 - lo, ny - 1 - lo, nz - 1 - lo)
+    p.pw_lambda0 = configs.LAMBDA0
+    p.pw_fwhm = 3.0
+    p.pw_delay = 4.0
+    ids = np.zeros((nx, ny, nz), dtype=np.uint8)
+    ids[nx // 3:nx // 2 + 7, ny // 3:ny // 2 + 3, nz // 3:nz // 2 + 5] = 1
+    eps = np.array([1.0, 4.0])
+    probes = np.array([[LX_EZ, x, ny // 2, z] for x in (nx // 4, 3 * nx // 4) for z in (nz // 3, 2 * nz // 3)],
+                      dtype=np.int32)
+    return p, ids, eps, probes
+
+
+def test_fused_two_cells_per_lane(gpu_ctx, monkeypatch):
+    """The fused step with two x cells per lane (128-cell chunks) against
+    the half-step kernels (1e-6) and the FP64 oracle (1e-5): two chunks per
+    row with a partial second one (nx = 200, padded pitch 256), and one
+    partial chunk on a plain grid (nx = 100); FP16 storage against the
+    one-cell-per-lane fused step."""
+    from luxum_amd.sim import Simulation
+
+    def run(p, steps, w, mode):
+        monkeypatch.setenv("LX_FUSED", mode)
+        s = Simulation(p, gpu_ctx)
+        if w is not None:
+            s.set_materials(w[0], w[1])
+            s.set_probes(w[2])
+        inf = s.info()
+        assert bool(inf.fused_step) == (mode != "0")
+        assert inf.cells_per_lane == (2 if mode == "2" else 1)
+        s.step(steps)
+        s.sync()
+        return s
+
+    def rel(a, b, comps):
+        num = den = 0.0
+        for c in comps:
+            x, y = a.field(c), b.field(c)
+            assert np.all(np.isfinite(x))
+            num += float(np.sum((x - y) ** 2))
+            den += float(np.sum(y ** 2))
+        return (num / den) ** 0.5
+
+    p, ids, eps, probes = _slab_box(200, 40, 44, 8, 5)
+    say(f"case vec2 cpml aux nx={p.nx} ny={p.ny} nz={p.nz}")
+    a, b = run(p, 161, (ids, eps, probes), "2"), run(p, 161, (ids, eps, probes), "0")
+    e, pe = rel(a, b, p.components()), rel_l2(a.probes(), b.probes())
+    say(f"vec2 vs half-step (cpml aux, 161 steps): fields {e:.2e} probes {pe:.2e}")
+    assert e <= 1e-6 and pe <= 1e-6
+    o = OracleSim(p)
+    o.set_materials(ids, eps)
+    o.set_probes(probes)
+    o.step(161)
+    check_fields(a, o, TOL32, "vec2 vs oracle")
+
+    q = default_params(nx=100, ny=36, nz=44, use_pw=1, incident=LX_INC_ANALYTIC, off_pw=90.0,
+                       pw_sigma=(20 * 5e-8) ** 2, tf_lo=(8, 7, 9), tf_hi=(90, 27, 33), courant=0.5)
+    q.k_pw[:] = (0.6, 0.48, 0.64)
+    q.e_pw[:] = (0.6246950475544243, -0.7808688094430304, 0.0)
+    say(f"case vec2 plain oblique nx={q.nx} ny={q.ny} nz={q.nz}")
+    a, b = run(q, 120, None, "2"), run(q, 120, None, "0")
+    e = rel(a, b, q.components())
+    say(f"vec2 vs half-step (plain, oblique analytic): fields {e:.2e}")
+    assert e <= 1e-6
+
+    p, ids, eps, probes = _slab_box(120, 36, 40, 8, 5, precision=LX_FP16)
+    say(f"case vec2 fp16 cpml aux nx={p.nx} ny={p.ny} nz={p.nz}")
+    a, b = run(p, 150, (ids, eps, probes), "2"), run(p, 150, (ids, eps, probes), "1")
+    e = rel(a, b, p.components())
+    say(f"fp16 vec2 vs vec1 fused (cpml aux, 150 steps): fields {e:.2e}")
+    assert e <= 1e-6
