@@ -282,8 +282,9 @@ typedef struct lx_server_stats {
 } lx_server_stats;
 
 /* Upper bound of the device bytes a simulation with these parameters uses
- * (batched: including the second field/psi buffers of a tiled batch member),
- * with `steps` probe records of `n_probes` probes. */
+ * (3D: including the second field/psi buffers of the fused step; batched:
+ * including those of a tiled 2D batch member), with `steps` probe records
+ * of `n_probes` probes. */
 lx_status lx_params_footprint(const lx_params *p, int32_t batched, int32_t steps, int32_t n_probes,
                               int64_t *bytes);
 lx_status lx_server_create(lx_ctx *ctx, int32_t mode, int64_t hbm_cap_bytes, int32_t max_active,

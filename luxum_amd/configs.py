@@ -103,6 +103,33 @@ def config1(n: int = 256, steps: int = 2000, precision: int = LX_FP32, pml: int 
     return Workload("3D CPML cube %d^3" % n, p, ids, eps, probes)
 
 
+def box3d(nx: int, ny: int, nz: int, pml: int = 8, margin: int = 5, precision: int = LX_FP32,
+          steps: int = 400) -> Workload:
+    """3D CPML grid of uneven extents: config 1's excitation (aux-1D plane
+    wave along +x, Ez polarised, TF/SF `margin` cells inside the CPML), a
+    dielectric block (eps_r 4) off the centre and 4 Ez probes."""
+    p = default_params(is_planar=0, nx=nx, ny=ny, nz=nz, dx=DX, dy=DX, dz=DX, courant=0.5, num_timesteps=steps,
+                       precision=precision)
+    _cpml(p, pml)
+    p.use_pw = 1
+    p.incident = LX_INC_AUX1D
+    p.k_pw[:] = (1.0, 0.0, 0.0)
+    p.e_pw[:] = (0.0, 0.0, 1.0)
+    lo = pml + margin
+    p.tf_lo[:] = (lo, lo, lo)
+    p.tf_hi[:] = (nx - 1 - lo, ny - 1 - lo, nz - 1 - lo)
+    p.pw_wave = LX_WAVE_GAUSSIAN_SINE
+    p.pw_lambda0 = LAMBDA0
+    p.pw_fwhm = 3.0
+    p.pw_delay = 4.0
+    ids = np.zeros((nx, ny, nz), dtype=np.uint8)
+    ids[nx // 3:nx // 2 + 7, ny // 3:ny // 2 + 3, nz // 3:nz // 2 + 5] = 1
+    eps = np.array([1.0, 4.0])
+    probes = np.array([[LX_EZ, x, ny // 2, z] for x in (nx // 4, 3 * nx // 4) for z in (nz // 3, 2 * nz // 3)],
+                      dtype=np.int32)
+    return Workload("3D CPML box %dx%dx%d" % (nx, ny, nz), p, ids, eps, probes)
+
+
 def config2(nxy: int = 1280, nz: int = 640, steps: int = 4000, precision: int = LX_FP32, pitch: int = 32,
             side: int = 13, slab: int = 64) -> Workload:
     """3D photonic-crystal slab waveguide: square lattice of square rods

@@ -816,15 +816,28 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
     if ((st = build_tfsf(s))) return bail(st);
     if ((st = build_incident(s))) return bail(st);
     if (!s->planar) {
-        // 3D: the half-step pair by default; LX_FUSED=1 selects the fused step
-        // where it applies, LX_FUSED=2 the fused step with two cells per lane
-        // where the row pitch allows it (lx_sim_autotune times them all and
-        // keeps the fastest)
+        // 3D: the fused step where it applies and its second buffer set fits
+        // (faster than the half-step pair on every grid measured), else the
+        // half-step pair.  LX_FUSED=0 selects the half-step pair, 1 the fused
+        // step with one cell per lane, 2 with two where the row pitch allows
+        // it; unset: two cells per lane for FP16 storage, one for FP32.
+        // lx_sim_autotune times them all and keeps the fastest.
         const char *fz = std::getenv("LX_FUSED");
-        if (fz && *fz && *fz != '0' && fused_ok(s)) {
-            if ((st = ensure_fused_buffers(s))) return bail(st);
-            s->fused = 1;
-            if (*fz == '2' && (s->N[0] + 127) / 128 * 128 <= s->px) s->vec = 2;
+        const char f = fz && *fz ? *fz : 'd';
+        if (f != '0' && fused_ok(s)) {
+            st = ensure_fused_buffers(s);
+            if (st == LX_OK) {
+                s->fused = 1;
+                s->kz = 8;
+                s->nw = 4;
+                const bool v2 = (s->N[0] + 127) / 128 * 128 <= s->px;
+                s->vec = v2 && (f == '2' || (f == 'd' && s->fp16)) ? 2 : 1;
+            } else if (st == LX_ENOMEM && f == 'd') {
+                (void)hipGetLastError();
+                release_fused_buffers(s);
+            } else {
+                return bail(st);
+            }
         }
     }
     if (p->use_ps) {
@@ -1311,6 +1324,7 @@ extern "C" lx_status lx_params_footprint(const lx_params *p, int32_t batched, in
     const int64_t probes = (int64_t)n_probes * std::max(steps, p->num_timesteps) * 8;
     int64_t total = fields + ncell + psi + tables + line + probes + (2 << 20);   // 2 MiB of allocation slack
     if (batched && p->is_planar) total += fields + psi + 2 * 8 * 8 * (std::max(N[0], N[2]) + 8);
+    if (!p->is_planar) total += fields + psi;   // second buffer set of the fused step
     *bytes = total;
     return LX_OK;
 }

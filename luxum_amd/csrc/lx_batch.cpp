@@ -6,8 +6,13 @@
 // launches of k_tile2d over the flattened tile list of every member (jobs
 // whose region touches a CPML slab, then the rest), one launch of
 // k_batch_finish.
-// Pairs of full chunks are captured once as a hipGraph and replayed.  Other
-// members step on their own streams with the single-simulation kernels.
+// Pairs of full chunks are captured once as a hipGraph and replayed.
+// 3D members that the fused step supports (no point source, no periodic
+// monitor) advance together too: per step, the H TF/SF launch, the fused
+// step over the interior boxes and over the CPML shells, and the E TF/SF
+// launch, each over a flattened block list of every member; one sequence-end
+// launch per chunk.  Full chunks are captured as hipGraphs.  Other members
+// step on their own streams with the single-simulation kernels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,6 +47,20 @@ struct Group {
     int stable = 0;                           // stepped at least once since the last rebuild
 };
 
+// 3D members of one storage precision.
+struct Group3 {
+    int fp16 = 0;
+    int vec = 1;                              // x cells per lane of the fused step
+    std::vector<lx_sim *> sims;
+    void *d_sims = nullptr;                   // B3Sim<T>[sims]
+    int2 *d_map = nullptr;                    // block maps: interior, shell, H TF/SF, E TF/SF
+    int nblk[4] = {0, 0, 0, 0};
+    int off[4] = {0, 0, 0, 0};                // first entry of each map in d_map
+    int par = 0;                              // buffer set holding the current state
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};   // one chunk starting at parity 0 / 1
+    int stable = 0;
+};
+
 }  // namespace
 
 struct lx_batch {
@@ -50,13 +69,14 @@ struct lx_batch {
     hipStream_t stream = nullptr, side = nullptr;   // general-path jobs run on `side`, concurrently
     hipEvent_t done = nullptr, pre = nullptr, fork = nullptr, join = nullptr;
     bool dirty = true;
-    Group grp[2];                   // FP32 and FP16 storage
+    Group grp[2];                   // planar members, FP32 and FP16 storage
+    Group3 grp3[2];                 // 3D members, FP32 and FP16 storage
     std::vector<lx_sim *> others;
     std::vector<void *> allocs;     // snapshot buffers
-    // descriptor arrays per group (sims, jobs, aux lines), kept across
-    // rebuilds and grown when a membership needs more
-    void *desc[2][3] = {};
-    size_t desc_cap[2][3] = {};
+    // descriptor arrays per precision (planar sims, jobs, aux lines; 3D sims,
+    // block maps), kept across rebuilds and grown when a membership needs more
+    void *desc[2][5] = {};
+    size_t desc_cap[2][5] = {};
     int graphs = 1;
 };
 
@@ -91,6 +111,23 @@ void free_group(Group &g) {
     g.d_jobs = nullptr;
     g.d_aux = nullptr;
     g.nouter = g.ninner = g.naux = 0;
+}
+
+void free_group3(Group3 &g) {
+    for (hipGraphExec_t &x : g.gexec) {
+        if (x) (void)hipGraphExecDestroy(x);
+        x = nullptr;
+    }
+    g.par = 0;
+    g.stable = 0;
+    g.sims.clear();
+    g.d_sims = nullptr;
+    g.d_map = nullptr;
+    for (int k = 0; k < 4; ++k) g.nblk[k] = g.off[k] = 0;
+}
+
+bool batch3_ok(const lx_sim *s) {
+    return !s->planar && fused_ok(s) && !s->p.use_ps && s->p.monitor_every <= 0;
 }
 
 void free_allocs(lx_batch *b) {
@@ -136,16 +173,97 @@ lx_status ensure_alt(lx_sim *s) {
     return LX_OK;
 }
 
+// Descriptors and block maps of the 3D members (large members first: their
+// long-running blocks start early, small ones fill the tail).
+template <typename T>
+lx_status build_group3(lx_batch *b, Group3 &g) {
+    const size_t n = g.sims.size();
+    g.vec = 1;
+    if (g.fp16) {
+        g.vec = 2;
+        for (const lx_sim *s : g.sims)
+            if ((s->N[0] + 127) / 128 * 128 > s->px) g.vec = 1;
+    }
+    std::vector<int> order(n);
+    for (size_t q = 0; q < n; ++q) order[q] = (int)q;
+    auto cells = [&](int q) { return (int64_t)g.sims[q]->N[0] * g.sims[q]->N[1] * g.sims[q]->N[2]; };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cells(x) > cells(y); });
+    std::vector<B3Sim<T>> hs(n);
+    std::vector<int2> map[4];
+    for (int q : order) {
+        lx_sim *s = g.sims[q];
+        B3Sim<T> &d = hs[q];
+        std::memset(&d, 0, sizeof(d));
+        d.g = make_grid<T>(s);
+        d.g.F = (T *)s->F;
+        d.g.Fn = (T *)s->F_alt;
+        d.g.psi = s->psi;
+        d.g.psin = s->psi_alt;
+        if (!s->fused) d.g.kz = 8;   // the half-step kernels' z-block depth does not apply
+        step3_unit_maps(s->N[0], s->N[1], s->N[2], d.g.kz, s->pml, s->slo, s->shi, g.vec, d.um, d.nunits);
+        d.tfH = s->tfH.dev;
+        d.tfE = s->tfE.dev;
+        d.ntfH = s->tfH.n;
+        d.ntfE = s->tfE.n;
+        d.totH = s->tfH.total;
+        d.totE = s->tfE.total;
+        d.inc = s->inc;
+        d.aux = s->aux;
+        d.pr = probe_args(s);
+        d.use_aux = s->use_aux;
+        d.probes = s->np > 0;
+        const int extra = (d.use_aux || d.probes) ? 1 : 0;
+        const int nb[4] = {(d.nunits[0] + kB3Waves - 1) / kB3Waves, (d.nunits[1] + kB3Waves - 1) / kB3Waves,
+                           (d.totH + 255) / 256 + extra, (d.totE + 255) / 256 + extra};
+        for (int k = 0; k < 4; ++k)
+            for (int j = 0; j < nb[k]; ++j) map[k].push_back(make_int2(q, j));
+    }
+    size_t total = 0;
+    for (int k = 0; k < 4; ++k) {
+        g.off[k] = (int)total;
+        g.nblk[k] = (int)map[k].size();
+        total += map[k].size();
+    }
+    const int gi = g.fp16;
+    void *p = nullptr;
+    lx_status st = desc_buf(b, gi, 3, sizeof(B3Sim<T>) * n, &p);
+    if (st) return st;
+    g.d_sims = p;
+    HIP_TRY(hipMemcpy(p, hs.data(), sizeof(B3Sim<T>) * n, hipMemcpyHostToDevice));
+    st = desc_buf(b, gi, 4, sizeof(int2) * std::max<size_t>(total, 1), &p);
+    if (st) return st;
+    g.d_map = (int2 *)p;
+    for (int k = 0; k < 4; ++k)
+        if (!map[k].empty())
+            HIP_TRY(hipMemcpy(g.d_map + g.off[k], map[k].data(), sizeof(int2) * map[k].size(), hipMemcpyHostToDevice));
+    return LX_OK;
+}
+
 lx_status rebuild(lx_batch *b) {
     HIP_TRY(hipStreamSynchronize(b->stream));
     for (Group &g : b->grp) free_group(g);
+    for (Group3 &g : b->grp3) free_group3(g);
     free_allocs(b);
     b->others.clear();
     b->grp[0].fp16 = 0;
     b->grp[1].fp16 = 1;
+    b->grp3[0].fp16 = 0;
+    b->grp3[1].fp16 = 1;
     for (lx_sim *s : b->sims) {
-        if (tiled_ok(s)) b->grp[s->fp16].sims.push_back(s);
-        else b->others.push_back(s);
+        if (tiled_ok(s)) {
+            b->grp[s->fp16].sims.push_back(s);
+        } else if (batch3_ok(s)) {
+            lx_status st = ensure_fused_buffers(s);
+            if (st) return st;
+            b->grp3[s->fp16].sims.push_back(s);
+        } else {
+            b->others.push_back(s);
+        }
+    }
+    for (Group3 &g : b->grp3) {
+        if (g.sims.empty()) continue;
+        lx_status st = g.fp16 ? build_group3<__half>(b, g) : build_group3<float>(b, g);
+        if (st) return st;
     }
     for (Group &g : b->grp) {
         if (g.sims.empty()) continue;
@@ -289,6 +407,50 @@ void enqueue_chunk(const lx_batch *b, const Group &g, int nt, int par) {
     launch_batch_finish(g.d_sims, (int)g.sims.size(), nt, s);
 }
 
+// nt steps of the 3D members from parity par: per step the H TF/SF
+// corrections on the read buffers, the fused step (interior boxes, then
+// shells), the E TF/SF corrections on the written buffers; then the
+// sequence end.
+template <typename T>
+void enqueue3_t(const lx_batch *b, const Group3 &g, int nt, int par) {
+    hipStream_t s = b->stream;
+    const B3Sim<T> *sims = (const B3Sim<T> *)g.d_sims;
+    for (int q = 0; q < nt; ++q) {
+        const int pq = (par + q) & 1;
+        launch_tfsf_batch<T>(sims, g.d_map + g.off[2], g.nblk[2], 1, pq, q, s);
+        launch_step3_batch<T>(sims, g.d_map + g.off[0], g.nblk[0], 0, pq, g.vec, s);
+        launch_step3_batch<T>(sims, g.d_map + g.off[1], g.nblk[1], 1, pq, g.vec, s);
+        launch_tfsf_batch<T>(sims, g.d_map + g.off[3], g.nblk[3], 0, pq, q, s);
+    }
+    launch_finish_batch<T>(sims, (int)g.sims.size(), nt, (par + nt) & 1, s);
+}
+
+void enqueue3(const lx_batch *b, const Group3 &g, int nt, int par) {
+    if (g.fp16) enqueue3_t<__half>(b, g, nt, par);
+    else enqueue3_t<float>(b, g, nt, par);
+}
+
+lx_status ensure_graph3(lx_batch *b, Group3 &g, int par) {
+    if (g.gexec[par] || !b->graphs) return LX_OK;
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    enqueue3(b, g, kTT, par);
+    hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (e != hipSuccess || !graph) {
+        (void)hipGetLastError();
+        b->graphs = 0;
+        return LX_OK;
+    }
+    e = hipGraphInstantiate(&g.gexec[par], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        g.gexec[par] = nullptr;
+        b->graphs = 0;
+    }
+    return LX_OK;
+}
+
 lx_status ensure_pair_graph(lx_batch *b, Group &g, int par) {
     if (g.gexec[par] || !b->graphs) return LX_OK;
     hipGraph_t graph = nullptr;
@@ -338,6 +500,7 @@ void lx_batch_destroy(lx_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (Group &g : b->grp) free_group(g);
+    for (Group3 &g : b->grp3) free_group3(g);
     free_allocs(b);
     for (auto &row : b->desc)
         for (void *p : row)
@@ -390,16 +553,18 @@ lx_status lx_batch_step(lx_batch *b, int32_t nsteps) {
     }
     // order after each member's pending work (members whose stream is idle
     // need no event: a served batch steps right after a sync)
-    for (Group &g : b->grp)
-        for (lx_sim *s : g.sims) {
-            if (!s->cs_dirty) continue;
-            s->cs_dirty = false;
-            const hipError_t q = hipStreamQuery(s->cs);
-            if (q == hipSuccess) continue;
-            if (q != hipErrorNotReady) HIP_TRY(q);
-            HIP_TRY(hipEventRecord(b->pre, s->cs));
-            HIP_TRY(hipStreamWaitEvent(b->stream, b->pre, 0));
-        }
+    std::vector<lx_sim *> grouped;
+    for (Group &g : b->grp) grouped.insert(grouped.end(), g.sims.begin(), g.sims.end());
+    for (Group3 &g : b->grp3) grouped.insert(grouped.end(), g.sims.begin(), g.sims.end());
+    for (lx_sim *s : grouped) {
+        if (!s->cs_dirty) continue;
+        s->cs_dirty = false;
+        const hipError_t q = hipStreamQuery(s->cs);
+        if (q == hipSuccess) continue;
+        if (q != hipErrorNotReady) HIP_TRY(q);
+        HIP_TRY(hipEventRecord(b->pre, s->cs));
+        HIP_TRY(hipStreamWaitEvent(b->stream, b->pre, 0));
+    }
     for (Group &g : b->grp) {
         if (g.sims.empty()) continue;
         int par = g.par;
@@ -436,9 +601,38 @@ lx_status lx_batch_step(lx_batch *b, int32_t nsteps) {
         for (lx_sim *s : g.sims) s->steps += nsteps;
         g.stable = 1;
     }
+    for (Group3 &g : b->grp3) {
+        if (g.sims.empty()) continue;
+        int par = g.par;
+        int full = nsteps / kTT;
+        const int rem = nsteps % kTT;
+        // a chunk of kTT (even) steps keeps the parity: one graph per parity
+        if (full >= 1 && (g.stable || full >= 16)) {
+            lx_status st = ensure_graph3(b, g, par);
+            if (st) return st;
+            if (g.gexec[par])
+                for (; full >= 1; --full) HIP_TRY(hipGraphLaunch(g.gexec[par], b->stream));
+        }
+        for (; full > 0; --full) enqueue3(b, g, kTT, par);
+        if (rem) {
+            enqueue3(b, g, rem, par);
+            par ^= rem & 1;
+        }
+        lx_status st = check_launch();
+        if (st) return st;
+        if (par != g.par) {
+            for (lx_sim *s : g.sims) {
+                std::swap(s->F, s->F_alt);
+                std::swap(s->psi, s->psi_alt);
+                drop_graph(s);
+            }
+            g.par = par;
+        }
+        for (lx_sim *s : g.sims) s->steps += nsteps;
+        g.stable = 1;
+    }
     HIP_TRY(hipEventRecord(b->done, b->stream));
-    for (Group &g : b->grp)
-        for (lx_sim *s : g.sims) s->batch_wait = b->done;   // the member's own next work waits for this step
+    for (lx_sim *s : grouped) s->batch_wait = b->done;   // the member's own next work waits for this step
     for (lx_sim *s : b->others) {
         lx_status st = lx_sim_step(s, nsteps);
         if (st) return st;

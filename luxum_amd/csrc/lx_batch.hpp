@@ -55,6 +55,37 @@ struct BAux {
     int *counter;
 };
 
+// One 3D simulation of a batch, advanced by the fused step (the single-
+// simulation kernels' bodies over a flattened block list of every member).
+template <typename T>
+struct B3Sim {
+    Grid<T> g;                 // F/psi: buffer set 0, Fn/psin: buffer set 1
+    UnitMap um[2];             // fused step: interior box, CPML shell
+    int nunits[2];
+    const TfBox *tfH, *tfE;    // device face boxes
+    int ntfH, ntfE, totH, totE;
+    IncParams inc;
+    AuxLine aux;
+    ProbeArgs pr;
+    int use_aux, probes;
+};
+constexpr int kB3Waves = 4;    // wavefronts per block of the batched fused step
+
+// Batched fused step, part 0 (interior boxes, whole grids without CPML) or 1
+// (CPML shells); bmap[b] = (member, member-local block) of launch block b.
+template <typename T>
+void launch_step3_batch(const B3Sim<T> *sims, const int2 *bmap, int nblocks, int part, int par, int vec,
+                        hipStream_t s);
+// Batched TF/SF launch of one step: pre = 1 the H corrections on the read
+// buffer (k_tfsf_hpre's work), pre = 0 the E corrections on the written
+// buffer (k_tfsf<E>'s work); each member's blocks end with its extra block.
+template <typename T>
+void launch_tfsf_batch(const B3Sim<T> *sims, const int2 *bmap, int nblocks, int pre, int par, int toff,
+                       hipStream_t s);
+// Sequence end of every member after n steps: last E probes, step counter.
+template <typename T>
+void launch_finish_batch(const B3Sim<T> *sims, int nsims, int n, int par_end, hipStream_t s);
+
 // inner: every job's region lies inside the domain, outside the CPML slabs,
 // on rows with one z coefficient; otherwise the general path
 template <typename T>

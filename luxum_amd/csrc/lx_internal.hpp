@@ -151,6 +151,17 @@ void launch_step3(const Grid<T> &g, int nw, int vec, int part, hipStream_t s);
 // Rows and planes of the part-0 box of a CPML grid.
 void step3_core_extent(int ny, int nz, int kz, const int slo[3], const int shi[3], int *rows, int *planes);
 
+// The units of one fused-step launch, all x chunks of each: the rows
+// [jlo, jhi) x the z blocks [tlo, thi) (mode 0), or every unit outside that
+// box (mode 1).
+struct UnitMap {
+    int mode, jlo, jhi, tlo, thi, ntz;
+};
+// Unit maps and unit counts of the two parts of a grid's fused step (part 1
+// is empty without CPML) at vec cells per lane.
+void step3_unit_maps(int nx, int ny, int nz, int kz, int pml, const int slo[3], const int shi[3], int vec,
+                     UnitMap um[2], int nunits[2]);
+
 // H TF/SF corrections of the step applied to H of the read buffer before the
 // fused step: H -= s*kh*delta per face cell (the fused step then computes
 // (H - s*kh*delta) - curl).  Needs every H face outside the CPML slabs (no

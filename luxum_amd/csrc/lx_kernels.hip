@@ -21,6 +21,7 @@
 // Block ids are remapped so that each of the 8 XCDs (which receive block ids
 // round robin) works on a contiguous range of tiles: tiles adjacent in y and z
 // share their halo rows and planes in that XCD's L2.
+#include "lx_batch.hpp"
 #include "lx_device.hpp"
 
 namespace lx {
@@ -449,19 +450,22 @@ __device__ __forceinline__ float tf_term(const Grid<T> &g, const TfBox &bx, cons
     return (bx.curl == 0 ? 1.f : -1.f) * coef * delta;
 }
 
+// Block bid of a TF/SF launch over the face cells of one grid (blocks past
+// the face cells run the extra block).
 template <typename T, bool IS_H>
-__global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict__ tb_g, int ntf, int total,
-                                              IncParams inc, AuxLine aux, ProbeArgs pr, Launch l) {
+__device__ __forceinline__ void tfsf_block(const Grid<T> &g, const TfBox *__restrict__ tb_g, int ntf, int total,
+                                           const IncParams &inc, const AuxLine &aux, const ProbeArgs &pr, Launch l,
+                                           int bid) {
     const int t = g.counter[0] + l.toff;
     const int nb = (total + 255) / 256;
-    if ((int)blockIdx.x >= nb) {
+    if (bid >= nb) {
         extra_block<T, IS_H>(g, aux, pr, l, t);
         return;
     }
     __shared__ TfBox sb[kMaxTf];
     stage_boxes(sb, tb_g, ntf);
     const TfBox *tb = sb;
-    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int gid = bid * 256 + threadIdx.x;
     if (gid >= total) return;
     int b = 0;
     while (b + 1 < ntf && gid >= tb[b + 1].start) ++b;
@@ -495,6 +499,12 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict
     }
 }
 
+template <typename T, bool IS_H>
+__global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict__ tb_g, int ntf, int total,
+                                              IncParams inc, AuxLine aux, ProbeArgs pr, Launch l) {
+    tfsf_block<T, IS_H>(g, tb_g, ntf, total, inc, aux, pr, l, blockIdx.x);
+}
+
 // Fused 3D step: the H corrections of the step applied to H of the buffer
 // the fused step reads, H -= s*kh*delta per face cell (same delta and
 // coefficient as k_tfsf<H>; the H face boxes of one component are disjoint,
@@ -503,18 +513,19 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict
 // runs the aux line's H phase (it writes h; the face cells read e) and
 // records the E probes of the previous step (E is not written here).
 template <typename T>
-__global__ __launch_bounds__(256) void k_tfsf_hpre(Grid<T> g, const TfBox *__restrict__ tb_g, int ntf, int total,
-                                                   IncParams inc, AuxLine aux, ProbeArgs pr, Launch l) {
+__device__ __forceinline__ void hpre_block(const Grid<T> &g, const TfBox *__restrict__ tb_g, int ntf, int total,
+                                           const IncParams &inc, const AuxLine &aux, const ProbeArgs &pr, Launch l,
+                                           int bid) {
     const int t = g.counter[0] + l.toff;
     const int nb = (total + 255) / 256;
-    if ((int)blockIdx.x >= nb) {
+    if (bid >= nb) {
         extra_block<T, true>(g, aux, pr, l, t);
         return;
     }
     __shared__ TfBox sb[kMaxTf];
     stage_boxes(sb, tb_g, ntf);
     const TfBox *tb = sb;
-    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int gid = bid * 256 + threadIdx.x;
     if (gid >= total) return;
     int b = 0;
     while (b + 1 < ntf && gid >= tb[b + 1].start) ++b;
@@ -536,6 +547,32 @@ __global__ __launch_bounds__(256) void k_tfsf_hpre(Grid<T> g, const TfBox *__res
     st1(H + o, ld1(H + o) - upd);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_tfsf_hpre(Grid<T> g, const TfBox *__restrict__ tb_g, int ntf, int total,
+                                                   IncParams inc, AuxLine aux, ProbeArgs pr, Launch l) {
+    hpre_block<T>(g, tb_g, ntf, total, inc, aux, pr, l, blockIdx.x);
+}
+
+// Batched TF/SF launch of one step over every member of a 3D batch: block b
+// runs block bmap[b].y of member bmap[b].x.  pre: the H corrections on the
+// buffer the step reads (buffer set par); else the E corrections on the
+// buffer it wrote (set par ^ 1).
+template <typename T, bool PRE>
+__global__ __launch_bounds__(256) void k_tfsf_batch(const B3Sim<T> *__restrict__ sims, const int2 *__restrict__ bmap,
+                                                    int par, int toff) {
+    const int2 mb = bmap[blockIdx.x];
+    const B3Sim<T> *d = sims + mb.x;
+    Grid<T> g = d->g;
+    const bool second = PRE ? par != 0 : par == 0;   // the buffer set these corrections apply to
+    if (second) {
+        g.F = g.Fn;
+        g.psi = g.psin;
+    }
+    const Launch l{toff, d->use_aux, d->probes};
+    if (PRE) hpre_block<T>(g, d->tfH, d->ntfH, d->totH, d->inc, d->aux, d->pr, l, mb.y);
+    else tfsf_block<T, false>(g, d->tfE, d->ntfE, d->totE, d->inc, d->aux, d->pr, l, mb.y);
+}
+
 // ---------------------------------------------------------------------------
 // point source, sequence end, gather, monitor, fills
 // ---------------------------------------------------------------------------
@@ -553,13 +590,27 @@ __global__ void k_point_source(Grid<T> g, PointSrc ps, int toff) {
 }
 
 template <typename T>
-__global__ void k_finish(Grid<T> g, ProbeArgs pr, int n) {
+__device__ __forceinline__ void finish_block(const Grid<T> &g, const ProbeArgs &pr, int n) {
     const int tlast = g.counter[0] + n - 1;
     for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
         if (pr.comp[q] < 3 && tlast < pr.cap)
             pr.rec[(long long)tlast * pr.n + q] = (double)ld1(g.F + pr.comp[q] * g.fstride + pr.idx[q]);
     __syncthreads();
     if (threadIdx.x == 0) g.counter[0] = tlast + 1;
+}
+
+template <typename T>
+__global__ void k_finish(Grid<T> g, ProbeArgs pr, int n) {
+    finish_block<T>(g, pr, n);
+}
+
+// one block per member of a 3D batch; the state is in buffer set par_end
+template <typename T>
+__global__ void k_finish_batch(const B3Sim<T> *__restrict__ sims, int n, int par_end) {
+    const B3Sim<T> *d = sims + blockIdx.x;
+    Grid<T> g = d->g;
+    if (par_end) g.F = g.Fn;
+    finish_block<T>(g, d->pr, n);
 }
 
 template <typename T>
@@ -712,6 +763,18 @@ void launch_finish(const Grid<T> &g, const ProbeArgs &pr, int n, hipStream_t s) 
     hipLaunchKernelGGL(k_finish<T>, dim3(1), dim3(64), 0, s, g, pr, n);
 }
 template <typename T>
+void launch_tfsf_batch(const B3Sim<T> *sims, const int2 *bmap, int nblocks, int pre, int par, int toff,
+                       hipStream_t s) {
+    if (nblocks <= 0) return;
+    if (pre) hipLaunchKernelGGL((k_tfsf_batch<T, true>), dim3(nblocks), dim3(256), 0, s, sims, bmap, par, toff);
+    else hipLaunchKernelGGL((k_tfsf_batch<T, false>), dim3(nblocks), dim3(256), 0, s, sims, bmap, par, toff);
+}
+template <typename T>
+void launch_finish_batch(const B3Sim<T> *sims, int nsims, int n, int par_end, hipStream_t s) {
+    if (nsims <= 0) return;
+    hipLaunchKernelGGL(k_finish_batch<T>, dim3(nsims), dim3(64), 0, s, sims, n, par_end);
+}
+template <typename T>
 void launch_point_source(const Grid<T> &g, const PointSrc &ps, int toff, hipStream_t s) {
     hipLaunchKernelGGL(k_point_source<T>, dim3(1), dim3(1), 0, s, g, ps, toff);
 }
@@ -759,7 +822,9 @@ void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
     template void launch_point_source<T>(const Grid<T> &, const PointSrc &, int, hipStream_t);               \
     template void launch_gather<T>(const Grid<T> &, int, int, int, int, double, double *, hipStream_t);      \
     template void launch_monitor<T>(const Grid<T> &, int, double *, hipStream_t);                            \
-    template void launch_gather_e2<T>(const Grid<T> &, int, int, int, double *, hipStream_t);
+    template void launch_gather_e2<T>(const Grid<T> &, int, int, int, double *, hipStream_t);              \
+    template void launch_tfsf_batch<T>(const B3Sim<T> *, const int2 *, int, int, int, int, hipStream_t);   \
+    template void launch_finish_batch<T>(const B3Sim<T> *, int, int, int, hipStream_t);
 LX_INST(float)
 LX_INST(__half)
 

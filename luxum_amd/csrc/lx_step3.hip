@@ -30,6 +30,7 @@
 // (k_tfsf_hpre), the E ones to E_new afterwards (k_tfsf): this is the bulk
 // update only.  V = 2 (two cells per lane, 8- or 4-byte loads) halves the
 // memory instructions and address arithmetic per cell.
+#include "lx_batch.hpp"
 #include "lx_device.hpp"
 
 namespace lx {
@@ -186,26 +187,16 @@ __device__ __forceinline__ vf<V> curl2(const vf<V> &d1, const vf<V> &k1, bool s1
     return r;
 }
 
-// The units of one launch, all x chunks of each: the rows [jlo, jhi) x the z
-// blocks [tlo, thi) (mode 0), or every unit outside that box (mode 1).
-struct UnitMap {
-    int mode, jlo, jhi, tlo, thi, ntz;
-};
-
 // PM: CPML axes the launch handles (bit 0: x slab, 1: y, 2: z).  A CPML grid
 // runs two launches: the units whose rows and planes lie outside the y and
 // z slabs (PM = 1, x slab only, far fewer registers) and the rest (PM = 7).
-template <typename T, int PM, int NW, int V>
-__global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nunits) {
+// One unit (x chunk, row, z block) of the step; cbs is the coefficient table
+// of the grid's materials in LDS.
+template <typename T, int PM, int V>
+__device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, int unit, const float *cbs) {
     constexpr bool PML = PM != 0;
     using F = vf<V>;
-    __shared__ float cbs[256];
     const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int q = threadIdx.x; q < 256; q += 64 * NW) cbs[q] = g.cb[q];
-    __syncthreads();
-    const int unit = xcd_remap(blockIdx.x, (nunits + NW - 1) / NW) * NW + w;
-    if (unit >= nunits) return;                    // whole wavefront; no barrier follows
     const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
     const int nch = (nx + 64 * V - 1) / (64 * V);
     const int ch = unit % nch;
@@ -261,7 +252,7 @@ __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nu
     const unsigned m_ez = jin ? vmask<V>(i0, 1, nx - 2) : 0u;
     const F khx = ldtab<V>(tab + (0 * 6 + TK_KH) * tn, i0), kex = ldtab<V>(tab + (0 * 6 + TK_KE) * tn, i0);
     // the x slab bounds are multiples of 4, so a lane's V cells are wholly in or out
-    const bool sxw = (PM & 1) && (x0 < g.slo[0] || x0 + 64 * V > g.shi[0]);   // chunk meets the x slab
+    const bool sxw = (PM & 1) && g.pml && (x0 < g.slo[0] || x0 + 64 * V > g.shi[0]);   // chunk meets the x slab
     const bool sx = sxw && (i0 < g.slo[0] || i0 >= g.shi[0]);
     const int xs = sx ? (i0 < g.slo[0] ? i0 : g.slo[0] + (i0 - g.shi[0])) : 0;
     F bhx = vs<V>(0.f), chx = vs<V>(0.f), bex = vs<V>(0.f), cex = vs<V>(0.f);
@@ -533,6 +524,46 @@ __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nu
     }
 }
 
+template <typename T, int PM, int NW, int V>
+__global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nunits) {
+    __shared__ float cbs[256];
+    for (int q = threadIdx.x; q < 256; q += 64 * NW) cbs[q] = g.cb[q];
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int unit = xcd_remap(blockIdx.x, (nunits + NW - 1) / NW) * NW + w;
+    if (unit >= nunits) return;                    // whole wavefront; no barrier follows
+    step3_unit<T, PM, V>(g, um, unit, cbs);
+}
+
+// Batched step: block b of the launch runs NW units of member bmap[b].x, its
+// units bmap[b].y*NW on (a member's blocks never hold another's units, so
+// the block's LDS table is that member's).  Part 0: interior boxes (and whole
+// grids without CPML), PM = 1; part 1: CPML shells, PM = 7.  par selects the
+// direction: 0 reads buffer set 0 and writes set 1, 1 the reverse.
+template <typename T, int PM, int NW, int V>
+__global__ __launch_bounds__(64 * NW) void k_step3_batch(const B3Sim<T> *__restrict__ sims,
+                                                         const int2 *__restrict__ bmap, int nblocks, int part,
+                                                         int par) {
+    __shared__ float cbs[256];
+    const int2 mb = bmap[xcd_remap(blockIdx.x, nblocks)];
+    const B3Sim<T> *d = sims + mb.x;
+    Grid<T> g = d->g;
+    if (par) {
+        T *f = g.F;
+        g.F = g.Fn;
+        g.Fn = f;
+        float *q = g.psi;
+        g.psi = g.psin;
+        g.psin = q;
+    }
+    for (int q = threadIdx.x; q < 256; q += 64 * NW) cbs[q] = g.cb[q];
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int unit = mb.y * NW + w;
+    if (unit >= d->nunits[part]) return;
+    step3_unit<T, PM, V>(g, d->um[part], unit, cbs);
+}
+
 template <typename T, int PM, int V>
 static void launch_units(const Grid<T> &g, int nw, const UnitMap &um, int nunits, hipStream_t s) {
     if (nunits <= 0) return;
@@ -561,21 +592,35 @@ void step3_core_extent(int ny, int nz, int kz, const int slo[3], const int shi[3
     *planes = std::min(um.thi * kz, nz) - um.tlo * kz;
 }
 
-template <typename T, int V>
-static void launch_parts(const Grid<T> &g, int nw, int part, hipStream_t s) {
-    const int ntz = (g.nz + g.kz - 1) / g.kz;
-    const int nch = (g.nx + 64 * V - 1) / (64 * V);   // one wavefront per (x chunk, row, z block)
-    if (!g.pml) {
-        if (part == 0) launch_units<T, 0, V>(g, nw, UnitMap{0, 0, g.ny, 0, ntz, ntz}, nch * g.ny * ntz, s);
+void step3_unit_maps(int nx, int ny, int nz, int kz, int pml, const int slo[3], const int shi[3], int vec,
+                     UnitMap um[2], int nunits[2]) {
+    const int ntz = (nz + kz - 1) / kz;
+    const int nch = (nx + 64 * vec - 1) / (64 * vec);   // one wavefront per (x chunk, row, z block)
+    if (!pml) {
+        um[0] = UnitMap{0, 0, ny, 0, ntz, ntz};
+        um[1] = um[0];
+        nunits[0] = nch * ny * ntz;
+        nunits[1] = 0;
         return;
     }
-    UnitMap um = core_box(g.ny, g.nz, g.kz, g.slo, g.shi);
-    const int nbox = nch * (um.jhi - um.jlo) * (um.thi - um.tlo);
-    if (part == 0) {
-        launch_units<T, 1, V>(g, nw, um, nbox, s);
+    um[0] = core_box(ny, nz, kz, slo, shi);
+    um[1] = um[0];
+    um[1].mode = 1;
+    nunits[0] = nch * (um[0].jhi - um[0].jlo) * (um[0].thi - um[0].tlo);
+    nunits[1] = nch * ny * ntz - nunits[0];
+}
+
+template <typename T, int V>
+static void launch_parts(const Grid<T> &g, int nw, int part, hipStream_t s) {
+    UnitMap um[2];
+    int nu[2];
+    step3_unit_maps(g.nx, g.ny, g.nz, g.kz, g.pml, g.slo, g.shi, V, um, nu);
+    if (!g.pml) {
+        if (part == 0) launch_units<T, 0, V>(g, nw, um[0], nu[0], s);
+    } else if (part == 0) {
+        launch_units<T, 1, V>(g, nw, um[0], nu[0], s);
     } else {
-        um.mode = 1;
-        launch_units<T, 7, V>(g, nw, um, nch * g.ny * ntz - nbox, s);
+        launch_units<T, 7, V>(g, nw, um[1], nu[1], s);
     }
 }
 
@@ -587,5 +632,26 @@ void launch_step3(const Grid<T> &g, int nw, int vec, int part, hipStream_t s) {
 
 template void launch_step3<float>(const Grid<float> &, int, int, int, hipStream_t);
 template void launch_step3<__half>(const Grid<__half> &, int, int, int, hipStream_t);
+
+template <typename T>
+void launch_step3_batch(const B3Sim<T> *sims, const int2 *bmap, int nblocks, int part, int par, int vec,
+                        hipStream_t s) {
+    if (nblocks <= 0) return;
+    constexpr int NW = kB3Waves;
+    if (part == 0) {
+        if (vec == 2)
+            hipLaunchKernelGGL((k_step3_batch<T, 1, NW, 2>), dim3(nblocks), dim3(64 * NW), 0, s, sims, bmap, nblocks, 0, par);
+        else
+            hipLaunchKernelGGL((k_step3_batch<T, 1, NW, 1>), dim3(nblocks), dim3(64 * NW), 0, s, sims, bmap, nblocks, 0, par);
+    } else {
+        if (vec == 2)
+            hipLaunchKernelGGL((k_step3_batch<T, 7, NW, 2>), dim3(nblocks), dim3(64 * NW), 0, s, sims, bmap, nblocks, 1, par);
+        else
+            hipLaunchKernelGGL((k_step3_batch<T, 7, NW, 1>), dim3(nblocks), dim3(64 * NW), 0, s, sims, bmap, nblocks, 1, par);
+    }
+}
+
+template void launch_step3_batch<float>(const B3Sim<float> *, const int2 *, int, int, int, int, hipStream_t);
+template void launch_step3_batch<__half>(const B3Sim<__half> *, const int2 *, int, int, int, int, hipStream_t);
 
 }  // namespace lx

@@ -1,5 +1,6 @@
 """Batched simulations: the temporally tiled 2D kernel over a flattened tile
-list (several planar simulations of uneven sizes in one launch) against the
+list (several planar simulations of uneven sizes in one launch) and the
+batched fused 3D step over a flattened block list, against the
 single-simulation kernels and the FP64 oracle.
 
 Tolerance: relative L2 <= 1e-5 (FP32) of the E and H fields and of the probe
@@ -12,7 +13,7 @@ import pytest
 
 from conftest import rel_l2
 from luxum_amd import configs
-from luxum_amd.params import LX_EY, LX_FP16, LX_SRC_SOFT, default_params
+from luxum_amd.params import LX_EY, LX_FP16, LX_INC_ANALYTIC, LX_SRC_SOFT, default_params
 from oracle.lxoracle import OracleSim
 
 pytestmark = pytest.mark.gpu
@@ -154,4 +155,61 @@ def test_batch_fp16_bound(gpu_ctx):
     pe = rel_l2(s.probes(), o.probes())
     say(f"fp16 batch vs oracle fields {e:.2e} probes {pe:.2e}")
     assert e <= 2e-2 and pe <= 2e-2
+    b.close()
+
+
+def plain_oblique_3d():
+    """3D grid without CPML, analytic oblique plane wave, vacuum."""
+    p = default_params(nx=40, ny=36, nz=44, use_pw=1, incident=LX_INC_ANALYTIC, off_pw=90.0,
+                       pw_sigma=(20 * 5e-8) ** 2, tf_lo=(8, 7, 9), tf_hi=(30, 27, 33), courant=0.5)
+    p.k_pw[:] = (0.6, 0.48, 0.64)
+    p.e_pw[:] = (0.6246950475544243, -0.7808688094430304, 0.0)
+    return configs.Workload("3D plain oblique 40x36x44", p, np.zeros((40, 36, 44), np.uint8), np.array([1.0]),
+                            None)
+
+
+def test_batch_3d_members(gpu_ctx):
+    """3D members of uneven sizes advance in the batched fused step (FP32 and
+    FP16 groups, CPML + aux TF/SF + probes, a grid without CPML, a planar
+    member beside them); odd step counts, a member removed and stepped alone,
+    then the batch again.  Same arithmetic as the single-simulation fused
+    step: tolerance 1e-6."""
+    from luxum_amd.sim import Batch
+    ws = [configs.box3d(72, 40, 36), configs.config1(n=40, steps=400, pml=6, margin=4), plain_oblique_3d(),
+          configs.box3d(100, 36, 40, precision=LX_FP16), configs.box3d(48, 44, 40, precision=LX_FP16),
+          phc(128, 1)]
+    members = [make(w, gpu_ctx) for w in ws]
+    b = Batch(gpu_ctx)
+    for s in members:
+        b.add(s)
+    say("batch 3d: 6 members (3 FP32 3D, 2 FP16 3D, 1 tiled 2D): batch 37, batch 100, remove #1, "
+        "single 5, batch 21")
+    b.step(37)
+    b.step(100)
+    b.remove(members[1])
+    members[1].step(5)
+    b.step(21)
+    b.sync()
+    counts = [158, 142, 158, 158, 158, 158]
+    for w, s, n in zip(ws, members, counts):
+        ref = make(w, gpu_ctx)
+        ref.step(n)
+        ref.sync()
+        e = field_err(s, ref)
+        msg = f"{w.name}: {n} steps, batch vs single fields {e:.2e}"
+        if w.probes is not None:
+            pe = rel_l2(s.probes(), ref.probes())
+            msg += f" probes {pe:.2e}"
+            assert pe <= 1e-6
+        say(msg)
+        assert e <= 1e-6
+        assert s.info().step == n
+    o = OracleSim(ws[0].p)
+    o.set_materials(ws[0].ids, ws[0].eps_r)
+    o.set_probes(ws[0].probes)
+    o.step(158)
+    e = field_err(members[0], o)
+    pe = rel_l2(members[0].probes(), o.probes())
+    say(f"{ws[0].name}: batch vs oracle fields {e:.2e} probes {pe:.2e}")
+    assert e <= TOL32 and pe <= TOL32
     b.close()

@@ -217,30 +217,6 @@ def test_fused_step_matches_half_steps(gpu_ctx, monkeypatch):
     assert e <= 1e-6
 
 
-def _slab_box(nx, ny, nz, pml, margin, precision=0):
-    """CPML grid of uneven extents, aux-1D plane wave along +x, a dielectric
-    block off the centre and 4 Ez probes."""
-    p = default_params(is_planar=0, nx=nx, ny=ny, nz=nz, dx=configs.DX, dy=configs.DX, dz=configs.DX,
-                       courant=0.5, precision=precision)
-    configs._cpml(p, pml)
-    p.use_pw = 1
-    p.incident = LX_INC_AUX1D
-    p.k_pw[:] = (1.0, 0.0, 0.0)
-    p.e_pw[:] = (0.0, 0.0, 1.0)
-    lo = pml + margin
-    p.tf_lo[:] = (lo, lo, lo)
-    p.tf_hi[:] = (nx - 1 - lo, ny - 1 - lo, nz - 1 - lo)
-    p.pw_lambda0 = configs.LAMBDA0
-    p.pw_fwhm = 3.0
-    p.pw_delay = 4.0
-    ids = np.zeros((nx, ny, nz), dtype=np.uint8)
-    ids[nx // 3:nx // 2 + 7, ny // 3:ny // 2 + 3, nz // 3:nz // 2 + 5] = 1
-    eps = np.array([1.0, 4.0])
-    probes = np.array([[LX_EZ, x, ny // 2, z] for x in (nx // 4, 3 * nx // 4) for z in (nz // 3, 2 * nz // 3)],
-                      dtype=np.int32)
-    return p, ids, eps, probes
-
-
 def test_fused_two_cells_per_lane(gpu_ctx, monkeypatch):
     """The fused step with two x cells per lane (128-cell chunks) against
     the half-step kernels (1e-6) and the FP64 oracle (1e-5): two chunks per
@@ -271,7 +247,8 @@ def test_fused_two_cells_per_lane(gpu_ctx, monkeypatch):
             den += float(np.sum(y ** 2))
         return (num / den) ** 0.5
 
-    p, ids, eps, probes = _slab_box(200, 40, 44, 8, 5)
+    w = configs.box3d(200, 40, 44)
+    p, ids, eps, probes = w.p, w.ids, w.eps_r, w.probes
     say(f"case vec2 cpml aux nx={p.nx} ny={p.ny} nz={p.nz}")
     a, b = run(p, 161, (ids, eps, probes), "2"), run(p, 161, (ids, eps, probes), "0")
     e, pe = rel(a, b, p.components()), rel_l2(a.probes(), b.probes())
@@ -293,7 +270,8 @@ def test_fused_two_cells_per_lane(gpu_ctx, monkeypatch):
     say(f"vec2 vs half-step (plain, oblique analytic): fields {e:.2e}")
     assert e <= 1e-6
 
-    p, ids, eps, probes = _slab_box(120, 36, 40, 8, 5, precision=LX_FP16)
+    w = configs.box3d(120, 36, 40, precision=LX_FP16)
+    p, ids, eps, probes = w.p, w.ids, w.eps_r, w.probes
     say(f"case vec2 fp16 cpml aux nx={p.nx} ny={p.ny} nz={p.nz}")
     a, b = run(p, 150, (ids, eps, probes), "2"), run(p, 150, (ids, eps, probes), "1")
     e = rel(a, b, p.components())
