@@ -127,7 +127,7 @@ typedef struct lx_sim_info {
     int32_t kz, waves;          /* z planes and wavefronts per block of the 3D kernels (autotuned) */
     int64_t core_bytes;         /* fused step on a CPML grid: algorithmic bytes of its interior-box launch
                                    (rows and planes outside the y and z slabs); else bytes_per_step */
-    int32_t cells_per_lane;     /* fused step: x cells per lane (1 or 2, autotuned); else 1 */
+    int32_t cells_per_lane;     /* fused step: x cells per lane (1, 2 or 4, autotuned); else 1 */
     int32_t reserved;
 } lx_sim_info;
 

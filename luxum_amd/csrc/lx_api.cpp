@@ -585,6 +585,13 @@ size_t psi_total(const lx_sim *s) {
 // The fused 3D step applies the H TF/SF corrections to the read buffer
 // before the step, which leaves out the CPML psi term: it needs every H face
 // outside the slabs along its face axis.
+// x cells per lane of the fused step that the row pitch (and, for 4, the
+// storage type) allows
+bool vec_ok(const lx_sim *s, int vec) {
+    if (vec == 4 && !s->fp16) return false;
+    return (s->N[0] + 64 * vec - 1) / (64 * vec) * (64 * vec) <= s->px;
+}
+
 bool fused_ok(const lx_sim *s) {
     if (s->planar) return false;
     for (int b = 0; b < s->tfH.n; ++b) {
@@ -819,8 +826,9 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
         // 3D: the fused step where it applies and its second buffer set fits
         // (faster than the half-step pair on every grid measured), else the
         // half-step pair.  LX_FUSED=0 selects the half-step pair, 1 the fused
-        // step with one cell per lane, 2 with two where the row pitch allows
-        // it; unset: two cells per lane for FP16 storage, one for FP32.
+        // step with one cell per lane, 2 with two and 4 with four (FP16
+        // storage) where the row pitch allows it; unset: the widest of four
+        // and two cells per lane for FP16 storage, one for FP32.
         // lx_sim_autotune times them all and keeps the fastest.
         const char *fz = std::getenv("LX_FUSED");
         const char f = fz && *fz ? *fz : 'd';
@@ -830,8 +838,9 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
                 s->fused = 1;
                 s->kz = 8;
                 s->nw = 4;
-                const bool v2 = (s->N[0] + 127) / 128 * 128 <= s->px;
-                s->vec = v2 && (f == '2' || (f == 'd' && s->fp16)) ? 2 : 1;
+                s->vec = 1;
+                if ((f == '4' || (f == 'd' && s->fp16)) && vec_ok(s, 4)) s->vec = 4;
+                else if ((f == '2' || (f == 'd' && s->fp16)) && vec_ok(s, 2)) s->vec = 2;
             } else if (st == LX_ENOMEM && f == 'd') {
                 (void)hipGetLastError();
                 release_fused_buffers(s);
@@ -1185,12 +1194,11 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         // 4 wavefronts per block and 1 or 2 cells per lane (its second buffer
         // set is kept only if it wins)
         const int can_fuse = fused_ok(s) && ensure_fused_buffers(s) == LX_OK;
-        const bool vec2 = (s->N[0] + 127) / 128 * 128 <= s->px;
         struct Cand { int nw, vec; };
-        for (Cand c : {Cand{0, 1}, Cand{8, 1}, Cand{4, 1}, Cand{4, 2}, Cand{8, 2}}) {
+        for (Cand c : {Cand{0, 1}, Cand{8, 1}, Cand{4, 1}, Cand{4, 2}, Cand{8, 2}, Cand{4, 4}}) {
             const int nw = c.nw;
             if (nw && !can_fuse) continue;
-            if (c.vec == 2 && !vec2) continue;
+            if (!vec_ok(s, c.vec)) continue;
             s->fused = nw != 0;
             for (int kz : {4, 5, 6, 8, 12, 16, 32, 64}) {
                 if (kz > s->N[2] && kz != 4) continue;

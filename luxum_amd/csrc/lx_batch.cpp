@@ -182,7 +182,7 @@ lx_status build_group3(lx_batch *b, Group3 &g) {
     if (g.fp16) {
         g.vec = 2;
         for (const lx_sim *s : g.sims)
-            if ((s->N[0] + 127) / 128 * 128 > s->px) g.vec = 1;
+            if (!vec_ok(s, 2)) g.vec = 1;
     }
     std::vector<int> order(n);
     for (size_t q = 0; q < n; ++q) order[q] = (int)q;

@@ -140,8 +140,9 @@ void launch_tfsf(const Grid<T> &g, const TfList &tf, const IncParams &inc, const
                  Launch l, int half, hipStream_t s);
 
 // Fused 3D timestep (H and E half steps, F/psi read, Fn/psin written), nw
-// wavefronts per block, vec (1 or 2) x cells per lane; vec 2 needs the
-// padded row pitch to hold every 128-cell chunk (px >= nx rounded up to 128).  Bulk update only: the H TF/SF corrections are
+// wavefronts per block, vec (1, 2, or 4 with FP16 storage) x cells per lane;
+// vec needs the padded row pitch to hold every 64*vec-cell chunk (px >= nx
+// rounded up to 64*vec).  Bulk update only: the H TF/SF corrections are
 // applied to the read buffer beforehand (launch_tfsf_hpre), the E ones to the
 // written buffer afterwards (launch_tfsf).  Two parts: part 0 is the whole
 // grid without CPML, else the interior box (rows and planes outside the y and

@@ -61,7 +61,7 @@ struct lx_sim {
     int kz = 32;
     int fused = 0;              // 3D: one fused launch per step over ping-pong buffers (F/psi and F_alt/psi_alt)
     int nw = 8;                 // fused step: wavefronts per block
-    int vec = 1;                // fused step: x cells per lane (1 or 2)
+    int vec = 1;                // fused step: x cells per lane (1, 2 or 4)
     lx::IncParams inc{};
     lx::AuxLine aux{};
     int use_aux = 0;
@@ -100,6 +100,7 @@ size_t psi_total(const lx_sim *s);
 ProbeArgs probe_args(const lx_sim *s);
 void drop_graph(lx_sim *s);
 bool fused_ok(const lx_sim *s);
+bool vec_ok(const lx_sim *s, int vec);
 // The side stream (snapshot copies) is taken from the context's pool on
 // first use.
 lx_status ensure_side_stream(lx_sim *s);

@@ -29,7 +29,10 @@
 // coefficient load.  The H TF/SF corrections are applied to H_old beforehand
 // (k_tfsf_hpre), the E ones to E_new afterwards (k_tfsf): this is the bulk
 // update only.  V = 2 (two cells per lane, 8- or 4-byte loads) halves the
-// memory instructions and address arithmetic per cell.
+// memory instructions and address arithmetic per cell; V = 4 (FP16 storage
+// only) gives 8-byte field loads.
+#include <type_traits>
+
 #include "lx_batch.hpp"
 #include "lx_device.hpp"
 
@@ -122,6 +125,18 @@ __device__ __forceinline__ vf<2> ldv<__half, 2>(const __half *b, unsigned o) {
     const float2 x = __half22float2(*reinterpret_cast<const __half2 *>(at(b, o)));
     return vf<2>{{x.x, x.y}};
 }
+template <>
+__device__ __forceinline__ vf<4> ldv<float, 4>(const float *b, unsigned o) {
+    const float4 x = *reinterpret_cast<const float4 *>(at(b, o));
+    return vf<4>{{x.x, x.y, x.z, x.w}};
+}
+template <>
+__device__ __forceinline__ vf<4> ldv<__half, 4>(const __half *b, unsigned o) {
+    const uint2 u = *reinterpret_cast<const uint2 *>(at(b, o));
+    const float2 lo = __half22float2(*reinterpret_cast<const __half2 *>(&u.x));
+    const float2 hi = __half22float2(*reinterpret_cast<const __half2 *>(&u.y));
+    return vf<4>{{lo.x, lo.y, hi.x, hi.y}};
+}
 template <typename T, int V>
 __device__ __forceinline__ void stv(T *base, unsigned boff, const vf<V> &v);
 template <>
@@ -135,6 +150,18 @@ __device__ __forceinline__ void stv<__half, 1>(__half *b, unsigned o, const vf<1
 template <>
 __device__ __forceinline__ void stv<__half, 2>(__half *b, unsigned o, const vf<2> &v) {
     *reinterpret_cast<__half2 *>(at(b, o)) = __floats2half2_rn(v.v[0], v.v[1]);
+}
+template <>
+__device__ __forceinline__ void stv<float, 4>(float *b, unsigned o, const vf<4> &v) {
+    *reinterpret_cast<float4 *>(at(b, o)) = make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
+}
+template <>
+__device__ __forceinline__ void stv<__half, 4>(__half *b, unsigned o, const vf<4> &v) {
+    const __half2 lo = __floats2half2_rn(v.v[0], v.v[1]), hi = __floats2half2_rn(v.v[2], v.v[3]);
+    uint2 u;
+    u.x = *reinterpret_cast<const unsigned *>(&lo);
+    u.y = *reinterpret_cast<const unsigned *>(&hi);
+    *reinterpret_cast<uint2 *>(at(b, o)) = u;
 }
 // coefficient row of the x axis (float table) at x index i0
 template <int V>
@@ -350,10 +377,14 @@ __device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, 
         unsigned id[V];
         if constexpr (V == 1) {
             id[0] = g.mat[kp + re];
-        } else {
+        } else if constexpr (V == 2) {
             const uchar2 u = *reinterpret_cast<const uchar2 *>(g.mat + kp + re);
             id[0] = u.x;
             id[1] = u.y;
+        } else {
+            const unsigned u = *reinterpret_cast<const unsigned *>(g.mat + kp + re);
+#pragma unroll
+            for (int e = 0; e < V; ++e) id[e] = (u >> (8 * e)) & 0xffu;
         }
         float ezN = 0.f, eyN = 0.f, eymN = 0.f;
         if (nxt) {
@@ -626,6 +657,12 @@ static void launch_parts(const Grid<T> &g, int nw, int part, hipStream_t s) {
 
 template <typename T>
 void launch_step3(const Grid<T> &g, int nw, int vec, int part, hipStream_t s) {
+    if constexpr (std::is_same<T, __half>::value) {
+        if (vec == 4) {
+            launch_parts<T, 4>(g, nw, part, s);
+            return;
+        }
+    }
     if (vec == 2) launch_parts<T, 2>(g, nw, part, s);
     else launch_parts<T, 1>(g, nw, part, s);
 }

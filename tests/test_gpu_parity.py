@@ -222,7 +222,7 @@ def test_fused_two_cells_per_lane(gpu_ctx, monkeypatch):
     the half-step kernels (1e-6) and the FP64 oracle (1e-5): two chunks per
     row with a partial second one (nx = 200, padded pitch 256), and one
     partial chunk on a plain grid (nx = 100); FP16 storage against the
-    one-cell-per-lane fused step."""
+    one-cell-per-lane fused step, with two and with four cells per lane."""
     from luxum_amd.sim import Simulation
 
     def run(p, steps, w, mode):
@@ -233,7 +233,7 @@ def test_fused_two_cells_per_lane(gpu_ctx, monkeypatch):
             s.set_probes(w[2])
         inf = s.info()
         assert bool(inf.fused_step) == (mode != "0")
-        assert inf.cells_per_lane == (2 if mode == "2" else 1)
+        assert inf.cells_per_lane == (int(mode) if mode in ("2", "4") else 1)
         s.step(steps)
         s.sync()
         return s
@@ -277,3 +277,11 @@ def test_fused_two_cells_per_lane(gpu_ctx, monkeypatch):
     e = rel(a, b, p.components())
     say(f"fp16 vec2 vs vec1 fused (cpml aux, 150 steps): fields {e:.2e}")
     assert e <= 1e-6
+
+    w = configs.box3d(200, 36, 40, precision=LX_FP16)
+    p, ids, eps, probes = w.p, w.ids, w.eps_r, w.probes
+    say(f"case vec4 fp16 cpml aux nx={p.nx} ny={p.ny} nz={p.nz}")
+    a, b = run(p, 133, (ids, eps, probes), "4"), run(p, 133, (ids, eps, probes), "1")
+    e, pe = rel(a, b, p.components()), rel_l2(a.probes(), b.probes())
+    say(f"fp16 vec4 vs vec1 fused (cpml aux, 133 steps): fields {e:.2e} probes {pe:.2e}")
+    assert e <= 1e-6 and pe <= 1e-6
