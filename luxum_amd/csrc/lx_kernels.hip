@@ -470,7 +470,8 @@ __device__ __forceinline__ void tfsf_block(const Grid<T> &g, const TfBox *__rest
     int b = 0;
     while (b + 1 < ntf && gid >= tb[b + 1].start) ++b;
     const TfBox &bx = tb[b];
-    const int fa = bx.fa, u = (fa + 1) % 3, v = (fa + 2) % 3;
+    // face-local index: x fastest where the face contains x (coalesced rows)
+    const int fa = bx.fa, u = fa == 0 ? 1 : 0, v = fa == 2 ? 1 : 2;
     const int nu = bx.hi[u] - bx.lo[u] + 1;
     const int loc = gid - bx.start;
     int ijk[3];
@@ -530,7 +531,8 @@ __device__ __forceinline__ void hpre_block(const Grid<T> &g, const TfBox *__rest
     int b = 0;
     while (b + 1 < ntf && gid >= tb[b + 1].start) ++b;
     const TfBox &bx = tb[b];
-    const int fa = bx.fa, u = (fa + 1) % 3, v = (fa + 2) % 3;
+    // face-local index: x fastest where the face contains x (coalesced rows)
+    const int fa = bx.fa, u = fa == 0 ? 1 : 0, v = fa == 2 ? 1 : 2;
     const int nu = bx.hi[u] - bx.lo[u] + 1;
     const int loc = gid - bx.start;
     int ijk[3];
