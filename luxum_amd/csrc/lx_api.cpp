@@ -511,6 +511,7 @@ void settle_parity(lx_sim *s, int n) {
     if (!s->fused || !(n & 1)) return;
     std::swap(s->F, s->F_alt);
     std::swap(s->psi, s->psi_alt);
+    s->gen++;
     drop_graph(s);
 }
 
@@ -566,6 +567,7 @@ lx_status ensure_probe_cap(lx_sim *s, int need) {
     s->prec = (double *)q;
     s->pcap = cap;
     s->bytes += (int64_t)sizeof(double) * cap * s->np;
+    s->gen++;
     drop_graph(s);
     return LX_OK;
 }
@@ -613,6 +615,7 @@ lx_status ensure_fused_buffers(lx_sim *s) {
     if ((st = dalloc(s, &pp, psi_total(s) * sizeof(float)))) return st;
     launch_zero_bytes(pp, psi_total(s) * sizeof(float), s->cs);
     s->psi_alt = (float *)pp;
+    s->gen++;
     HIP_TRY(hipStreamSynchronize(s->cs));
     return LX_OK;
 }
@@ -634,6 +637,7 @@ void release_fused_buffers(lx_sim *s) {
     s->F_alt = nullptr;
     s->psi_alt = nullptr;
     s->fused = 0;
+    s->gen++;
 }
 
 template Grid<float> make_grid<float>(const lx_sim *s);
@@ -925,6 +929,7 @@ lx_status lx_sim_set_probes(lx_sim *s, const int32_t *cijk, int32_t n) {
     }
     HIP_TRY(hipStreamSynchronize(s->cs));
     drop_graph(s);
+    s->gen++;
     if (s->pidx) { (void)hipFree(s->pidx); s->pidx = nullptr; }
     if (s->pcomp) { (void)hipFree(s->pcomp); s->pcomp = nullptr; }
     if (s->prec) { (void)hipFree(s->prec); s->prec = nullptr; s->bytes -= (int64_t)sizeof(double) * s->pcap * s->np; }
@@ -1179,6 +1184,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
             } else if (s->F_alt && !s->planar) {
                 release_fused_buffers(s);
             }
+            s->gen++;
             drop_graph(s);
             if (kz_out) *kz_out = s->kz;
             return LX_OK;
@@ -1257,6 +1263,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         if (!s->fused) release_fused_buffers(s);
     }
     drop_graph(s);
+    s->gen++;
     lx_status st = reset_state(s);
     if (st) return st;
     {

@@ -169,6 +169,7 @@ lx_status ensure_alt(lx_sim *s) {
     if (st) return st;
     launch_zero_bytes(p, psi_total(s) * sizeof(float), s->cs);
     s->psi_alt = (float *)p;
+    s->gen++;
     HIP_TRY(hipStreamSynchronize(s->cs));
     return LX_OK;
 }
@@ -378,6 +379,7 @@ lx_status rebuild(lx_batch *b) {
             g.naux = (int)aux.size();
         }
     }
+    for (lx_sim *s : b->sims) s->batch_gen = s->gen;
     b->dirty = false;
     return LX_OK;
 }
@@ -540,12 +542,13 @@ lx_status lx_batch_step(lx_batch *b, int32_t nsteps) {
     if (!b || nsteps < 0) return fail(LX_EINVAL, "lx_batch_step: bad arguments");
     if (nsteps == 0 || b->sims.empty()) return LX_OK;
     HIP_TRY(hipSetDevice(b->ctx->device));
-    // probe buffers must hold the new steps before the descriptors are built
+    // probe buffers must hold the new steps before the descriptors are built;
+    // descriptors are rebuilt when a member's buffers or settings changed
+    // since the last rebuild (its own steps, probes, autotuning)
     for (lx_sim *s : b->sims) {
-        const int cap = s->pcap;
         lx_status st = ensure_probe_cap(s, s->steps + nsteps);
         if (st) return st;
-        if (s->pcap != cap) b->dirty = true;
+        if (s->gen != s->batch_gen) b->dirty = true;
     }
     if (b->dirty) {
         lx_status st = rebuild(b);

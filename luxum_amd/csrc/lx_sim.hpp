@@ -34,6 +34,11 @@ struct lx_sim {
     // (batch_wait, set by the batch) only when there is such work
     hipEvent_t batch_wait = nullptr;
     bool cs_dirty = false;
+    // bumped whenever device buffers or step settings that a batch's
+    // descriptors copy change (probe buffers, the buffer pair's order, the
+    // second buffer set, the autotuned settings); a batch rebuilds its
+    // descriptors when a member's gen differs from the one it recorded
+    uint32_t gen = 0, batch_gen = 0;
     lx_ctx *ctx = nullptr;
     lx_params p{};
     int planar = 0, fp16 = 0, pml = 0;

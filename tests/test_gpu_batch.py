@@ -213,3 +213,37 @@ def test_batch_3d_members(gpu_ctx):
     say(f"{ws[0].name}: batch vs oracle fields {e:.2e} probes {pe:.2e}")
     assert e <= TOL32 and pe <= TOL32
     b.close()
+
+
+def test_batch_members_change_between_steps(gpu_ctx):
+    """Members change their device buffers between batch steps while staying
+    in the batch: a 3D member steps an odd count alone (its buffer pair swaps
+    order) and a planar member outgrows its probe buffer by stepping alone
+    and again inside a batch step.  The batch must pick up the new buffers."""
+    from luxum_amd.sim import Batch
+    w3 = configs.box3d(64, 40, 36, steps=30)
+    w2 = phc(160, 7, delay=1.0)
+    w2.p.num_timesteps = 40
+    ws = [w3, w2]
+    members = [make(w, gpu_ctx) for w in ws]
+    b = Batch(gpu_ctx)
+    for s in members:
+        b.add(s)
+    say("members change: batch 20, 3D alone 5, 2D alone 30, batch 41")
+    b.step(20)
+    members[0].step(5)
+    members[1].step(30)
+    b.step(41)
+    b.sync()
+    counts = [66, 91]
+    for w, s, n in zip(ws, members, counts):
+        ref = make(w, gpu_ctx)
+        ref.step(n)
+        ref.sync()
+        e = field_err(s, ref)
+        pe = rel_l2(s.probes(), ref.probes())
+        say(f"{w.name}: {n} steps, fields {e:.2e} probes {pe:.2e}")
+        assert s.probes().shape == ref.probes().shape
+        assert e <= 1e-6 and pe <= 1e-6
+        assert s.info().step == n
+    b.close()
