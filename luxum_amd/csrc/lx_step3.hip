@@ -302,12 +302,12 @@ __device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, 
     const unsigned rjp = (unsigned)(jp * px + i0) * S, ri = (unsigned)i0 * 4u, rx = (unsigned)xs * 4u;
     const unsigned re = (unsigned)(jr * px + i0);  // material ids (one byte per cell)
     // byte offsets of the edge cells x0+64V (rows j, j-1) and x0-1 (rows j,
-    // j+1; psi rows): equal in every lane, kept in vector registers like the
-    // others
+    // j+1; psi rows): equal in every lane, derived from the lane offsets so
+    // that they share their base registers
     const unsigned bn = rj + (unsigned)(64 * V - lane * V) * S, bnm = rjm + (unsigned)(64 * V - lane * V) * S;
     const unsigned bl = rj - (unsigned)(lane * V + 1) * S, blp = rjp - (unsigned)(lane * V + 1) * S;
     const unsigned ril = ri - (unsigned)(lane * V + 1) * 4u;
-    const unsigned rxl = (unsigned)xsl * 4u + (unsigned)(lane - lane);
+    const unsigned rxl = (unsigned)xsl * 4u;
 
     // z queue at k0 (row j: Ex, Ey, Ez; row j-1: Ey; cell x0-1: Ex) and H_new(j, k0-1)
     F ex0, ey0, ez0, eym0;
