@@ -149,6 +149,37 @@ def test_monitor_and_snapshot(gpu_ctx):
     assert np.array_equal(snap, g.field(LX_EZ))
 
 
+def test_3d_cpml_aux_2000_steps(gpu_ctx):
+    """Config 1's set-up at 128^3 over config 1's 2000 steps, FP32 and FP16
+    storage side by side against one FP64 oracle run.  The error grows as the
+    field decays (the pulse has left the grid; the cube rings down): FP32
+    stays <= 1e-5, FP16 <= 5e-2 on the fields and 1e-2 on the probes."""
+    from luxum_amd.sim import Simulation
+    w = configs.config1(n=128, steps=2000, pml=12, margin=10)
+    w16 = configs.config1(n=128, steps=2000, pml=12, margin=10, precision=LX_FP16)
+    gs = {}
+    for tag, ww in (("fp32", w), ("fp16", w16)):
+        g = Simulation(ww.p, gpu_ctx)
+        g.set_materials(ww.ids, ww.eps_r)
+        g.set_probes(ww.probes)
+        gs[tag] = g
+    o = OracleSim(w.p)
+    o.set_materials(w.ids, w.eps_r)
+    o.set_probes(w.probes)
+    for chunk in range(4):
+        for g in gs.values():
+            g.step(500)
+            g.sync()
+        o.step(500)
+        say(f"128^3 step {(chunk + 1) * 500}: " + " ".join(
+            f"{tag} fields {max(field_errors(g, o).values()):.2e} probes {rel_l2(g.probes(), o.probes()):.2e}"
+            for tag, g in gs.items()))
+    e32, p32 = max(field_errors(gs["fp32"], o).values()), rel_l2(gs["fp32"].probes(), o.probes())
+    e16, p16 = max(field_errors(gs["fp16"], o).values()), rel_l2(gs["fp16"].probes(), o.probes())
+    assert e32 <= TOL32 and p32 <= TOL32
+    assert e16 <= 5e-2 and p16 <= 1e-2
+
+
 def test_fp16_bound(gpu_ctx):
     w = configs.config1(n=48, steps=300, pml=8, margin=6, precision=LX_FP16)
     g, o = run_pair(gpu_ctx, w.p, 300, w.ids, w.eps_r, w.probes)
