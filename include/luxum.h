@@ -128,7 +128,8 @@ typedef struct lx_sim_info {
     int64_t core_bytes;         /* fused step on a CPML grid: algorithmic bytes of its interior-box launch
                                    (rows and planes outside the y and z slabs); else bytes_per_step */
     int32_t cells_per_lane;     /* fused step: x cells per lane (1, 2 or 4, autotuned); else 1 */
-    int32_t reserved;
+    int32_t shell_kz;           /* fused step on a CPML grid: z planes per block of the shell launch
+                                   (autotuned); else kz */
 } lx_sim_info;
 
 /* ---- parameters ------------------------------------------------------- */

@@ -58,7 +58,7 @@ class SimInfo(C.Structure):
         ("waves", C.c_int32),
         ("core_bytes", C.c_int64),
         ("cells_per_lane", C.c_int32),
-        ("reserved", C.c_int32),
+        ("shell_kz", C.c_int32),
     ]
 
 

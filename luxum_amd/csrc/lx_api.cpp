@@ -72,6 +72,7 @@ Grid<T> make_grid(const lx_sim *s) {
     }
     g.pml = s->pml;
     g.kz = s->kz;
+    g.kzs = s->kzs;
     g.ntx = (g.nx + 255) / 256;
     if (s->planar) {
         g.nty = (g.nz + kBY - 1) / kBY;
@@ -845,6 +846,8 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
                 s->vec = 1;
                 if ((f == '4' || (f == 'd' && s->fp16)) && vec_ok(s, 4)) s->vec = 4;
                 else if ((f == '2' || (f == 'd' && s->fp16)) && vec_ok(s, 2)) s->vec = 2;
+                const char *sk = std::getenv("LX_SHELL_KZ");   // z-block depth of the CPML shell launch
+                if (sk && *sk) s->kzs = std::max(0, std::atoi(sk));
             } else if (st == LX_ENOMEM && f == 'd') {
                 (void)hipGetLastError();
                 release_fused_buffers(s);
@@ -1132,7 +1135,7 @@ lx_status lx_sim_get_info(lx_sim *s, lx_sim_info *info) {
     info->waves = s->fused ? s->nw : 4;
     info->core_bytes = info->bytes_per_step;
     info->cells_per_lane = s->fused ? s->vec : 1;
-    info->reserved = 0;
+    info->shell_kz = s->fused && s->pml && s->kzs > 0 ? s->kzs : s->kz;
     if (s->fused && s->pml) {
         int rows = 0, planes = 0;
         step3_core_extent(s->N[1], s->N[2], s->kz, s->slo, s->shi, &rows, &planes);
@@ -1168,7 +1171,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
     member_enter(s);
     using Key = std::tuple<int, int, int, int, int, int, int>;
     static std::mutex mu;
-    static std::map<Key, std::array<int, 4>> cache;   // (kz, nw, fused, vec)
+    static std::map<Key, std::array<int, 5>> cache;   // (kz, nw, fused, vec, shell kz)
     const Key key{s->N[0], s->N[1], s->N[2], s->fp16, s->pml, s->planar, 0};
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -1177,6 +1180,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
             s->kz = it->second[0];
             s->nw = it->second[1];
             s->vec = it->second[3];
+            s->kzs = it->second[4];
             if (it->second[2] && fused_ok(s)) {
                 lx_status st = ensure_fused_buffers(s);
                 if (st) return st;
@@ -1194,72 +1198,103 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
+        // milliseconds per step of the current settings (4 steps, second of
+        // two repetitions)
+        auto time_steps = [&](float *out) -> lx_status {
+            for (int rep = 0; rep < 2; ++rep) {
+                HIP_TRY(hipEventRecord(e0, s->cs));
+                for (int q = 0; q < 4; ++q) {
+                    if (s->fp16) {
+                        Grid<__half> g = make_grid<__half>(s);
+                        if (s->fused) {
+                            g.Fn = (__half *)s->F_alt;
+                            g.psin = s->psi_alt;
+                            launch_step3(g, s->nw, s->vec, 0, s->cs);
+                            launch_step3(g, s->nw, s->vec, 1, s->cs);
+                        } else {
+                            launch_bulk(g, 0, 0, s->cs);
+                            launch_bulk(g, 0, 1, s->cs);
+                        }
+                    } else {
+                        Grid<float> g = make_grid<float>(s);
+                        if (s->fused) {
+                            g.Fn = (float *)s->F_alt;
+                            g.psin = s->psi_alt;
+                            launch_step3(g, s->nw, s->vec, 0, s->cs);
+                            launch_step3(g, s->nw, s->vec, 1, s->cs);
+                        } else {
+                            launch_bulk(g, 0, 0, s->cs);
+                            launch_bulk(g, 0, 1, s->cs);
+                        }
+                    }
+                }
+                HIP_TRY(hipEventRecord(e1, s->cs));
+                HIP_TRY(hipEventSynchronize(e1));
+                HIP_TRY(hipEventElapsedTime(out, e0, e1));
+            }
+            *out /= 4;
+            return check_launch();
+        };
+        const bool verbose = std::getenv("LX_VERBOSE") != nullptr;
         float best = 1e30f;
         int best_kz = s->kz, best_nw = s->nw, best_fused = s->fused, best_vec = s->vec;
         // candidates: the half-step pair (nw = 0) and the fused step with 8 or
-        // 4 wavefronts per block and 1 or 2 cells per lane (its second buffer
-        // set is kept only if it wins)
+        // 4 wavefronts per block and 1, 2 or 4 cells per lane (its second
+        // buffer set is kept only if it wins); the CPML shell first takes the
+        // interior's z-block depth
         const int can_fuse = fused_ok(s) && ensure_fused_buffers(s) == LX_OK;
+        s->kzs = 0;
         struct Cand { int nw, vec; };
         for (Cand c : {Cand{0, 1}, Cand{8, 1}, Cand{4, 1}, Cand{4, 2}, Cand{8, 2}, Cand{4, 4}}) {
-            const int nw = c.nw;
-            if (nw && !can_fuse) continue;
+            if (c.nw && !can_fuse) continue;
             if (!vec_ok(s, c.vec)) continue;
-            s->fused = nw != 0;
+            s->fused = c.nw != 0;
+            if (c.nw) s->nw = c.nw;
+            s->vec = c.vec;
             for (int kz : {4, 5, 6, 8, 12, 16, 32, 64}) {
                 if (kz > s->N[2] && kz != 4) continue;
                 s->kz = kz;
-                if (nw) s->nw = nw;
-                for (int rep = 0; rep < 2; ++rep) {
-                    HIP_TRY(hipEventRecord(e0, s->cs));
-                    for (int q = 0; q < 4; ++q) {
-                        if (s->fp16) {
-                            Grid<__half> g = make_grid<__half>(s);
-                            if (s->fused) {
-                                g.Fn = (__half *)s->F_alt;
-                                g.psin = s->psi_alt;
-                                launch_step3(g, nw, c.vec, 0, s->cs);
-                                launch_step3(g, nw, c.vec, 1, s->cs);
-                            } else {
-                                launch_bulk(g, 0, 0, s->cs);
-                                launch_bulk(g, 0, 1, s->cs);
-                            }
-                        } else {
-                            Grid<float> g = make_grid<float>(s);
-                            if (s->fused) {
-                                g.Fn = (float *)s->F_alt;
-                                g.psin = s->psi_alt;
-                                launch_step3(g, nw, c.vec, 0, s->cs);
-                                launch_step3(g, nw, c.vec, 1, s->cs);
-                            } else {
-                                launch_bulk(g, 0, 0, s->cs);
-                                launch_bulk(g, 0, 1, s->cs);
-                            }
-                        }
-                    }
-                    HIP_TRY(hipEventRecord(e1, s->cs));
-                    HIP_TRY(hipEventSynchronize(e1));
-                    float ms = 0;
-                    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                    if (rep == 1 && ms < best) {
-                        best = ms;
-                        best_kz = kz;
-                        best_fused = s->fused;
-                        if (nw) best_nw = nw;
-                        best_vec = c.vec;
-                    }
-                    if (rep == 1 && std::getenv("LX_VERBOSE"))
-                        std::fprintf(stderr, "lx_sim_autotune: %s nw=%d vec=%d kz=%d  %.4f ms per step\n",
-                                     s->fused ? "fused" : "half-step", nw, c.vec, kz, ms / 4);
+                float ms = 0;
+                lx_status st = time_steps(&ms);
+                if (st) return st;
+                if (ms < best) {
+                    best = ms;
+                    best_kz = kz;
+                    best_fused = s->fused;
+                    best_nw = s->nw;
+                    best_vec = c.vec;
                 }
+                if (verbose)
+                    std::fprintf(stderr, "lx_sim_autotune: %s nw=%d vec=%d kz=%d  %.4f ms per step\n",
+                                 s->fused ? "fused" : "half-step", c.nw, c.vec, kz, ms);
             }
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         s->kz = best_kz;
         s->nw = best_nw;
         s->vec = best_vec;
         s->fused = best_fused;
+        // the CPML shell's own z-block depth (its unit count and plane
+        // overhead differ from the interior's)
+        int best_kzs = 0;
+        if (s->fused && s->pml) {
+            for (int kzs : {4, 8, 12, 16}) {
+                if (kzs == s->kz || kzs > s->N[2]) continue;
+                s->kzs = kzs;
+                float ms = 0;
+                lx_status st = time_steps(&ms);
+                if (st) return st;
+                if (ms < best) {
+                    best = ms;
+                    best_kzs = kzs;
+                }
+                if (verbose)
+                    std::fprintf(stderr, "lx_sim_autotune: fused nw=%d vec=%d kz=%d shell kz=%d  %.4f ms per step\n",
+                                 s->nw, s->vec, s->kz, kzs, ms);
+            }
+        }
+        s->kzs = best_kzs;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
         if (!s->fused) release_fused_buffers(s);
     }
     drop_graph(s);
@@ -1268,7 +1303,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
     if (st) return st;
     {
         std::lock_guard<std::mutex> lk(mu);
-        cache[key] = {s->kz, s->nw, s->fused, s->vec};
+        cache[key] = {s->kz, s->nw, s->fused, s->vec, s->kzs};
     }
     if (kz_out) *kz_out = s->kz;
     return LX_OK;

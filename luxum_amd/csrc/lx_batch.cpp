@@ -201,7 +201,8 @@ lx_status build_group3(lx_batch *b, Group3 &g) {
         d.g.psi = s->psi;
         d.g.psin = s->psi_alt;
         if (!s->fused) d.g.kz = 8;   // the half-step kernels' z-block depth does not apply
-        step3_unit_maps(s->N[0], s->N[1], s->N[2], d.g.kz, s->pml, s->slo, s->shi, g.vec, d.um, d.nunits);
+        step3_unit_maps(s->N[0], s->N[1], s->N[2], d.g.kz, s->fused ? s->kzs : 0, s->pml, s->slo, s->shi, g.vec,
+                        d.um, d.nunits);
         d.tfH = s->tfH.dev;
         d.tfE = s->tfE.dev;
         d.ntfH = s->tfH.n;

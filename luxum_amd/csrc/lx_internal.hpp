@@ -118,6 +118,7 @@ struct Grid {
     int slo[3], shi[3], sw[3];
     int pml;
     int kz;                   // z planes per block (3D) / rows per block (2D)
+    int kzs;                  // fused 3D step: z planes per block of the CPML shell (0: kz)
     int ntx, nty, ntz, ntiles;
     int *counter;             // counter[0] = steps completed before this launch sequence
     T *Fn;                    // fused 3D step: fields written (F is read)
@@ -152,15 +153,46 @@ void launch_step3(const Grid<T> &g, int nw, int vec, int part, hipStream_t s);
 // Rows and planes of the part-0 box of a CPML grid.
 void step3_core_extent(int ny, int nz, int kz, const int slo[3], const int shi[3], int *rows, int *planes);
 
-// The units of one fused-step launch, all x chunks of each: the rows
-// [jlo, jhi) x the z blocks [tlo, thi) (mode 0), or every unit outside that
-// box (mode 1).
+// The units of one fused-step launch, all x chunks of each.  Mode 0: the
+// rows [jlo, jhi) x the z blocks [tlo, thi) of depth kz (the interior box).
+// Mode 1: every cell outside that box, in z blocks of depth kz: the rows
+// outside [jlo, jhi) over the whole z range (ntz blocks), then the rows inside
+// over [0, zlo) (nlo blocks) and [zhi, nz).
 struct UnitMap {
-    int mode, jlo, jhi, tlo, thi, ntz;
+    int mode, jlo, jhi, tlo, thi, ntz, kz, zlo, zhi, nlo;
 };
+// Row jr and planes [k0, k1) of the rest-th (row, z block) pair of a map.
+__host__ __device__ inline void unit_span(const UnitMap &um, int ny, int nz, int rest, int &jr, int &k0, int &k1) {
+    if (um.mode == 0) {
+        const int nr = um.jhi - um.jlo;
+        jr = um.jlo + rest % nr;
+        k0 = (um.tlo + rest / nr) * um.kz;
+        k1 = min(k0 + um.kz, nz);
+        return;
+    }
+    const int nro = um.jlo + (ny - um.jhi);    // rows outside the box: every z block
+    if (rest < nro * um.ntz) {
+        const int r = rest % nro;
+        jr = r < um.jlo ? r : um.jhi + (r - um.jlo);
+        k0 = (rest / nro) * um.kz;
+        k1 = min(k0 + um.kz, nz);
+        return;
+    }
+    rest -= nro * um.ntz;                      // rows inside: z blocks below and above the box
+    const int nr = um.jhi - um.jlo, t = rest / nr;
+    jr = um.jlo + rest % nr;
+    if (t < um.nlo) {
+        k0 = t * um.kz;
+        k1 = min(k0 + um.kz, um.zlo);
+    } else {
+        k0 = um.zhi + (t - um.nlo) * um.kz;
+        k1 = min(k0 + um.kz, nz);
+    }
+}
 // Unit maps and unit counts of the two parts of a grid's fused step (part 1
-// is empty without CPML) at vec cells per lane.
-void step3_unit_maps(int nx, int ny, int nz, int kz, int pml, const int slo[3], const int shi[3], int vec,
+// is empty without CPML) at vec cells per lane: interior box in z blocks of
+// kz planes, shell in blocks of kzs (0: kz).
+void step3_unit_maps(int nx, int ny, int nz, int kz, int kzs, int pml, const int slo[3], const int shi[3], int vec,
                      UnitMap um[2], int nunits[2]);
 
 // H TF/SF corrections of the step applied to H of the read buffer before the

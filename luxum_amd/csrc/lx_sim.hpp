@@ -64,6 +64,7 @@ struct lx_sim {
     int64_t slab_cells[3] = {0, 0, 0};
     lx::TfList tfH{}, tfE{};
     int kz = 32;
+    int kzs = 0;                // fused step: z-block depth of the CPML shell launch (0: kz)
     int fused = 0;              // 3D: one fused launch per step over ping-pong buffers (F/psi and F_alt/psi_alt)
     int nw = 8;                 // fused step: wavefronts per block
     int vec = 1;                // fused step: x cells per lane (1, 2 or 4)

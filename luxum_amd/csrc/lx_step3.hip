@@ -227,25 +227,8 @@ __device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, 
     const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
     const int nch = (nx + 64 * V - 1) / (64 * V);
     const int ch = unit % nch;
-    int rest = unit / nch, jr, tz;
-    if (um.mode == 0) {
-        const int nr = um.jhi - um.jlo;
-        jr = um.jlo + rest % nr;
-        tz = um.tlo + rest / nr;
-    } else {
-        const int nro = um.jlo + (ny - um.jhi);    // rows outside the box: every z block
-        if (rest < nro * um.ntz) {
-            const int r = rest % nro;
-            jr = r < um.jlo ? r : um.jhi + (r - um.jlo);
-            tz = rest / nro;
-        } else {                                   // rows inside: z blocks outside the box
-            rest -= nro * um.ntz;
-            const int nr = um.jhi - um.jlo, t = rest / nr;
-            jr = um.jlo + rest % nr;
-            tz = t < um.tlo ? t : um.thi + (t - um.tlo);
-        }
-    }
-    const int k0 = tz * g.kz, k1 = min(k0 + g.kz, nz);
+    int jr, k0, k1;
+    unit_span(um, ny, nz, unit / nch, jr, k0, k1);
     const int x0 = ch * 64 * V;
     const long long plane = g.plane, fs = g.fstride;
     const int tn = g.tn;
@@ -609,7 +592,10 @@ static void launch_units(const Grid<T> &g, int nw, const UnitMap &um, int nunits
 // outside the z slabs.
 static UnitMap core_box(int ny, int nz, int kz, const int slo[3], const int shi[3]) {
     const int ntz = (nz + kz - 1) / kz;
-    UnitMap um{0, 0, 0, 0, 0, ntz};
+    UnitMap um{};
+    um.mode = 0;
+    um.ntz = ntz;
+    um.kz = kz;
     um.jlo = slo[1] > 0 ? slo[1] + 1 : 0;
     um.jhi = std::max(um.jlo, shi[1]);
     um.tlo = std::min(slo[2] > 0 ? (slo[2] + 1 + kz - 1) / kz : 0, ntz);
@@ -623,29 +609,43 @@ void step3_core_extent(int ny, int nz, int kz, const int slo[3], const int shi[3
     *planes = std::min(um.thi * kz, nz) - um.tlo * kz;
 }
 
-void step3_unit_maps(int nx, int ny, int nz, int kz, int pml, const int slo[3], const int shi[3], int vec,
+void step3_unit_maps(int nx, int ny, int nz, int kz, int kzs, int pml, const int slo[3], const int shi[3], int vec,
                      UnitMap um[2], int nunits[2]) {
     const int ntz = (nz + kz - 1) / kz;
     const int nch = (nx + 64 * vec - 1) / (64 * vec);   // one wavefront per (x chunk, row, z block)
     if (!pml) {
-        um[0] = UnitMap{0, 0, ny, 0, ntz, ntz};
+        um[0] = UnitMap{};
+        um[0].jhi = ny;
+        um[0].thi = um[0].ntz = ntz;
+        um[0].kz = kz;
         um[1] = um[0];
         nunits[0] = nch * ny * ntz;
         nunits[1] = 0;
         return;
     }
     um[0] = core_box(ny, nz, kz, slo, shi);
-    um[1] = um[0];
-    um[1].mode = 1;
     nunits[0] = nch * (um[0].jhi - um[0].jlo) * (um[0].thi - um[0].tlo);
-    nunits[1] = nch * ny * ntz - nunits[0];
+    // the shell in blocks of its own depth: rows outside the box over the
+    // whole z range, rows inside below and above the box's planes
+    if (kzs <= 0) kzs = kz;
+    UnitMap &sh = um[1];
+    sh = um[0];
+    sh.mode = 1;
+    sh.kz = kzs;
+    sh.ntz = (nz + kzs - 1) / kzs;
+    sh.zlo = std::min(um[0].tlo * kz, nz);
+    sh.zhi = std::min(um[0].thi * kz, nz);
+    sh.nlo = (sh.zlo + kzs - 1) / kzs;
+    const int nhi = (nz - sh.zhi + kzs - 1) / kzs;
+    const int nro = um[0].jlo + (ny - um[0].jhi), nr = um[0].jhi - um[0].jlo;
+    nunits[1] = nch * (nro * sh.ntz + nr * (sh.nlo + nhi));
 }
 
 template <typename T, int V>
 static void launch_parts(const Grid<T> &g, int nw, int part, hipStream_t s) {
     UnitMap um[2];
     int nu[2];
-    step3_unit_maps(g.nx, g.ny, g.nz, g.kz, g.pml, g.slo, g.shi, V, um, nu);
+    step3_unit_maps(g.nx, g.ny, g.nz, g.kz, g.kzs, g.pml, g.slo, g.shi, V, um, nu);
     if (!g.pml) {
         if (part == 0) launch_units<T, 0, V>(g, nw, um[0], nu[0], s);
     } else if (part == 0) {

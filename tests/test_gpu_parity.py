@@ -248,6 +248,33 @@ def test_fused_step_matches_half_steps(gpu_ctx, monkeypatch):
     assert e <= 1e-6
 
 
+def test_fused_shell_block_depth(gpu_ctx, monkeypatch):
+    """The CPML shell launch in z blocks of its own depth (5, 11, 3 planes
+    against the interior's 8) tiles the grid with the interior box exactly:
+    fields and probes bit-identical to the shell at the interior's depth."""
+    from luxum_amd.sim import Simulation
+    w = configs.box3d(72, 44, 50, pml=9, margin=4)
+
+    def run(shell_kz):
+        monkeypatch.setenv("LX_FUSED", "1")
+        monkeypatch.setenv("LX_SHELL_KZ", str(shell_kz))
+        s = Simulation(w.p, gpu_ctx)
+        s.set_materials(w.ids, w.eps_r)
+        s.set_probes(w.probes)
+        assert s.info().shell_kz == (shell_kz or 8)
+        s.step(57)
+        s.sync()
+        return s
+
+    ref = run(0)
+    for d in (5, 11, 3):
+        say(f"case shell kz {d}")
+        s = run(d)
+        for c in w.p.components():
+            assert np.array_equal(s.field(c), ref.field(c)), f"shell kz {d} component {c}"
+        assert np.array_equal(s.probes(), ref.probes())
+
+
 def test_fused_two_cells_per_lane(gpu_ctx, monkeypatch):
     """The fused step with two x cells per lane (128-cell chunks) against
     the half-step kernels (1e-6) and the FP64 oracle (1e-5): two chunks per
