@@ -179,12 +179,18 @@ lx_status ensure_alt(lx_sim *s) {
 template <typename T>
 lx_status build_group3(lx_batch *b, Group3 &g) {
     const size_t n = g.sims.size();
+    // the widest cells-per-lane width every member's row pitch allows (FP16:
+    // 4 or 2; FP32: 1, the faster one at every size measured)
     g.vec = 1;
-    if (g.fp16) {
-        g.vec = 2;
-        for (const lx_sim *s : g.sims)
-            if (!vec_ok(s, 2)) g.vec = 1;
-    }
+    if (g.fp16)
+        for (int v : {4, 2}) {
+            bool ok = true;
+            for (const lx_sim *s : g.sims) ok = ok && vec_ok(s, v);
+            if (ok) {
+                g.vec = v;
+                break;
+            }
+        }
     std::vector<int> order(n);
     for (size_t q = 0; q < n; ++q) order[q] = (int)q;
     auto cells = [&](int q) { return (int64_t)g.sims[q]->N[0] * g.sims[q]->N[1] * g.sims[q]->N[2]; };

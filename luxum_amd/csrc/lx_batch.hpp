@@ -72,7 +72,8 @@ struct B3Sim {
 constexpr int kB3Waves = 4;    // wavefronts per block of the batched fused step
 
 // Batched fused step, part 0 (interior boxes, whole grids without CPML) or 1
-// (CPML shells); bmap[b] = (member, member-local block) of launch block b.
+// (CPML shells); bmap[b] = (member, member-local block) of launch block b;
+// vec cells per lane (4 with FP16 storage only).
 template <typename T>
 void launch_step3_batch(const B3Sim<T> *sims, const int2 *bmap, int nblocks, int part, int par, int vec,
                         hipStream_t s);

@@ -675,6 +675,15 @@ void launch_step3_batch(const B3Sim<T> *sims, const int2 *bmap, int nblocks, int
                         hipStream_t s) {
     if (nblocks <= 0) return;
     constexpr int NW = kB3Waves;
+    if constexpr (std::is_same<T, __half>::value) {
+        if (vec == 4) {
+            if (part == 0)
+                hipLaunchKernelGGL((k_step3_batch<T, 1, NW, 4>), dim3(nblocks), dim3(64 * NW), 0, s, sims, bmap, nblocks, 0, par);
+            else
+                hipLaunchKernelGGL((k_step3_batch<T, 7, NW, 4>), dim3(nblocks), dim3(64 * NW), 0, s, sims, bmap, nblocks, 1, par);
+            return;
+        }
+    }
     if (part == 0) {
         if (vec == 2)
             hipLaunchKernelGGL((k_step3_batch<T, 1, NW, 2>), dim3(nblocks), dim3(64 * NW), 0, s, sims, bmap, nblocks, 0, par);

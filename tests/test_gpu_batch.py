@@ -247,3 +247,28 @@ def test_batch_members_change_between_steps(gpu_ctx):
         assert e <= 1e-6 and pe <= 1e-6
         assert s.info().step == n
     b.close()
+
+
+def test_batch_3d_fp16_four_cells(gpu_ctx):
+    """FP16 3D members whose row pitch allows four cells per lane advance in
+    the batched fused step at that width: bit-identical to their single
+    runs (which take the same width by default)."""
+    from luxum_amd.sim import Batch
+    ws = [configs.box3d(200, 36, 40, precision=LX_FP16), configs.box3d(240, 32, 36, precision=LX_FP16)]
+    members = [make(w, gpu_ctx) for w in ws]
+    assert all(s.info().cells_per_lane == 4 for s in members)
+    b = Batch(gpu_ctx)
+    for s in members:
+        b.add(s)
+    say("batch fp16 four cells: 2 members, batch 45")
+    b.step(45)
+    b.sync()
+    for w, s in zip(ws, members):
+        ref = make(w, gpu_ctx)
+        ref.step(45)
+        ref.sync()
+        e = field_err(s, ref)
+        pe = rel_l2(s.probes(), ref.probes())
+        say(f"{w.name}: fields {e:.2e} probes {pe:.2e}")
+        assert e <= 1e-6 and pe <= 1e-6
+    b.close()
