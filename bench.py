@@ -12,13 +12,16 @@ dominant kernel (algorithmic bytes per launch / average launch duration,
 timed with HIP events on the simulation's stream) and a CPU baseline (the
 FP64 reference-structured oracle on a bounded sample of the same workload).
 
-Two more objects ride on the same line:
+More objects ride on the same line:
   "batched": 256 seeded 2D photonic-crystal simulations of sizes
      {256..2048}^2 (FP32, CPML 16, TF/SF plane wave), 3000 steps, advanced
      together by one batch (temporally tiled kernel over a flattened tile
      list, hipGraph-captured chunk pairs), against the same simulations run
      one at a time with the single-simulation kernels: solves/s of both, and
      the relative L2 between their final fields on a sample.
+  "snapshots": the main line's simulation with Ez HDF5 snapshots every 50
+     steps and with the reference's |E|^2-every-3-steps output, against the
+     same steps without output (the PCIe- and disk-inclusive cost).
   "serving": a seeded Poisson request stream (80% 2D 256^2-2048^2, 20% 3D
      64^3-320^3 dielectric scatterers, 1000-5000 steps, FP32/FP16 storage,
      200 GB admission cap) through the continuous-batching server and through
@@ -183,6 +186,82 @@ def serve_section(args):
     return out
 
 
+def snapshot_section(args, w, ctx, sim):
+    """Snapshot output beside the time loop, on the main line's simulation:
+    Ez HDF5 snapshots every --snap-every steps (gather kernel on the compute
+    stream, D2H on the side stream into pinned slots, writer thread), and the
+    reference's solve3d_pml cadence, |E|^2 every 3 steps.  Reported against the
+    same steps without output; the files go to a temporary directory."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+    from luxum_amd.params import LX_EZ
+    from luxum_amd.sim import LX_OUT_E2, LX_OUT_FIELD, Writer
+    n, every = args.snap_steps, args.snap_every
+    d = tempfile.mkdtemp(prefix="lx_snap_")
+    out = {"workload": f"main line, {n} steps, Ez snapshots every {every} steps (HDF5, FP64, the reference's "
+                       f"layout); |E|^2 every 3 steps over 30 steps (maxwell_three_dim_pml.cc:279 cadence)"}
+    try:
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(n)
+        sim.sync()
+        t_plain = time.perf_counter() - t0
+        # the device-to-host part alone: gather + D2H into pinned host memory
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        m = int(np.prod(sim.p.field_shape(LX_EZ)))
+        ptrs = [ctypes.c_void_p() for _ in range(2)]
+        for q in ptrs:
+            if hip.hipHostMalloc(ctypes.byref(q), ctypes.c_size_t(8 * m), 0) != 0:
+                raise RuntimeError("hipHostMalloc failed")
+        pinned = [np.ctypeslib.as_array(ctypes.cast(q, ctypes.POINTER(ctypes.c_double)), shape=(m,)) for q in ptrs]
+        t0 = time.perf_counter()
+        for q in range(n // every):
+            sim.step(every)
+            sim.snapshot_async(LX_EZ, pinned[q & 1])
+        sim.sync()
+        t_d2h = time.perf_counter() - t0
+        del pinned
+        for q in ptrs:
+            hip.hipHostFree(q)
+        wr = Writer(sim, os.path.join(d, "Ez_data.h5"), LX_OUT_FIELD, LX_EZ)
+        t0 = time.perf_counter()
+        for _ in range(n // every):
+            sim.step(every)
+            wr.write(sim.info().step)
+        wr.close()
+        sim.sync()
+        t_snap = time.perf_counter() - t0
+        nsnap = n // every
+        per = os.path.getsize(os.path.join(d, "Ez_data.h5")) / max(nsnap, 1)
+        out.update({"snapshots": nsnap, "bytes_per_snapshot": int(per),
+                    "ms_per_step_plain": round(t_plain * 1e3 / n, 4),
+                    "ms_per_step_with_d2h_only": round(t_d2h * 1e3 / (nsnap * every), 4),
+                    "ms_per_step_with_snapshots": round(t_snap * 1e3 / (nsnap * every), 4),
+                    "overhead_frac": round(t_snap / (nsnap * every) / (t_plain / n) - 1, 4)})
+        print(f"bench: snapshots every {every}: {out['ms_per_step_with_snapshots']} ms/step against "
+              f"{out['ms_per_step_plain']}", file=sys.stderr, flush=True)
+        wr = Writer(sim, os.path.join(d, "Ez_data_e2.h5"), LX_OUT_E2, 0)
+        t0 = time.perf_counter()
+        for _ in range(10):
+            sim.step(3)
+            wr.write(sim.info().step)
+        wr.close()
+        sim.sync()
+        t_e2 = time.perf_counter() - t0
+        per2 = os.path.getsize(os.path.join(d, "Ez_data_e2.h5")) / 10
+        out["e2_every_3"] = {"snapshots": 10, "bytes_per_snapshot": int(per2),
+                             "ms_per_step": round(t_e2 * 1e3 / 30, 4),
+                             "written_GBps": round(10 * per2 / t_e2 / 1e9, 3)}
+        print(f"bench: |E|^2 every 3 steps: {out['e2_every_3']['ms_per_step']} ms/step "
+              f"({out['e2_every_3']['written_GBps']} GB/s written)", file=sys.stderr, flush=True)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +285,9 @@ def main():
     ap.add_argument("--serve2d-requests", type=int, default=64)
     ap.add_argument("--serve2d-rate", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-snap", action="store_true", help="skip the snapshot-output section")
+    ap.add_argument("--snap-steps", type=int, default=400)
+    ap.add_argument("--snap-every", type=int, default=50)
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (PMC traffic calibration)")
     args = ap.parse_args()
@@ -302,6 +384,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": dbytes, "ms_per_launch": round(dms, 5)},
     }
+    if not args.no_snap and args.workload == "cube":
+        out["snapshots"] = snapshot_section(args, w, ctx, sim)
     sim.close()
     if not args.no_batch:
         out["batched"] = batch_section(args)

@@ -45,3 +45,15 @@ def test_named_dataset(tmp_path):
     assert np.array_equal(h5_read(path, "data"), a)
     with pytest.raises(LuxumError):
         h5_read(path, "t=0")
+
+
+def test_large_datasets_round_trip(tmp_path):
+    # snapshot-sized datasets appended to one file read back dataset by dataset
+    path = str(tmp_path / "Ez_data.h5")
+    rng = np.random.default_rng(5)
+    arrs = [rng.standard_normal((64, 80, 101)) for _ in range(3)]   # 4.1 MB each
+    arrs.append(rng.standard_normal((130, 128, 127)))               # 16.9 MB
+    for q, a in enumerate(arrs):
+        h5_write(path, 3 * q, a, append=q > 0)
+    for q, a in enumerate(arrs):
+        assert np.array_equal(h5_read(path, f"t={3 * q}"), a)
