@@ -615,31 +615,41 @@ __global__ void k_finish_batch(const B3Sim<T> *__restrict__ sims, int n, int par
     finish_block<T>(g, d->pr, n);
 }
 
-template <typename T>
-__global__ void k_gather(Grid<T> g, const T *F, int ex, int ey, int ez, double scale, double *dst) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int k = blockIdx.z;
-    if (i >= ex) return;
-    const double v = (double)ld1(F + (long long)k * g.plane + (long long)j * g.px + i) * scale;
-    dst[((long long)i * ey + j) * ez + k] = v;
-}
-
-// |E|^2 at equal storage indices into the Ez-shaped host layout, zero
-// outside i < nx-1, j < ny-1, k < nz-1 (maxwell_three_dim_pml.cc:264-272).
-template <typename T>
-__global__ void k_gather_e2(Grid<T> g, int ex, int ey, int ez, double *dst) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int k = blockIdx.z;
-    if (i >= ex) return;
-    double v = 0.0;
-    if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
-        const long long o = (long long)k * g.plane + (long long)j * g.px + i;
-        const double a = ld1(g.F + o), b = ld1(g.F + g.fstride + o), c = ld1(g.F + 2 * g.fstride + o);
-        v = c * c + b * b + a * a;
+// Device layout (k, j, i), i fastest, to the host layout (i, j, k), k
+// fastest, in FP64: a transpose of each (k, i) plane of row j through a
+// 64 x 64 LDS tile, so that the loads run along i and the stores along k.
+// E2: |E|^2 at equal storage indices into the Ez-shaped layout, zero outside
+// i < nx-1, j < ny-1, k < nz-1 (maxwell_three_dim_pml.cc:264-272); else one
+// component (F) times scale.  Block: 256 threads, tile (i0, j, k0).
+template <typename T, bool E2>
+__global__ __launch_bounds__(256) void k_gather(Grid<T> g, const T *F, int ex, int ey, int ez, double scale,
+                                               double *dst) {
+    __shared__ double tile[64][65];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = blockIdx.x * 64, j = blockIdx.y, k0 = blockIdx.z * 64;
+    const int i = i0 + lane;
+    for (int r = w; r < 64; r += 4) {
+        const int k = k0 + r;
+        double v = 0.0;
+        if (i < ex && k < ez) {
+            const long long o = (long long)k * g.plane + (long long)j * g.px + i;
+            if (E2) {
+                if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
+                    const double a = ld1(g.F + o), b = ld1(g.F + g.fstride + o), c = ld1(g.F + 2 * g.fstride + o);
+                    v = c * c + b * b + a * a;
+                }
+            } else {
+                v = (double)ld1(F + o) * scale;
+            }
+        }
+        tile[r][lane] = v;
     }
-    dst[((long long)i * ey + j) * ez + k] = v;
+    __syncthreads();
+    const int k = k0 + lane;
+    for (int r = w; r < 64; r += 4) {
+        const int ii = i0 + r;
+        if (ii < ex && k < ez) dst[((long long)ii * ey + j) * ez + k] = tile[lane][r];
+    }
 }
 
 __device__ __forceinline__ void atomic_max_pos(double *addr, double v) {
@@ -783,13 +793,13 @@ void launch_point_source(const Grid<T> &g, const PointSrc &ps, int toff, hipStre
 template <typename T>
 void launch_gather(const Grid<T> &g, int comp, int ex, int ey, int ez, double scale, double *dst, hipStream_t s) {
     const T *F = g.F + comp * g.fstride;
-    dim3 blk(256), grd((ex + 255) / 256, ey, ez);
-    hipLaunchKernelGGL(k_gather<T>, grd, blk, 0, s, g, F, ex, ey, ez, scale, dst);
+    dim3 blk(256), grd((ex + 63) / 64, ey, (ez + 63) / 64);
+    hipLaunchKernelGGL((k_gather<T, false>), grd, blk, 0, s, g, F, ex, ey, ez, scale, dst);
 }
 template <typename T>
 void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double *dst, hipStream_t s) {
-    dim3 blk(256), grd((ex + 255) / 256, ey, ez);
-    hipLaunchKernelGGL(k_gather_e2<T>, grd, blk, 0, s, g, ex, ey, ez, dst);
+    dim3 blk(256), grd((ex + 63) / 64, ey, (ez + 63) / 64);
+    hipLaunchKernelGGL((k_gather<T, true>), grd, blk, 0, s, g, (const T *)g.F, ex, ey, ez, 1.0, dst);
 }
 template <typename T>
 void launch_monitor(const Grid<T> &g, int planar, double *out, hipStream_t s) {
