@@ -214,11 +214,11 @@ __device__ __forceinline__ vf<V> curl2(const vf<V> &d1, const vf<V> &k1, bool s1
     return r;
 }
 
-// PM: CPML axes the launch handles (bit 0: x slab, 1: y, 2: z).  A CPML grid
-// runs two launches: the units whose rows and planes lie outside the y and
-// z slabs (PM = 1, x slab only, far fewer registers) and the rest (PM = 7).
 // One unit (x chunk, row, z block) of the step; cbs is the coefficient table
-// of the grid's materials in LDS.
+// of the grid's materials in LDS.  PM: CPML axes the launch handles (bit 0:
+// x slab, 1: y, 2: z).  A CPML grid runs two launches: the units whose rows
+// and planes lie outside the y and z slabs (PM = 1, x slab only, far fewer
+// registers) and the rest (PM = 7).
 template <typename T, int PM, int V>
 __device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, int unit, const float *cbs) {
     constexpr bool PML = PM != 0;

@@ -248,6 +248,26 @@ def test_fused_step_matches_half_steps(gpu_ctx, monkeypatch):
     assert e <= 1e-6
 
 
+def test_fused_odd_and_thin_grids(gpu_ctx):
+    """The default 3D path (the fused step where it applies) on uneven, thin and small grids
+    against the FP64 oracle: x extents off the 64-cell chunk, few rows or
+    planes (z blocks shorter than kz, slabs covering most of an axis), CPML
+    thinner than the slabs' 4-cell x alignment, aux TF/SF and probes."""
+    cases = [(33, 13, 12, 2, 3), (70, 12, 23, 2, 3), (129, 14, 12, 2, 3), (20, 30, 44, 3, 2)]
+    for nx, ny, nz, pml, margin in cases:
+        w = configs.box3d(nx, ny, nz, pml=pml, margin=margin, steps=160)
+        w.p.pw_delay = 1.5                       # pulse peak at the TF face by step ~110
+        yc, zc = (w.p.tf_lo[1] + w.p.tf_hi[1]) // 2, (w.p.tf_lo[2] + w.p.tf_hi[2]) // 2
+        probes = np.array([[LX_EZ, w.p.tf_lo[0] + 2, yc, zc], [LX_EZ, nx // 2, yc, zc],
+                           [LX_EY, nx // 2, yc, zc]], dtype=np.int32)   # inside the TF box
+        g, o = run_pair(gpu_ctx, w.p, 160, w.ids, w.eps_r, probes)
+        path = "fused" if g.info().fused_step else "half-step"   # fused unless an H face lies in a slab
+        worst = check_fields(g, o, TOL32, f"{path} {nx}x{ny}x{nz} pml {pml}")
+        e = rel_l2(g.probes(), o.probes())
+        say(f"{path} {nx}x{ny}x{nz}: fields {worst:.2e} probes {e:.2e}")
+        assert e <= TOL32
+
+
 def test_fused_shell_block_depth(gpu_ctx, monkeypatch):
     """The CPML shell launch in z blocks of its own depth (5, 11, 3 planes
     against the interior's 8) tiles the grid with the interior box exactly:
