@@ -19,6 +19,8 @@ More objects ride on the same line:
      list, hipGraph-captured chunk pairs), against the same simulations run
      one at a time with the single-simulation kernels: solves/s of both, and
      the relative L2 between their final fields on a sample.
+  "batched_3d": 32 seeded 3D scatterers of 64^3-128^3 advanced together by
+     the batched fused step, against the same simulations one at a time.
   "snapshots": the main line's simulation with Ez HDF5 snapshots every 50
      steps and with the reference's |E|^2-every-3-steps output, against the
      same steps without output (the PCIe- and disk-inclusive cost).
@@ -127,6 +129,67 @@ def batch_section(args):
             "rel_l2_vs_one_at_a_time": (num / den) ** 0.5 if den > 0 else None,
             "l2_sample_sims": len(sample)})
     return out
+
+
+def batch3d_section(args):
+    """Batched 3D solves/s (the batched fused step over a flattened block
+    list) against the same simulations one at a time."""
+    import numpy as np
+    from luxum_amd import configs
+    from luxum_amd.sim import Batch, Context, Simulation
+    ctx = Context(0)
+    rng = np.random.default_rng(21)
+    sizes = [int(n) for n in rng.choice([64, 80, 96, 112, 128], size=args.batch3d_sims)]
+    ws = [configs.scatterer3d(n, rng, steps=args.batch3d_steps) for n in sizes]
+    cells = sum(w.cells for w in ws)
+    print(f"bench: batched 3D {len(ws)} sims, {cells / 1e6:.1f} Mcells, {args.batch3d_steps} steps", file=sys.stderr,
+          flush=True)
+
+    def make(w):
+        s = Simulation(w.p, ctx)
+        s.set_materials(w.ids, w.eps_r)
+        s.set_probes(w.probes)
+        return s
+
+    sims = [make(w) for w in ws]
+    b = Batch(ctx)
+    for s in sims:
+        b.add(s)
+    b.step(8)   # descriptors, graphs
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(args.batch3d_steps)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    ref = {c: sims[0].field(c) for c in ws[0].p.components()}
+    b.close()
+    for s in sims:
+        s.close()
+    t_single = 0.0
+    num = den = 0.0
+    for q, w in enumerate(ws):
+        s = make(w)
+        s.step(8)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(args.batch3d_steps)
+        s.sync()
+        t_single += time.perf_counter() - t0
+        if q == 0:
+            for c, a in ref.items():
+                o = s.field(c)
+                num += float(np.sum((a - o) ** 2))
+                den += float(np.sum(o ** 2))
+        s.close()
+    n = len(ws)
+    return {"workload": f"{n} seeded 3D scatterers of config 4 (CPML 12, aux TF/SF, 8 probes), sizes "
+                        f"{sorted(set(sizes))}^3, FP32, {args.batch3d_steps} steps",
+            "sims": n, "cells": cells, "steps": args.batch3d_steps,
+            "solves_per_s": round(n / t_batch, 3), "gcell_per_s": round(cells * args.batch3d_steps / t_batch / 1e9, 2),
+            "one_at_a_time_solves_per_s": round(n / t_single, 3),
+            "one_at_a_time_gcell_per_s": round(cells * args.batch3d_steps / t_single / 1e9, 2),
+            "speedup": round(t_single / t_batch, 2),
+            "rel_l2_vs_one_at_a_time": (num / den) ** 0.5 if den > 0 else None}
 
 
 def _serve_stream(ctx, stream):
@@ -279,6 +342,8 @@ def main():
     ap.add_argument("--batch-sims", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3000)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch3d-sims", type=int, default=32)
+    ap.add_argument("--batch3d-steps", type=int, default=500)
     ap.add_argument("--no-single", action="store_true", help="skip the one-at-a-time baseline of the batch section")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -389,6 +454,7 @@ def main():
     sim.close()
     if not args.no_batch:
         out["batched"] = batch_section(args)
+        out["batched_3d"] = batch3d_section(args)
     if not args.no_serve:
         out["serving"] = serve_section(args)
     if not args.no_cpu:
