@@ -466,13 +466,13 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
         T *buf[2] = {(T *)s->F, (T *)s->F_alt};
         float *pbuf[2] = {s->psi, s->psi_alt};
         for (int q = 0; q < n; ++q) {
-            Launch l{q, s->use_aux, s->np > 0};
+            Launch l{q, s->use_aux, s->np > 0, s->np > 0 && !s->p.use_ps};
             Grid<T> gq = g;
             gq.F = buf[q & 1];
             gq.psi = pbuf[q & 1];
             gq.Fn = buf[(q + 1) & 1];
             gq.psin = pbuf[(q + 1) & 1];
-            if (s->p.use_ps) launch_point_source(gq, s->ps, q, s->cs);
+            if (s->p.use_ps) launch_point_source(gq, s->ps, pr, q, s->cs);
             if (corr) launch_tfsf_hpre(gq, s->tfH, s->inc, s->aux, pr, l, s->cs);
             // timed parts: [0,1] interior box, [2,3] CPML shell
             if (ev) (void)hipEventRecord((*ev)[4 * q + 0], s->cs);
@@ -492,8 +492,8 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
         return;
     }
     for (int q = 0; q < n; ++q) {
-        Launch l{q, s->use_aux, s->np > 0};
-        if (s->p.use_ps) launch_point_source(g, s->ps, q, s->cs);
+        Launch l{q, s->use_aux, s->np > 0, s->np > 0 && !s->p.use_ps};
+        if (s->p.use_ps) launch_point_source(g, s->ps, pr, q, s->cs);
         if (ev) (void)hipEventRecord((*ev)[4 * q + 0], s->cs);
         launch_bulk(g, s->planar, 0, s->cs);
         if (ev) (void)hipEventRecord((*ev)[4 * q + 1], s->cs);

@@ -122,7 +122,7 @@ __device__ inline void extra_block(const Grid<T> &g, const AuxLine &aux, const P
     if (IS_H) {
         if (l.aux)
             for (int m = threadIdx.x; m < aux.L - 1; m += blockDim.x) aux.h[m] = aux.h[m] - aux.S * (aux.e[m + 1] - aux.e[m]);
-        if (l.probes && t >= 1 && t - 1 < pr.cap)
+        if (l.eprobes && t >= 1 && t - 1 < pr.cap)
             for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
                 if (pr.comp[q] < 3)
                     pr.rec[(long long)(t - 1) * pr.n + q] = (double)ld1(g.F + pr.comp[q] * g.fstride + pr.idx[q]);

@@ -129,6 +129,8 @@ struct Launch {
     int toff;                 // step offset of this launch from counter[0]
     int aux;                  // advance the aux line in the extra block
     int probes;               // record probes in the extra block
+    int eprobes;              // the H launch records the E probes of the previous step (else the
+                              // point-source kernel does, before it changes its cell)
 };
 
 // Bulk half-step kernel: half 0 = H update, 1 = E update.
@@ -216,8 +218,10 @@ struct PointSrc {
     int omega;            // hard: int(omega)
     double dt, c, lambda0, fwhm, delay;
 };
+// Point source of step counter[0] + toff on E of g.F; first records the E
+// probes of the previous step (E as that step left it).
 template <typename T>
-void launch_point_source(const Grid<T> &g, const PointSrc &ps, int toff, hipStream_t s);
+void launch_point_source(const Grid<T> &g, const PointSrc &ps, const ProbeArgs &pr, int toff, hipStream_t s);
 
 // Gather one component into FP64 reference layout (C order, z fastest).
 template <typename T>

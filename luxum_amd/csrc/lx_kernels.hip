@@ -570,7 +570,7 @@ __global__ __launch_bounds__(256) void k_tfsf_batch(const B3Sim<T> *__restrict__
         g.F = g.Fn;
         g.psi = g.psin;
     }
-    const Launch l{toff, d->use_aux, d->probes};
+    const Launch l{toff, d->use_aux, d->probes, d->probes};   // members have no point source
     if (PRE) hpre_block<T>(g, d->tfH, d->ntfH, d->totH, d->inc, d->aux, d->pr, l, mb.y);
     else tfsf_block<T, false>(g, d->tfE, d->ntfE, d->totE, d->inc, d->aux, d->pr, l, mb.y);
 }
@@ -579,8 +579,16 @@ __global__ __launch_bounds__(256) void k_tfsf_batch(const B3Sim<T> *__restrict__
 // point source, sequence end, gather, monitor, fills
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_point_source(Grid<T> g, PointSrc ps, int toff) {
+__global__ void k_point_source(Grid<T> g, PointSrc ps, ProbeArgs pr, int toff) {
     const int t = g.counter[0] + toff;
+    // the E probes of step t-1 first: the source below changes one E cell
+    // before step t's update, and the probes record E as step t-1 left it
+    if (t >= 1 && t - 1 < pr.cap)
+        for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
+            if (pr.comp[q] < 3)
+                pr.rec[(long long)(t - 1) * pr.n + q] = (double)ld1(g.F + pr.comp[q] * g.fstride + pr.idx[q]);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     T *E = g.F + ps.comp * g.fstride;
     if (ps.kind == LX_SRC_HARD_CW) {
         const int prod = (int)((unsigned)ps.omega * (unsigned)t);
@@ -787,8 +795,8 @@ void launch_finish_batch(const B3Sim<T> *sims, int nsims, int n, int par_end, hi
     hipLaunchKernelGGL(k_finish_batch<T>, dim3(nsims), dim3(64), 0, s, sims, n, par_end);
 }
 template <typename T>
-void launch_point_source(const Grid<T> &g, const PointSrc &ps, int toff, hipStream_t s) {
-    hipLaunchKernelGGL(k_point_source<T>, dim3(1), dim3(1), 0, s, g, ps, toff);
+void launch_point_source(const Grid<T> &g, const PointSrc &ps, const ProbeArgs &pr, int toff, hipStream_t s) {
+    hipLaunchKernelGGL(k_point_source<T>, dim3(1), dim3(64), 0, s, g, ps, pr, toff);
 }
 template <typename T>
 void launch_gather(const Grid<T> &g, int comp, int ex, int ey, int ez, double scale, double *dst, hipStream_t s) {
@@ -831,7 +839,7 @@ void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
     template void launch_finish<T>(const Grid<T> &, const ProbeArgs &, int, hipStream_t);                    \
     template void launch_tfsf_hpre<T>(const Grid<T> &, const TfList &, const IncParams &, const AuxLine &,   \
                                       const ProbeArgs &, Launch, hipStream_t);                              \
-    template void launch_point_source<T>(const Grid<T> &, const PointSrc &, int, hipStream_t);               \
+    template void launch_point_source<T>(const Grid<T> &, const PointSrc &, const ProbeArgs &, int, hipStream_t); \
     template void launch_gather<T>(const Grid<T> &, int, int, int, int, double, double *, hipStream_t);      \
     template void launch_monitor<T>(const Grid<T> &, int, double *, hipStream_t);                            \
     template void launch_gather_e2<T>(const Grid<T> &, int, int, int, double *, hipStream_t);              \

@@ -180,6 +180,41 @@ def test_3d_cpml_aux_2000_steps(gpu_ctx):
     assert e16 <= 5e-2 and p16 <= 1e-2
 
 
+def test_phc_slab_scaled(gpu_ctx):
+    """Config 2's structure (square eps 12 rods in a slab, a removed rod row,
+    soft Ez dipole at a/lambda 0.3, CPML 16) at half the pitch (16, side 6)
+    on 128 x 128 x 80 with a 32-cell slab, FP32 and FP16 storage against one
+    FP64 oracle run; one probe on the source cell (its E series is recorded
+    before each step's source term).  FP32 <= 1e-5 (measured 8.4e-6 fields,
+    2.1e-7 probes); FP16 storage <= 6e-2 on the fields and 1e-2 on the probes
+    (measured 4.2e-2, 2.8e-3)."""
+    from luxum_amd.sim import Simulation
+    steps = 700
+    runs = {}
+    c = 64
+    probes = np.array([[LX_EZ, c + d, c, 40] for d in (0, 8, 16, 32)], dtype=np.int32)
+    for tag, prec in (("fp32", 0), ("fp16", LX_FP16)):
+        w = configs.config2(nxy=128, nz=80, steps=steps, precision=prec, pitch=16, side=6, slab=32)
+        w.p.src_delay = 2.5                       # smooth start, pulse peak near step 480
+        g = Simulation(w.p, gpu_ctx)
+        g.set_materials(w.ids, w.eps_r)
+        g.set_probes(probes)
+        g.step(steps)
+        g.sync()
+        runs[tag] = (w, g)
+    w, _ = runs["fp32"]
+    o = OracleSim(w.p)
+    o.set_materials(w.ids, w.eps_r)
+    o.set_probes(probes)
+    o.step(steps)
+    errs = {}
+    for tag, (_, g) in runs.items():
+        errs[tag] = (max(field_errors(g, o).values()), rel_l2(g.probes(), o.probes()))
+        say(f"phc slab 128x128x80 {tag}: fields {errs[tag][0]:.2e} probes {errs[tag][1]:.2e}")
+    assert errs["fp32"][0] <= TOL32 and errs["fp32"][1] <= TOL32
+    assert errs["fp16"][0] <= 6e-2 and errs["fp16"][1] <= 1e-2
+
+
 def test_fp16_bound(gpu_ctx):
     w = configs.config1(n=48, steps=300, pml=8, margin=6, precision=LX_FP16)
     g, o = run_pair(gpu_ctx, w.p, 300, w.ids, w.eps_r, w.probes)
