@@ -77,6 +77,29 @@ def test_3d_cpml_aux_cube(gpu_ctx):
     assert e <= TOL32
 
 
+def test_probes_on_tfsf_faces(gpu_ctx):
+    """Probes of every component on and beside the TF/SF faces (where the
+    fused step applies the H corrections before the step and the E ones after
+    it) against the oracle's end-of-step records."""
+    w = configs.config1(n=48, steps=260, pml=8, margin=6)
+    lo, hi, m = w.p.tf_lo[0], w.p.tf_hi[0], 24
+    probes = np.array([[LX_EZ, lo, m, m], [LX_EZ, hi, m, m], [LX_HY, lo - 1, m, m], [LX_HY, hi, m, m],
+                       [LX_HY, lo, m, m], [LX_EZ, lo - 1, m, m], [LX_EX, m, m, lo], [LX_EX, m, m, hi],
+                       [LX_HX, m, lo - 1, m], [LX_HX, m, hi, m], [LX_EY, lo, m, lo], [LX_HZ, lo - 1, lo - 1, m]],
+                      dtype=np.int32)
+    g, o = run_pair(gpu_ctx, w.p, 260, w.ids, w.eps_r, probes)
+    assert g.info().fused_step
+    pg, po = g.probes(), o.probes()
+    scale = np.abs(po).max()
+    for q in range(len(probes)):
+        d = np.abs(pg[:, q] - po[:, q]).max() / scale
+        say(f"probe {probes[q].tolist()}: max |diff| / max |probe| = {d:.2e}")
+        assert d <= 1e-5
+    e = rel_l2(pg, po)
+    say(f"tfsf-face probes: rel L2 {e:.2e}")
+    assert e <= TOL32
+
+
 def test_3d_plain_analytic_oblique(gpu_ctx):
     p = default_params(nx=40, ny=36, nz=34, use_pw=1, incident=LX_INC_ANALYTIC, off_pw=90.0,
                        pw_sigma=(20 * 5e-8) ** 2, tf_lo=(8, 7, 6), tf_hi=(30, 27, 26))
