@@ -345,14 +345,13 @@ __device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, 
         const T *Fk = g.F + kp;
         T *Fnk = g.Fn + kp;
         // ---------------- loads of the plane
-        F ex1 = vs<V>(0.f), ey1 = vs<V>(0.f), ez1 = vs<V>(0.f), eym1 = vs<V>(0.f);
+        // E of plane k+1 (the z queue); the grid's last plane reloads its own
+        // plane instead (nothing reads the values there), so no branch and no
+        // register clearing precede the loads
+        const long long kn = kin ? plane : 0;
+        const F ex1 = ldv<T, V>(Fk + kn, rj), ey1 = ldv<T, V>(Fk + kn + fs, rj);
+        const F ez1 = ldv<T, V>(Fk + kn + 2 * fs, rj), eym1 = ldv<T, V>(Fk + kn + fs, rjm);
         float exl1 = 0.f;
-        if (kin) {
-            ex1 = ldv<T, V>(Fk + plane, rj);
-            ey1 = ldv<T, V>(Fk + plane + fs, rj);
-            ez1 = ldv<T, V>(Fk + plane + 2 * fs, rj);
-            eym1 = ldv<T, V>(Fk + plane + fs, rjm);
-        }
         const F ezp = ldv<T, V>(Fk + 2 * fs, rjp), exq = ldv<T, V>(Fk, rjp);
         F hx = ldv<T, V>(Fk + 3 * fs, rj), hy = ldv<T, V>(Fk + 4 * fs, rj), hz = ldv<T, V>(Fk + 5 * fs, rj);
         F hxj = ldv<T, V>(Fk + 3 * fs, rjm), hzj = ldv<T, V>(Fk + 5 * fs, rjm);
@@ -377,7 +376,7 @@ __device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, 
         }
         float hyl = 0.f, hzl = 0.f, ezl = 0.f, eyl = 0.f, exlp = 0.f;
         if (lft) {
-            if (kin) exl1 = ldb(Fk + plane, bl);
+            exl1 = ldb(Fk + kn, bl);
             hyl = ldb(Fk + 4 * fs, bl);
             hzl = ldb(Fk + 5 * fs, bl);
             ezl = ldb(Fk + 2 * fs, bl);
@@ -388,9 +387,10 @@ __device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, 
         // psi: H of row j (p*), H of row j-1 (q*), E of row j (e*), H of cell
         // x0-1 (l*); row bases of the read (P*) and written (Q*) arrays, slot s
         // of axis a lying s*scells[a] further on
-        F py0 = vs<V>(0.f), py1 = vs<V>(0.f), pz0 = vs<V>(0.f), pz1 = vs<V>(0.f), px0 = vs<V>(0.f), px1 = vs<V>(0.f);
-        F qy0 = vs<V>(0.f), qy1 = vs<V>(0.f), qz1 = vs<V>(0.f), qx0 = vs<V>(0.f);
-        F ey2 = vs<V>(0.f), ey3 = vs<V>(0.f), ez2 = vs<V>(0.f), ez3 = vs<V>(0.f), ex2 = vs<V>(0.f), ex3 = vs<V>(0.f);
+        F py0 = vs<V>(0.f), py1 = vs<V>(0.f), pz0 = vs<V>(0.f), pz1 = vs<V>(0.f);
+        F qy0 = vs<V>(0.f), qy1 = vs<V>(0.f), qz1 = vs<V>(0.f);
+        F ey2 = vs<V>(0.f), ey3 = vs<V>(0.f), ez2 = vs<V>(0.f), ez3 = vs<V>(0.f);
+        F px0, px1, ex2, ex3, qx0;   // x-slab psi, defined below on every path
         float ly1 = 0.f, lz0 = 0.f, lx0 = 0.f, lx1 = 0.f;
         const float *PY = nullptr, *PZ = nullptr, *PX = nullptr;
         float *QY = nullptr, *QZ = nullptr, *QX = nullptr;
@@ -426,17 +426,24 @@ __device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, 
             const long long xb = uo(((long long)k * ny + jr) * g.sw[0]);
             PX = psi_base(g, 0, 0) + xb;
             QX = psi_new(g, 0, 0) + xb;
-            if (sx) {
-                px0 = ldv<float, V>(PX, rx);
-                px1 = ldv<float, V>(PX + cx, rx);
-                ex2 = ldv<float, V>(PX + 2 * cx, rx);
-                ex3 = ldv<float, V>(PX + 3 * cx, rx);
-                qx0 = ldv<float, V>(psi_base(g, 0, 0) + uo(((long long)k * ny + jm) * g.sw[0]), rx);
-            }
             if (sxl) {
                 lx0 = ldb(PX, rxl);
                 lx1 = ldb(PX + cx, rxl);
             }
+        }
+        // interior launch: every lane of a chunk that meets the slab loads
+        // (lanes outside the slab read its first cell and never use the
+        // value), so no per-lane branch precedes the loads; the shell keeps
+        // the per-lane branch (one register fewer holds it at 5 waves/SIMD)
+        const bool xload = PM == 1 ? sxw : sx;
+        if (xload) {
+            px0 = ldv<float, V>(PX, rx);
+            px1 = ldv<float, V>(PX + cx, rx);
+            ex2 = ldv<float, V>(PX + 2 * cx, rx);
+            ex3 = ldv<float, V>(PX + 3 * cx, rx);
+            qx0 = ldv<float, V>(psi_base(g, 0, 0) + uo(((long long)k * ny + jm) * g.sw[0]), rx);
+        } else {
+            px0 = px1 = ex2 = ex3 = qx0 = vs<V>(0.f);
         }
         const float khz = tab[(2 * 6 + TK_KH) * tn + min(k, nz - 2)];
         const float bhz = sz ? tab[(2 * 6 + TK_BH) * tn + k] : 0.f, chz = sz ? tab[(2 * 6 + TK_CH) * tn + k] : 0.f;
