@@ -454,7 +454,8 @@ def main():
     sim.close()
     if not args.no_batch:
         out["batched"] = batch_section(args)
-        out["batched_3d"] = batch3d_section(args)
+        if args.batch3d_sims > 0:
+            out["batched_3d"] = batch3d_section(args)
     if not args.no_serve:
         out["serving"] = serve_section(args)
     if not args.no_cpu:

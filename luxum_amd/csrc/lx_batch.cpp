@@ -41,7 +41,8 @@ struct Group {
     BSim *d_sims = nullptr;
     BJob *d_jobs = nullptr;
     BAux *d_aux = nullptr;
-    int nouter = 0, ninner = 0, naux = 0;
+    int njobs[5] = {0, 0, 0, 0, 0};           // jobs per class, stored in class order (general 0..3, inner)
+    int naux = 0;
     int par = 0;                              // buffer index holding the current state
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // two chunks starting at parity 0 / 1
     int stable = 0;                           // stepped at least once since the last rebuild
@@ -91,12 +92,19 @@ bool tiled_ok(const lx_sim *s) {
     return true;
 }
 
-// is the region of the job starting at (i0, k0) inside [1, N-2] on both
-// axes and inside the uniform-coefficient range (no slab, one z coefficient)?
-bool job_inner(const lx_sim *s, int i0, int k0) {
+// Class of the job starting at (i0, k0): kInnerJobs when its region lies
+// inside [1, N-2] on both axes and inside the uniform-coefficient range (no
+// slab, one z coefficient); otherwise bit 0 = the region meets an x slab,
+// bit 1 = it meets a z slab.  Every x or z index with a nonzero CPML c lies
+// in a slab, and the kernel's coefficients outside the domain are zero or
+// taken from its edge index, so a class without a bit has zero psi there.
+int job_class(const lx_sim *s, int i0, int k0) {
     const int x0 = i0 - kTT, x1 = x0 + 64, z0 = k0 - kTT, z1 = z0 + kRH;
-    return x0 >= std::max(1, s->ulo[0]) && x1 <= std::min(s->N[0] - 1, s->uhi[0]) && z0 >= std::max(1, s->ulo[2]) &&
-           z1 <= std::min(s->N[2] - 1, s->uhi[2]);
+    if (x0 >= std::max(1, s->ulo[0]) && x1 <= std::min(s->N[0] - 1, s->uhi[0]) && z0 >= std::max(1, s->ulo[2]) &&
+        z1 <= std::min(s->N[2] - 1, s->uhi[2]))
+        return kInnerJobs;
+    if (!s->pml) return 0;
+    return (x0 < s->slo[0] || x1 > s->shi[0] ? 1 : 0) | (z0 < s->slo[2] || z1 > s->shi[2] ? 2 : 0);
 }
 
 void free_group(Group &g) {
@@ -110,7 +118,8 @@ void free_group(Group &g) {
     g.d_sims = nullptr;
     g.d_jobs = nullptr;
     g.d_aux = nullptr;
-    g.nouter = g.ninner = g.naux = 0;
+    for (int &n : g.njobs) n = 0;
+    g.naux = 0;
 }
 
 void free_group3(Group3 &g) {
@@ -276,7 +285,7 @@ lx_status rebuild(lx_batch *b) {
     for (Group &g : b->grp) {
         if (g.sims.empty()) continue;
         std::vector<BSim> hs(g.sims.size());
-        std::vector<BJob> jobs, ojobs;
+        std::vector<BJob> cjobs[5];
         std::vector<BAux> aux;
         for (size_t q = 0; q < g.sims.size(); ++q) {
             lx_sim *s = g.sims[q];
@@ -361,13 +370,14 @@ lx_status rebuild(lx_batch *b) {
             for (int tz = 0; tz < (s->N[2] + kTH - 1) / kTH; ++tz)
                 for (int tx = 0; tx < (s->N[0] + kTW - 1) / kTW; ++tx) {
                     const BJob j{q, tx * kTW, tz * kTH};
-                    (job_inner(s, j.i0, j.k0) ? jobs : ojobs).push_back(j);
+                    cjobs[job_class(s, j.i0, j.k0)].push_back(j);
                 }
         }
-        g.nouter = (int)ojobs.size();
-        g.ninner = (int)jobs.size();
-        ojobs.insert(ojobs.end(), jobs.begin(), jobs.end());
-        jobs.swap(ojobs);
+        std::vector<BJob> jobs;
+        for (int c = 0; c < 5; ++c) {
+            g.njobs[c] = (int)cjobs[c].size();
+            jobs.insert(jobs.end(), cjobs[c].begin(), cjobs[c].end());
+        }
         void *p = nullptr;
         const int gi = g.fp16;
         lx_status st = desc_buf(b, gi, 0, sizeof(BSim) * hs.size(), &p);
@@ -391,23 +401,26 @@ lx_status rebuild(lx_batch *b) {
     return LX_OK;
 }
 
-// One chunk: incident lines, then the general-path jobs (side stream) and the
-// inner jobs (main stream) concurrently, then the step counters.
+// One chunk: incident lines, then the tile jobs on two streams at once: the
+// inner jobs and then the corner class (x and z slabs, the fewest jobs) on
+// the main stream, the other general classes on the side stream; then the
+// step counters.
 void enqueue_chunk(const lx_batch *b, const Group &g, int nt, int par) {
     hipStream_t s = b->stream;
     if (g.naux) launch_aux_chunk(g.d_aux, g.naux, nt, s);
-    const bool fork = g.nouter && g.ninner;
+    const bool fork = (g.njobs[0] || g.njobs[1] || g.njobs[2]) && (g.njobs[3] || g.njobs[kInnerJobs]);
     if (fork) {
         (void)hipEventRecord(b->fork, s);
         (void)hipStreamWaitEvent(b->side, b->fork, 0);
     }
     hipStream_t so = fork ? b->side : s;
-    if (g.fp16) {
-        launch_tile2d<__half>(g.d_sims, g.d_jobs, g.nouter, false, nt, par, so);
-        launch_tile2d<__half>(g.d_sims, g.d_jobs + g.nouter, g.ninner, true, nt, par, s);
-    } else {
-        launch_tile2d<float>(g.d_sims, g.d_jobs, g.nouter, false, nt, par, so);
-        launch_tile2d<float>(g.d_sims, g.d_jobs + g.nouter, g.ninner, true, nt, par, s);
+    int off[6] = {0};
+    for (int c = 0; c < 5; ++c) off[c + 1] = off[c] + g.njobs[c];
+    static const int order[5] = {kInnerJobs, 3, 0, 1, 2};
+    for (int c : order) {
+        hipStream_t sc = (c == kInnerJobs || c == 3) ? s : so;
+        if (g.fp16) launch_tile2d<__half>(g.d_sims, g.d_jobs + off[c], g.njobs[c], c, nt, par, sc);
+        else launch_tile2d<float>(g.d_sims, g.d_jobs + off[c], g.njobs[c], c, nt, par, sc);
     }
     if (fork) {
         (void)hipEventRecord(b->join, b->side);

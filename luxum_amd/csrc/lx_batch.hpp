@@ -87,10 +87,12 @@ void launch_tfsf_batch(const B3Sim<T> *sims, const int2 *bmap, int nblocks, int 
 template <typename T>
 void launch_finish_batch(const B3Sim<T> *sims, int nsims, int n, int par_end, hipStream_t s);
 
-// inner: every job's region lies inside the domain, outside the CPML slabs,
-// on rows with one z coefficient; otherwise the general path
+// Job class: kInnerJobs = every job's region lies inside the domain, outside
+// the CPML slabs, on rows with one z coefficient; 0..3 = the general path for
+// regions that meet an x slab (bit 0) and/or a z slab (bit 1).
+constexpr int kInnerJobs = 4;
 template <typename T>
-void launch_tile2d(const BSim *sims, const BJob *jobs, int njobs, bool inner, int nt, int par, hipStream_t s);
+void launch_tile2d(const BSim *sims, const BJob *jobs, int njobs, int cls, int nt, int par, hipStream_t s);
 void launch_aux_chunk(const BAux *aux, int naux, int nt, hipStream_t s);
 void launch_batch_finish(const BSim *sims, int nsims, int nt, hipStream_t s);
 

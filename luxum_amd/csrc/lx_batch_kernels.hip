@@ -113,11 +113,15 @@ __device__ __forceinline__ float rdlane(float v, int l) {
 //
 // INNER jobs have their region inside [1, N-2] on both axes, outside the
 // CPML slabs and on rows with one z coefficient (the host checks): no psi,
-// one z coefficient per wave.
-template <typename T, bool INNER>
+// one z coefficient per wave.  The other jobs come in four classes XZ by the
+// slabs their region meets (bit 0: an x slab, bit 1: a z slab); the psi of a
+// slab the region does not meet is zero throughout (c = 0 there), so those
+// classes carry only the psi they need.
+template <typename T, bool INNER, int XZ>
 __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ sims, const BJob *__restrict__ jobs,
                                                      int nt, int par) {
     constexpr int NT = 64 * kNW;
+    constexpr bool XP = !INNER && (XZ & 1), ZP = !INNER && (XZ & 2);   // x / z psi carried
     __shared__ float scb[256];
     __shared__ float sEtop[kNW][64], sHbot[kNW][64];
     __shared__ int sprb[256];                 // probe index
@@ -219,12 +223,12 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
             cb[m] = scb[S.mat[o]];
             if (!INNER && !(exok && k >= 1 && k <= nz - 2)) cb[m] = 0.f;   // PEC wall
             if (!INNER && pml) {
-                if (xslab) {
+                if (XP && xslab) {
                     const long long off = (long long)k * S.sw[0] + bslab(i, slo0, shi0);
                     pxh[m] = px0[off];                         // H slot0 (Hz)
                     pxe[m] = px0[3 * S.scells[0] + off];       // E slot1 (Ey)
                 }
-                if (k < slo2 || k >= shi2) {
+                if (ZP && (k < slo2 || k >= shi2)) {
                     const long long off = (long long)bslab(k, slo2, shi2) * px + i;
                     pzh[m] = pz0[S.scells[2] + off];           // H slot1 (Hx)
                     pze[m] = pz0[2 * S.scells[2] + off];       // E slot0 (Ey)
@@ -268,12 +272,16 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
             } else {
                 const float4 z = szh[16 * w + m];
                 float acc = -d2 * z.x;
-                pzh[m] = z.y * pzh[m] + z.z * d2;
-                acc -= pzh[m];
+                if (ZP) {
+                    pzh[m] = z.y * pzh[m] + z.z * d2;
+                    acc -= pzh[m];
+                }
                 hx[m] -= acc;
                 float acc2 = d1 * khx;
-                pxh[m] = bhx * pxh[m] + chx * d1;
-                acc2 += pxh[m];
+                if (XP) {
+                    pxh[m] = bhx * pxh[m] + chx * d1;
+                    acc2 += pxh[m];
+                }
                 hz[m] -= acc2;
             }
         }
@@ -299,7 +307,7 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
                 float coef;
                 if (a == 0) {
                     coef = khx;
-                    if (!INNER && face_lane) {
+                    if (XP && face_lane) {
                         pxh[m] += chx * delta;
                         coef += chx;
                     }
@@ -307,8 +315,11 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
                     coef = khz0;
                 } else {
                     const float4 z = szh[16 * w + m];
-                    if (face_lane) pzh[m] += z.z * delta;
-                    coef = z.x + z.z;
+                    coef = z.x;
+                    if (ZP) {
+                        if (face_lane) pzh[m] += z.z * delta;
+                        coef += z.z;
+                    }
                 }
                 const float v = sgn * coef * delta;
                 if (face_lane) {
@@ -343,10 +354,14 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
             } else {
                 const float4 z = sze[16 * w + m];
                 float acc = d1 * z.x - d2 * kex;
-                pze[m] = z.y * pze[m] + z.z * d1;
-                acc += pze[m];
-                pxe[m] = bex * pxe[m] + cex * d2;
-                acc -= pxe[m];
+                if (ZP) {
+                    pze[m] = z.y * pze[m] + z.z * d1;
+                    acc += pze[m];
+                }
+                if (XP) {
+                    pxe[m] = bex * pxe[m] + cex * d2;
+                    acc -= pxe[m];
+                }
                 ey[m] += cb[m] * acc;
             }
         }
@@ -370,7 +385,7 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
                 float coef;
                 if (a == 0) {
                     coef = kex;
-                    if (!INNER && face_lane) {
+                    if (XP && face_lane) {
                         pxe[m] += cex * delta;
                         coef += cex;
                     }
@@ -378,8 +393,11 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
                     coef = kez0;
                 } else {
                     const float4 z = sze[16 * w + m];
-                    if (face_lane) pze[m] += z.z * delta;
-                    coef = z.x + z.z;
+                    coef = z.x;
+                    if (ZP) {
+                        if (face_lane) pze[m] += z.z * delta;
+                        coef += z.z;
+                    }
                 }
                 if (face_lane) ey[m] += cb[m] * (sgn * coef * delta);
             }
@@ -408,12 +426,12 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
             bst(Fout + 3 * fs + o, hx[m]);
             bst(Fout + 5 * fs + o, hz[m]);
             if (!INNER && pml) {
-                if (xslab) {
+                if (XP && xslab) {
                     const long long off = (long long)k * S.sw[0] + bslab(i, slo0, shi0);
                     qx0[off] = pxh[m];
                     qx0[3 * S.scells[0] + off] = pxe[m];
                 }
-                if (k < slo2 || k >= shi2) {
+                if (ZP && (k < slo2 || k >= shi2)) {
                     const long long off = (long long)bslab(k, slo2, shi2) * px + i;
                     qz0[S.scells[2] + off] = pzh[m];
                     qz0[2 * S.scells[2] + off] = pze[m];
@@ -424,10 +442,16 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
 }
 
 template <typename T>
-void launch_tile2d(const BSim *sims, const BJob *jobs, int njobs, bool inner, int nt, int par, hipStream_t s) {
+void launch_tile2d(const BSim *sims, const BJob *jobs, int njobs, int cls, int nt, int par, hipStream_t s) {
     if (!njobs) return;
-    if (inner) hipLaunchKernelGGL((k_tile2d<T, true>), dim3((unsigned)njobs), dim3(64 * kNW), 0, s, sims, jobs, nt, par);
-    else hipLaunchKernelGGL((k_tile2d<T, false>), dim3((unsigned)njobs), dim3(64 * kNW), 0, s, sims, jobs, nt, par);
+    const dim3 grd((unsigned)njobs), blk(64 * kNW);
+    switch (cls) {
+    case kInnerJobs: hipLaunchKernelGGL((k_tile2d<T, true, 0>), grd, blk, 0, s, sims, jobs, nt, par); break;
+    case 0: hipLaunchKernelGGL((k_tile2d<T, false, 0>), grd, blk, 0, s, sims, jobs, nt, par); break;
+    case 1: hipLaunchKernelGGL((k_tile2d<T, false, 1>), grd, blk, 0, s, sims, jobs, nt, par); break;
+    case 2: hipLaunchKernelGGL((k_tile2d<T, false, 2>), grd, blk, 0, s, sims, jobs, nt, par); break;
+    default: hipLaunchKernelGGL((k_tile2d<T, false, 3>), grd, blk, 0, s, sims, jobs, nt, par); break;
+    }
 }
 void launch_aux_chunk(const BAux *aux, int naux, int nt, hipStream_t s) {
     hipLaunchKernelGGL(k_aux_chunk, dim3((unsigned)naux), dim3(256), 0, s, aux, nt);
@@ -435,7 +459,7 @@ void launch_aux_chunk(const BAux *aux, int naux, int nt, hipStream_t s) {
 void launch_batch_finish(const BSim *sims, int nsims, int nt, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_finish, dim3((unsigned)((nsims + 255) / 256)), dim3(256), 0, s, sims, nsims, nt);
 }
-template void launch_tile2d<float>(const BSim *, const BJob *, int, bool, int, int, hipStream_t);
-template void launch_tile2d<__half>(const BSim *, const BJob *, int, bool, int, int, hipStream_t);
+template void launch_tile2d<float>(const BSim *, const BJob *, int, int, int, int, hipStream_t);
+template void launch_tile2d<__half>(const BSim *, const BJob *, int, int, int, int, hipStream_t);
 
 }  // namespace lx
