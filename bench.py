@@ -443,6 +443,7 @@ def main():
                    "step_GBps": round(step_gbs, 1), "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
                    "ms_h": round(ms_h, 5), "ms_e": round(ms_e, 5), "max_E2": mx, "kz": info.kz,
                    "waves_per_block": info.waves, "cells_per_lane": info.cells_per_lane, "shell_kz": info.shell_kz,
+                   "shell_concurrent": bool(info.shell_concurrent),
                    "fused_step": bool(info.fused_step),
                    "fused_step_roofline_frac": None if fused_frac is None else round(fused_frac, 4)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

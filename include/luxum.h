@@ -130,6 +130,8 @@ typedef struct lx_sim_info {
     int32_t cells_per_lane;     /* fused step: x cells per lane (1, 2 or 4, autotuned); else 1 */
     int32_t shell_kz;           /* fused step on a CPML grid: z planes per block of the shell launch
                                    (autotuned); else kz */
+    int32_t shell_concurrent;   /* fused step on a CPML grid: 1 when the shell launch runs on a second
+                                   stream beside the interior launch (autotuned); else 0 */
 } lx_sim_info;
 
 /* ---- parameters ------------------------------------------------------- */

@@ -59,6 +59,7 @@ class SimInfo(C.Structure):
         ("core_bytes", C.c_int64),
         ("cells_per_lane", C.c_int32),
         ("shell_kz", C.c_int32),
+        ("shell_concurrent", C.c_int32),
     ]
 
 

@@ -451,6 +451,25 @@ ProbeArgs probe_args(const lx_sim *s) {
     return pr;
 }
 
+// The fused step's two launches: the interior box (part 0) and the CPML
+// shell (part 1).  Both read one buffer set and write disjoint cells of the
+// other, so the shell may run on the second stream ks at the same time, its
+// blocks filling the interior launch's tail.
+template <typename T>
+void launch_fused_step(lx_sim *s, const Grid<T> &g) {
+    if (s->ks && s->kconc) {
+        (void)hipEventRecord(s->kfork, s->cs);
+        (void)hipStreamWaitEvent(s->ks, s->kfork, 0);
+        launch_step3(g, s->nw, s->vec, 1, s->ks);
+        launch_step3(g, s->nw, s->vec, 0, s->cs);
+        (void)hipEventRecord(s->kjoin, s->ks);
+        (void)hipStreamWaitEvent(s->cs, s->kjoin, 0);
+    } else {
+        launch_step3(g, s->nw, s->vec, 0, s->cs);
+        launch_step3(g, s->nw, s->vec, 1, s->cs);
+    }
+}
+
 // One launch sequence of n timesteps: per step [point source] H launch,
 // E launch; then the sequence-end kernel.  Kernels read the step index as
 // counter[0] + toff, so the same sequence can be captured once and replayed.
@@ -475,12 +494,16 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
             if (s->p.use_ps) launch_point_source(gq, s->ps, pr, q, s->cs);
             if (corr) launch_tfsf_hpre(gq, s->tfH, s->inc, s->aux, pr, l, s->cs);
             // timed parts: [0,1] interior box, [2,3] CPML shell
-            if (ev) (void)hipEventRecord((*ev)[4 * q + 0], s->cs);
-            launch_step3(gq, s->nw, s->vec, 0, s->cs);
-            if (ev) (void)hipEventRecord((*ev)[4 * q + 1], s->cs);
-            if (ev) (void)hipEventRecord((*ev)[4 * q + 2], s->cs);
-            launch_step3(gq, s->nw, s->vec, 1, s->cs);
-            if (ev) (void)hipEventRecord((*ev)[4 * q + 3], s->cs);
+            if (!ev) {
+                launch_fused_step(s, gq);
+            } else {
+                if (ev) (void)hipEventRecord((*ev)[4 * q + 0], s->cs);
+                launch_step3(gq, s->nw, s->vec, 0, s->cs);
+                if (ev) (void)hipEventRecord((*ev)[4 * q + 1], s->cs);
+                if (ev) (void)hipEventRecord((*ev)[4 * q + 2], s->cs);
+                launch_step3(gq, s->nw, s->vec, 1, s->cs);
+                if (ev) (void)hipEventRecord((*ev)[4 * q + 3], s->cs);
+            }
             Grid<T> gn = g;
             gn.F = gq.Fn;
             gn.psi = gq.psin;
@@ -755,6 +778,7 @@ void lx_sim_destroy(lx_sim *s) {
     (void)hipSetDevice(s->ctx->device);
     if (s->cs) (void)hipStreamSynchronize(s->cs);
     if (s->ss) (void)hipStreamSynchronize(s->ss);
+    if (s->ks) (void)hipStreamSynchronize(s->ks);
     drop_graph(s);
     for (const auto &a : s->allocs) release_block(s->ctx, a.first, a.second);
     if (s->prec) (void)hipFree(s->prec);
@@ -766,6 +790,9 @@ void lx_sim_destroy(lx_sim *s) {
     if (s->rd_stage) (void)hipFree(s->rd_stage);
     give_stream(s->ctx, s->cs);
     give_stream(s->ctx, s->ss);
+    give_stream(s->ctx, s->ks);
+    if (s->kfork) (void)hipEventDestroy(s->kfork);
+    if (s->kjoin) (void)hipEventDestroy(s->kjoin);
     delete s;
 }
 
@@ -848,6 +875,14 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
                 else if ((f == '2' || (f == 'd' && s->fp16)) && vec_ok(s, 2)) s->vec = 2;
                 const char *sk = std::getenv("LX_SHELL_KZ");   // z-block depth of the CPML shell launch
                 if (sk && *sk) s->kzs = std::max(0, std::atoi(sk));
+                // LX_SHELL_STREAM=0 keeps the shell launch behind the interior launch on one stream
+                const char *ks = std::getenv("LX_SHELL_STREAM");
+                if (s->pml && !(ks && *ks == '0')) {
+                    if ((st = take_stream(ctx, &s->ks))) return bail(st);
+                    if (hipEventCreateWithFlags(&s->kfork, hipEventDisableTiming) != hipSuccess ||
+                        hipEventCreateWithFlags(&s->kjoin, hipEventDisableTiming) != hipSuccess)
+                        return bail(fail(LX_EDEVICE, "event creation failed"));
+                }
             } else if (st == LX_ENOMEM && f == 'd') {
                 (void)hipGetLastError();
                 release_fused_buffers(s);
@@ -1136,6 +1171,7 @@ lx_status lx_sim_get_info(lx_sim *s, lx_sim_info *info) {
     info->core_bytes = info->bytes_per_step;
     info->cells_per_lane = s->fused ? s->vec : 1;
     info->shell_kz = s->fused && s->pml && s->kzs > 0 ? s->kzs : s->kz;
+    info->shell_concurrent = s->fused && s->ks && s->kconc ? 1 : 0;
     if (s->fused && s->pml) {
         int rows = 0, planes = 0;
         step3_core_extent(s->N[1], s->N[2], s->kz, s->slo, s->shi, &rows, &planes);
@@ -1171,7 +1207,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
     member_enter(s);
     using Key = std::tuple<int, int, int, int, int, int, int>;
     static std::mutex mu;
-    static std::map<Key, std::array<int, 5>> cache;   // (kz, nw, fused, vec, shell kz)
+    static std::map<Key, std::array<int, 6>> cache;   // (kz, nw, fused, vec, shell kz, shell stream)
     const Key key{s->N[0], s->N[1], s->N[2], s->fp16, s->pml, s->planar, 0};
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -1181,6 +1217,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
             s->nw = it->second[1];
             s->vec = it->second[3];
             s->kzs = it->second[4];
+            s->kconc = it->second[5];
             if (it->second[2] && fused_ok(s)) {
                 lx_status st = ensure_fused_buffers(s);
                 if (st) return st;
@@ -1198,43 +1235,59 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
-        // milliseconds per step of the current settings: the fastest of reps
-        // repetitions of n steps
-        auto time_steps = [&](int n, int reps, float *out) -> lx_status {
-            *out = 1e30f;
-            for (int rep = 0; rep < reps; ++rep) {
-                HIP_TRY(hipEventRecord(e0, s->cs));
-                for (int q = 0; q < n; ++q) {
-                    if (s->fp16) {
-                        Grid<__half> g = make_grid<__half>(s);
-                        if (s->fused) {
-                            g.Fn = (__half *)s->F_alt;
-                            g.psin = s->psi_alt;
-                            launch_step3(g, s->nw, s->vec, 0, s->cs);
-                            launch_step3(g, s->nw, s->vec, 1, s->cs);
-                        } else {
-                            launch_bulk(g, 0, 0, s->cs);
-                            launch_bulk(g, 0, 1, s->cs);
-                        }
+        // n steps of the current settings on cs (H and E of the half-step
+        // pair, or the fused step's launches)
+        auto enq = [&](int n) {
+            for (int q = 0; q < n; ++q) {
+                if (s->fp16) {
+                    Grid<__half> g = make_grid<__half>(s);
+                    if (s->fused) {
+                        g.Fn = (__half *)s->F_alt;
+                        g.psin = s->psi_alt;
+                        launch_fused_step(s, g);
                     } else {
-                        Grid<float> g = make_grid<float>(s);
-                        if (s->fused) {
-                            g.Fn = (float *)s->F_alt;
-                            g.psin = s->psi_alt;
-                            launch_step3(g, s->nw, s->vec, 0, s->cs);
-                            launch_step3(g, s->nw, s->vec, 1, s->cs);
-                        } else {
-                            launch_bulk(g, 0, 0, s->cs);
-                            launch_bulk(g, 0, 1, s->cs);
-                        }
+                        launch_bulk(g, 0, 0, s->cs);
+                        launch_bulk(g, 0, 1, s->cs);
+                    }
+                } else {
+                    Grid<float> g = make_grid<float>(s);
+                    if (s->fused) {
+                        g.Fn = (float *)s->F_alt;
+                        g.psin = s->psi_alt;
+                        launch_fused_step(s, g);
+                    } else {
+                        launch_bulk(g, 0, 0, s->cs);
+                        launch_bulk(g, 0, 1, s->cs);
                     }
                 }
+            }
+        };
+        // milliseconds per step of the current settings: the fastest of reps
+        // repetitions of n steps, launched directly or, with graph, replayed
+        // from a captured graph as the time loop runs them
+        auto time_steps = [&](int n, int reps, float *out, bool graph = false) -> lx_status {
+            *out = 1e30f;
+            hipGraphExec_t gx = nullptr;
+            if (graph) {
+                hipGraph_t gr = nullptr;
+                HIP_TRY(hipStreamBeginCapture(s->cs, hipStreamCaptureModeThreadLocal));
+                enq(n);
+                HIP_TRY(hipStreamEndCapture(s->cs, &gr));
+                const hipError_t e = hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (e != hipSuccess) return fail(LX_EDEVICE, "autotune graph: %s", hipGetErrorString(e));
+            }
+            for (int rep = 0; rep < reps; ++rep) {
+                HIP_TRY(hipEventRecord(e0, s->cs));
+                if (gx) HIP_TRY(hipGraphLaunch(gx, s->cs));
+                else enq(n);
                 HIP_TRY(hipEventRecord(e1, s->cs));
                 HIP_TRY(hipEventSynchronize(e1));
                 float ms = 0;
                 HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
                 if (rep > 0 || reps == 1) *out = std::min(*out, ms / n);   // the first repetition warms up
             }
+            if (gx) (void)hipGraphExecDestroy(gx);
             return check_launch();
         };
         const bool verbose = std::getenv("LX_VERBOSE") != nullptr;
@@ -1242,14 +1295,18 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         // 4 wavefronts per block and 1, 2 or 4 cells per lane (its second
         // buffer set is kept only if it wins); the CPML shell takes the
         // interior's z-block depth.  A first pass times every candidate over
-        // 4 steps; the three fastest are timed again over 12 steps (best of
-        // 3), which separates near-ties the short pass cannot.
+        // 4 steps, best of 2 (the shell on the second stream where there is one); the
+        // three fastest are timed again over 12 steps (best of 3) replayed
+        // from a graph, as the time loop runs them, which separates
+        // near-ties the short pass cannot, each with the shell launch beside
+        // and behind the interior launch.
         const int can_fuse = fused_ok(s) && ensure_fused_buffers(s) == LX_OK;
         s->kzs = 0;
-        struct Cand { int nw, vec, kz; float ms; };
+        s->kconc = 1;
+        struct Cand { int nw, vec, kz; float ms; int conc; };
         std::vector<Cand> tried;
-        for (Cand c : {Cand{0, 1, 0, 0}, Cand{8, 1, 0, 0}, Cand{4, 1, 0, 0}, Cand{4, 2, 0, 0}, Cand{8, 2, 0, 0},
-                       Cand{4, 4, 0, 0}}) {
+        for (Cand c : {Cand{0, 1, 0, 0, 1}, Cand{8, 1, 0, 0, 1}, Cand{4, 1, 0, 0, 1}, Cand{4, 2, 0, 0, 1},
+                       Cand{8, 2, 0, 0, 1}, Cand{4, 4, 0, 0, 1}}) {
             if (c.nw && !can_fuse) continue;
             if (!vec_ok(s, c.vec)) continue;
             for (int kz : {4, 5, 6, 8, 12, 16, 32, 64}) {
@@ -1258,7 +1315,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
                 if (c.nw) s->nw = c.nw;
                 s->vec = c.vec;
                 s->kz = kz;
-                lx_status st = time_steps(4, 2, &c.ms);
+                lx_status st = time_steps(4, 3, &c.ms);
                 if (st) return st;
                 c.kz = kz;
                 tried.push_back(c);
@@ -1271,22 +1328,29 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         Cand best = tried.front();
         best.ms = 1e30f;
         for (size_t q = 0; q < std::min<size_t>(3, tried.size()); ++q) {
-            Cand c = tried[q];
-            s->fused = c.nw != 0;
-            if (c.nw) s->nw = c.nw;
-            s->vec = c.vec;
-            s->kz = c.kz;
-            lx_status st = time_steps(12, 3, &c.ms);
-            if (st) return st;
-            if (verbose)
-                std::fprintf(stderr, "lx_sim_autotune: finalist %s nw=%d vec=%d kz=%d  %.4f ms per step\n",
-                             s->fused ? "fused" : "half-step", c.nw, c.vec, c.kz, c.ms);
-            if (c.ms < best.ms) best = c;
+            for (int conc = 1; conc >= 0; --conc) {
+                Cand c = tried[q];
+                if (conc == 0 && !(c.nw && s->ks)) continue;   // only the fused step has a shell launch
+                c.conc = conc;
+                s->fused = c.nw != 0;
+                if (c.nw) s->nw = c.nw;
+                s->vec = c.vec;
+                s->kz = c.kz;
+                s->kconc = conc;
+                lx_status st = time_steps(12, 3, &c.ms, s->graphs != 0);
+                if (st) return st;
+                if (verbose)
+                    std::fprintf(stderr, "lx_sim_autotune: finalist %s nw=%d vec=%d kz=%d shell %s  %.4f ms per step\n",
+                                 s->fused ? "fused" : "half-step", c.nw, c.vec, c.kz,
+                                 conc ? "beside" : "behind", c.ms);
+                if (c.ms < best.ms) best = c;
+            }
         }
         s->fused = best.nw != 0;
         if (best.nw) s->nw = best.nw;
         s->vec = best.vec;
         s->kz = best.kz;
+        s->kconc = best.conc;
         s->kzs = 0;   // the shell takes the interior's depth: a depth of its own was never faster when measured
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -1298,7 +1362,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
     if (st) return st;
     {
         std::lock_guard<std::mutex> lk(mu);
-        cache[key] = {s->kz, s->nw, s->fused, s->vec, s->kzs};
+        cache[key] = {s->kz, s->nw, s->fused, s->vec, s->kzs, s->kconc};
     }
     if (kz_out) *kz_out = s->kz;
     return LX_OK;

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -79,6 +80,7 @@ struct lx_batch {
     void *desc[2][5] = {};
     size_t desc_cap[2][5] = {};
     int graphs = 1;
+    int shell_side = 1;             // 3D: shell launch on `side` beside the interior launch (LX_SHELL_STREAM)
 };
 
 namespace {
@@ -437,11 +439,23 @@ template <typename T>
 void enqueue3_t(const lx_batch *b, const Group3 &g, int nt, int par) {
     hipStream_t s = b->stream;
     const B3Sim<T> *sims = (const B3Sim<T> *)g.d_sims;
+    // the interior boxes and the CPML shells read one buffer set and write
+    // disjoint cells of the other: the shells may run beside on `side`
+    const bool fork = b->shell_side && g.nblk[0] && g.nblk[1];
     for (int q = 0; q < nt; ++q) {
         const int pq = (par + q) & 1;
         launch_tfsf_batch<T>(sims, g.d_map + g.off[2], g.nblk[2], 1, pq, q, s);
-        launch_step3_batch<T>(sims, g.d_map + g.off[0], g.nblk[0], 0, pq, g.vec, s);
-        launch_step3_batch<T>(sims, g.d_map + g.off[1], g.nblk[1], 1, pq, g.vec, s);
+        if (fork) {
+            (void)hipEventRecord(b->fork, s);
+            (void)hipStreamWaitEvent(b->side, b->fork, 0);
+            launch_step3_batch<T>(sims, g.d_map + g.off[1], g.nblk[1], 1, pq, g.vec, b->side);
+            launch_step3_batch<T>(sims, g.d_map + g.off[0], g.nblk[0], 0, pq, g.vec, s);
+            (void)hipEventRecord(b->join, b->side);
+            (void)hipStreamWaitEvent(s, b->join, 0);
+        } else {
+            launch_step3_batch<T>(sims, g.d_map + g.off[0], g.nblk[0], 0, pq, g.vec, s);
+            launch_step3_batch<T>(sims, g.d_map + g.off[1], g.nblk[1], 1, pq, g.vec, s);
+        }
         launch_tfsf_batch<T>(sims, g.d_map + g.off[3], g.nblk[3], 0, pq, q, s);
     }
     launch_finish_batch<T>(sims, (int)g.sims.size(), nt, (par + nt) & 1, s);
@@ -504,6 +518,8 @@ lx_status lx_batch_create(lx_ctx *ctx, lx_batch **out) {
     lx_batch *b = new (std::nothrow) lx_batch;
     if (!b) return fail(LX_ENOMEM, "out of host memory");
     b->ctx = ctx;
+    const char *ks = std::getenv("LX_SHELL_STREAM");
+    b->shell_side = !(ks && *ks == '0');
     if (hipSetDevice(ctx->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||

@@ -49,6 +49,11 @@ struct lx_sim {
     // constants (maxwell.cc:63-84)
     double eps0 = 0, mu0 = 0, c0 = 0, dt = 0, eta0 = 0, sigma_max = 0;
     hipStream_t cs = nullptr, ss = nullptr;
+    // fused step: the CPML shell launch runs on ks beside the interior launch
+    // on cs (both read one buffer set and write disjoint cells of the other)
+    hipStream_t ks = nullptr;
+    hipEvent_t kfork = nullptr, kjoin = nullptr;
+    int kconc = 1;              // use ks (when it exists); autotuned
     std::vector<std::pair<void *, size_t>> allocs;   // device blocks and their sizes
     int64_t bytes = 0;
     void *F = nullptr;          // six components, component c at F + c*fstride elements
