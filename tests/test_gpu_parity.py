@@ -353,6 +353,40 @@ def test_fused_shell_block_depth(gpu_ctx, monkeypatch):
         assert np.array_equal(s.probes(), ref.probes())
 
 
+def test_fused_shell_beside_interior(gpu_ctx, monkeypatch):
+    """The CPML shell launch on a second stream, running at the same time as
+    the interior launch, writes the same fields and probes as the shell
+    behind it on one stream (the two read one buffer set and write disjoint
+    cells of the other): FP32 one cell per lane and FP16 four cells per
+    lane, an odd step count, graph replay and eager launches."""
+    from luxum_amd.sim import Simulation
+
+    def run(w, mode, beside, graphs):
+        monkeypatch.setenv("LX_FUSED", mode)
+        monkeypatch.setenv("LX_SHELL_STREAM", "1" if beside else "0")
+        monkeypatch.setenv("LX_NO_GRAPH", "0" if graphs else "1")
+        s = Simulation(w.p, gpu_ctx)
+        s.set_materials(w.ids, w.eps_r)
+        s.set_probes(w.probes)
+        assert s.info().shell_concurrent == int(beside)
+        assert s.info().cells_per_lane == int(mode)
+        s.step(45)
+        s.sync()
+        return s
+
+    for prec, mode in ((configs.LX_FP32, "1"), (LX_FP16, "4")):
+        w = configs.box3d(200, 70, 60, pml=10, margin=5, precision=prec)
+        ref = run(w, mode, False, True)
+        for graphs in (True, False):
+            say(f"case {prec} shell beside, graphs {graphs}")
+            s = run(w, mode, True, graphs)
+            for c in w.p.components():
+                assert np.array_equal(s.field(c), ref.field(c)), f"{prec} graphs {graphs} component {c}"
+            assert np.array_equal(s.probes(), ref.probes())
+            s.close()
+        ref.close()
+
+
 def test_fused_two_cells_per_lane(gpu_ctx, monkeypatch):
     """The fused step with two x cells per lane (128-cell chunks) against
     the half-step kernels (1e-6) and the FP64 oracle (1e-5): two chunks per
