@@ -419,10 +419,12 @@ lx_status build_incident(lx_sim *s) {
     A.fwhm = p.pw_fwhm;
     A.delay = p.pw_delay;
     void *q = nullptr;
-    lx_status st = dalloc(s, &q, sizeof(double) * 2 * (size_t)A.L);
+    lx_status st = dalloc(s, &q, sizeof(double) * 4 * (size_t)A.L);   // two sets of (e, h)
     if (st) return st;
     A.e = (double *)q;
     A.h = A.e + A.L;
+    A.e2 = A.e + 2 * (size_t)A.L;
+    A.h2 = A.e2 + A.L;
     std::vector<double> init(2 * (size_t)A.L, 0.0);
     init[0] = waveform_host(A.wave, 0.0, A.c, A.lambda0, A.fwhm, A.delay);
     HIP_TRY(hipMemcpy(A.e, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice));
@@ -470,12 +472,35 @@ void launch_fused_step(lx_sim *s, const Grid<T> &g) {
     }
 }
 
+// Can the fused sequence run one TF/SF launch between steps?  Not when an H
+// probe lies on an H face cell (that launch records the H probes of a step
+// while it applies the next step's H corrections).
+bool tf_merge_ok(const lx_sim *s) {
+    if (!s->tf_merge || !s->fused) return false;
+    for (int q = 0; q < s->np; ++q) {
+        const int *c = &s->probe_host[4 * (size_t)q];
+        if (c[0] < 3) continue;
+        const int ijk[3] = {c[1], c[2], c[3]};
+        for (int b = 0; b < s->tfH.n; ++b) {
+            const TfBox &x = s->tfH.b[b];
+            if (x.comp == c[0] - 3 && ijk[0] >= x.lo[0] && ijk[0] <= x.hi[0] && ijk[1] >= x.lo[1] &&
+                ijk[1] <= x.hi[1] && ijk[2] >= x.lo[2] && ijk[2] <= x.hi[2])
+                return false;
+        }
+    }
+    return true;
+}
+
 // One launch sequence of n timesteps: per step [point source] H launch,
 // E launch; then the sequence-end kernel.  Kernels read the step index as
 // counter[0] + toff, so the same sequence can be captured once and replayed.
 // Fused 3D: per step [point source on the current buffer], the fused step
 // (current -> other buffer), the E TF/SF corrections on the other buffer;
 // the buffers alternate, so the state sits in F_alt after an odd count.
+// Merged TF/SF (tf_merge_ok): the H corrections precede the first step
+// only; each later step's ride in the launch between steps
+// (k_tfsf_between), whose aux line alternates between its two sets and
+// ends the sequence in the first.
 template <typename T>
 void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
     Grid<T> g = make_grid<T>(s);
@@ -484,6 +509,10 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
     if (s->fused) {
         T *buf[2] = {(T *)s->F, (T *)s->F_alt};
         float *pbuf[2] = {s->psi, s->psi_alt};
+        const bool merge = corr && !ev && tf_merge_ok(s);
+        AuxLine aset[2] = {s->aux, s->aux};
+        aset[1].e = s->aux.e2;
+        aset[1].h = s->aux.h2;
         for (int q = 0; q < n; ++q) {
             Launch l{q, s->use_aux, s->np > 0, s->np > 0 && !s->p.use_ps};
             Grid<T> gq = g;
@@ -492,7 +521,7 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
             gq.Fn = buf[(q + 1) & 1];
             gq.psin = pbuf[(q + 1) & 1];
             if (s->p.use_ps) launch_point_source(gq, s->ps, pr, q, s->cs);
-            if (corr) launch_tfsf_hpre(gq, s->tfH, s->inc, s->aux, pr, l, s->cs);
+            if (corr && (!merge || q == 0)) launch_tfsf_hpre(gq, s->tfH, s->inc, s->aux, pr, l, s->cs);
             // timed parts: [0,1] interior box, [2,3] CPML shell
             if (!ev) {
                 launch_fused_step(s, gq);
@@ -507,7 +536,14 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
             Grid<T> gn = g;
             gn.F = gq.Fn;
             gn.psi = gq.psin;
-            if (corr) launch_tfsf(gn, s->tfE, s->inc, s->aux, pr, l, 1, s->cs);
+            if (merge) {
+                const int last = q == n - 1;
+                const Launch lb{q, s->use_aux, s->np > 0, s->np > 0 && !s->p.use_ps && q >= 1};
+                launch_tfsf_between(gn, (const T *)gq.F, s->tfE, s->tfH, s->inc, aset[q & 1], aset[last ? 0 : (q + 1) & 1],
+                                    pr, lb, last, s->cs);
+            } else if (corr) {
+                launch_tfsf(gn, s->tfE, s->inc, s->aux, pr, l, 1, s->cs);
+            }
         }
         Grid<T> gf = g;
         gf.F = buf[n & 1];
@@ -814,6 +850,8 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
     s->esize = s->fp16 ? 2 : 4;
     const char *ng = std::getenv("LX_NO_GRAPH");
     s->graphs = !(ng && *ng && *ng != '0');
+    const char *mg = std::getenv("LX_TFSF_MERGE");   // 0: two TF/SF launches per fused step
+    s->tf_merge = !(mg && *mg == '0');
     // constants exactly as maxwell.cc:63-84
     s->eps0 = 8.85 * std::pow(10, -12);
     s->mu0 = M_PI * 4 * std::pow(10, -7);

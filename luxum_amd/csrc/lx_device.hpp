@@ -115,26 +115,35 @@ __device__ inline double inc_value(const IncParams &in, double cx, double cy, do
 }
 
 
+// The aux line's E phase at node m: e_{t+1} from e_t and h_{t+1/2} (the
+// source sample at node 0, a zero far end).  Written as explicit fused
+// multiply-adds so that every caller rounds alike: the launch between two
+// fused steps computes the nodes its face cells need with it while its extra
+// block writes the whole line with it.
+__device__ __forceinline__ double aux_e_next(const AuxLine &a, int m, int t) {
+    if (m == 0) return waveform(a.wave, (t + 1) * a.dt, a.c, a.lambda0, a.fwhm, a.delay);
+    if (m == a.L - 1) return 0.0;
+    return fma(-a.S, a.h[m] - a.h[m - 1], a.e[m]);
+}
+// The H phase at node m (m < L-1): h_{t+1/2} from h_{t-1/2} and e_t.
+__device__ __forceinline__ double aux_h_next(const double *h, const double *e, double S, int m) {
+    return fma(-S, e[m + 1] - e[m], h[m]);
+}
+
 // H half (IS_H): aux line H phase, E probes of step t-1.
 // E half:        aux line E phase (source sample t+1), H probes of step t.
 template <typename T, bool IS_H>
 __device__ inline void extra_block(const Grid<T> &g, const AuxLine &aux, const ProbeArgs &pr, Launch l, int t) {
     if (IS_H) {
         if (l.aux)
-            for (int m = threadIdx.x; m < aux.L - 1; m += blockDim.x) aux.h[m] = aux.h[m] - aux.S * (aux.e[m + 1] - aux.e[m]);
+            for (int m = threadIdx.x; m < aux.L - 1; m += blockDim.x) aux.h[m] = aux_h_next(aux.h, aux.e, aux.S, m);
         if (l.eprobes && t >= 1 && t - 1 < pr.cap)
             for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
                 if (pr.comp[q] < 3)
                     pr.rec[(long long)(t - 1) * pr.n + q] = (double)ld1(g.F + pr.comp[q] * g.fstride + pr.idx[q]);
     } else {
-        if (l.aux) {
-            for (int m = 1 + threadIdx.x; m < aux.L - 1; m += blockDim.x)
-                aux.e[m] = aux.e[m] - aux.S * (aux.h[m] - aux.h[m - 1]);
-            if (threadIdx.x == 0) {
-                aux.e[0] = waveform(aux.wave, (t + 1) * aux.dt, aux.c, aux.lambda0, aux.fwhm, aux.delay);
-                aux.e[aux.L - 1] = 0.0;
-            }
-        }
+        if (l.aux)
+            for (int m = threadIdx.x; m < aux.L; m += blockDim.x) aux.e[m] = aux_e_next(aux, m, t);
         if (l.probes && t < pr.cap)
             for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
                 if (pr.comp[q] >= 3)

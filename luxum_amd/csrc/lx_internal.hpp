@@ -85,6 +85,7 @@ struct IncParams {
 // FP64 aux-1D incident line, advanced by an extra block of the bulk kernels.
 struct AuxLine {
     double *e, *h;
+    double *e2, *h2;     // second set: the launch between two fused steps reads one set and writes the other
     int L;
     double S;            // c*dt/d_axis
     int wave;
@@ -205,6 +206,17 @@ void step3_unit_maps(int nx, int ny, int nz, int kz, int kzs, int pml, const int
 template <typename T>
 void launch_tfsf_hpre(const Grid<T> &g, const TfList &tf, const IncParams &inc, const AuxLine &aux,
                       const ProbeArgs &pr, Launch l, hipStream_t s);
+
+// Fused 3D step, between steps t and t+1 (g.F = the buffer step t wrote,
+// Fprev = the one it read): the E corrections of step t and the H
+// corrections of step t+1 in one launch, the aux line from set ain to set
+// aout, the H probes of step t and (l.eprobes) the E probes of step t-1.
+// last: no H corrections (the sequence's final step).  Requires no H probe
+// on an H face cell.
+template <typename T>
+void launch_tfsf_between(const Grid<T> &g, const T *Fprev, const TfList &tfE, const TfList &tfH, const IncParams &inc,
+                         const AuxLine &ain, const AuxLine &aout, const ProbeArgs &pr, Launch l, int last,
+                         hipStream_t s);
 
 // End of a launch sequence of n steps: records the E probes of the last step
 // and advances counter[0] by n.

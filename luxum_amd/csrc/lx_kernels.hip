@@ -513,14 +513,18 @@ __global__ __launch_bounds__(256) void k_tfsf(Grid<T> g, const TfBox *__restrict
 // slabs (checked on the host), so there is no psi term.  The extra block
 // runs the aux line's H phase (it writes h; the face cells read e) and
 // records the E probes of the previous step (E is not written here).
-template <typename T>
+// NEXT: the corrections of step t+1 instead, inside the launch between steps
+// t and t+1 (`aux` then holds e_t and h_{t+1/2}; the incident E of step t+1
+// at a face node is computed here with aux_e_next, as the launch's extra
+// block computes the whole line).
+template <typename T, bool NEXT = false>
 __device__ __forceinline__ void hpre_block(const Grid<T> &g, const TfBox *__restrict__ tb_g, int ntf, int total,
                                            const IncParams &inc, const AuxLine &aux, const ProbeArgs &pr, Launch l,
                                            int bid) {
     const int t = g.counter[0] + l.toff;
     const int nb = (total + 255) / 256;
     if (bid >= nb) {
-        extra_block<T, true>(g, aux, pr, l, t);
+        if (!NEXT) extra_block<T, true>(g, aux, pr, l, t);
         return;
     }
     __shared__ TfBox sb[kMaxTf];
@@ -539,8 +543,15 @@ __device__ __forceinline__ void hpre_block(const Grid<T> &g, const TfBox *__rest
     ijk[fa] = bx.lo[fa];
     ijk[u] = bx.lo[u] + loc % nu;
     ijk[v] = bx.lo[v] + loc / nu;
-    const double gi = inc_value(inc, ijk[0] + (double)bx.off[0], ijk[1] + (double)bx.off[1], ijk[2] + (double)bx.off[2],
-                                t - 0.5, true);
+    const double cx = ijk[0] + (double)bx.off[0], cy = ijk[1] + (double)bx.off[1], cz = ijk[2] + (double)bx.off[2];
+    double gi;
+    if (NEXT && inc.mode == LX_INC_AUX1D) {
+        const double cc = inc.axis == 0 ? cx : (inc.axis == 1 ? cy : cz);
+        const int m = (int)floor(cc) - inc.x0;
+        gi = (m < 0 || m >= inc.L) ? 0.0 : aux_e_next(aux, m, t);
+    } else {
+        gi = inc_value(inc, cx, cy, cz, NEXT ? t + 0.5 : t - 0.5, true);
+    }
     const float delta = (float)(bx.dir * gi);
     const int a = (bx.comp + 1 + bx.curl) % 3;
     const float upd = (bx.curl == 0 ? 1.f : -1.f) * g.tab[(a * 6 + TK_KH) * g.tn + ijk[a]] * delta;
@@ -553,6 +564,57 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_tfsf_hpre(Grid<T> g, const TfBox *__restrict__ tb_g, int ntf, int total,
                                                    IncParams inc, AuxLine aux, ProbeArgs pr, Launch l) {
     hpre_block<T>(g, tb_g, ntf, total, inc, aux, pr, l, blockIdx.x);
+}
+
+// Fused 3D step, between steps t and t+1: one launch on the buffer step t
+// wrote does what k_tfsf<E> of step t and k_tfsf_hpre of step t+1 do there.
+// Blocks [0, nbE) apply the E corrections of step t, blocks [nbE, nbE+nbH)
+// the H corrections of step t+1 (H there is what step t+1 reads), the last
+// block moves the aux line from set ain (e_t, h_{t+1/2}) to set aout (e_{t+1}
+// and h_{t+3/2}) and records probes.  No thread reads what another writes:
+// the E corrections read h of ain, the H corrections compute their e_{t+1}
+// nodes from ain themselves, E and H cells are distinct, the extra block
+// writes only aout; the host allows the launch only when no H probe lies on
+// an H face cell.  H probes are those of step t; E probes those of step t-1,
+// read from the other buffer (final there, and nothing writes it now).
+// last: the sequence's final step: no H corrections, aout gets e_{t+1} and
+// h_{t+1/2} (the state the next sequence's k_tfsf_hpre expects).
+template <typename T>
+__global__ __launch_bounds__(256) void k_tfsf_between(Grid<T> g, const T *__restrict__ Fprev,
+                                                      const TfBox *__restrict__ tbE, int ntfE, int totE,
+                                                      const TfBox *__restrict__ tbH, int ntfH, int totH, IncParams inc,
+                                                      AuxLine ain, AuxLine aout, ProbeArgs pr, Launch l, int last) {
+    const int nbE = (totE + 255) / 256, nbH = last ? 0 : (totH + 255) / 256;
+    const int bid = blockIdx.x;
+    if (bid < nbE) {
+        IncParams ie = inc;
+        ie.e = ain.e;
+        ie.h = ain.h;
+        tfsf_block<T, false>(g, tbE, ntfE, totE, ie, ain, pr, l, bid);
+        return;
+    }
+    if (bid < nbE + nbH) {
+        hpre_block<T, true>(g, tbH, ntfH, totH, inc, ain, pr, l, bid - nbE);
+        return;
+    }
+    const int t = g.counter[0] + l.toff;
+    if (l.aux) {
+        for (int m = threadIdx.x; m < ain.L; m += blockDim.x) aout.e[m] = aux_e_next(ain, m, t);
+        __syncthreads();
+        if (!last) {
+            for (int m = threadIdx.x; m < ain.L - 1; m += blockDim.x) aout.h[m] = aux_h_next(ain.h, aout.e, ain.S, m);
+        } else if (aout.h != ain.h) {
+            for (int m = threadIdx.x; m < ain.L - 1; m += blockDim.x) aout.h[m] = ain.h[m];
+        }
+    }
+    if (l.probes && t < pr.cap)
+        for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
+            if (pr.comp[q] >= 3)
+                pr.rec[(long long)t * pr.n + q] = (double)ld1(g.F + pr.comp[q] * g.fstride + pr.idx[q]) * pr.hscale;
+    if (l.eprobes && t >= 1 && t - 1 < pr.cap)
+        for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
+            if (pr.comp[q] < 3)
+                pr.rec[(long long)(t - 1) * pr.n + q] = (double)ld1(Fprev + pr.comp[q] * g.fstride + pr.idx[q]);
 }
 
 // Batched TF/SF launch of one step over every member of a 3D batch: block b
@@ -779,6 +841,17 @@ void launch_tfsf_hpre(const Grid<T> &g, const TfList &tf, const IncParams &inc, 
 }
 
 template <typename T>
+void launch_tfsf_between(const Grid<T> &g, const T *Fprev, const TfList &tfE, const TfList &tfH, const IncParams &inc,
+                         const AuxLine &ain, const AuxLine &aout, const ProbeArgs &pr, Launch l, int last,
+                         hipStream_t s) {
+    const int nbE = (tfE.total + 255) / 256, nbH = last ? 0 : (tfH.total + 255) / 256;
+    const unsigned blocks = (unsigned)(nbE + nbH) + ((l.aux || l.probes || l.eprobes) ? 1u : 0u);
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_tfsf_between<T>, dim3(blocks), dim3(256), 0, s, g, Fprev, tfE.dev, tfE.n, tfE.total, tfH.dev,
+                       tfH.n, tfH.total, inc, ain, aout, pr, l, last);
+}
+
+template <typename T>
 void launch_finish(const Grid<T> &g, const ProbeArgs &pr, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_finish<T>, dim3(1), dim3(64), 0, s, g, pr, n);
 }
@@ -837,6 +910,9 @@ void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
     template void launch_tfsf<T>(const Grid<T> &, const TfList &, const IncParams &, const AuxLine &,        \
                                  const ProbeArgs &, Launch, int, hipStream_t);                               \
     template void launch_finish<T>(const Grid<T> &, const ProbeArgs &, int, hipStream_t);                    \
+    template void launch_tfsf_between<T>(const Grid<T> &, const T *, const TfList &, const TfList &,           \
+                                         const IncParams &, const AuxLine &, const AuxLine &, const ProbeArgs &, \
+                                         Launch, int, hipStream_t);                                           \
     template void launch_tfsf_hpre<T>(const Grid<T> &, const TfList &, const IncParams &, const AuxLine &,   \
                                       const ProbeArgs &, Launch, hipStream_t);                              \
     template void launch_point_source<T>(const Grid<T> &, const PointSrc &, const ProbeArgs &, int, hipStream_t); \

@@ -54,6 +54,7 @@ struct lx_sim {
     hipStream_t ks = nullptr;
     hipEvent_t kfork = nullptr, kjoin = nullptr;
     int kconc = 1;              // use ks (when it exists); autotuned
+    int tf_merge = 1;           // fused step: one TF/SF launch between steps (LX_TFSF_MERGE=0: two per step)
     std::vector<std::pair<void *, size_t>> allocs;   // device blocks and their sizes
     int64_t bytes = 0;
     void *F = nullptr;          // six components, component c at F + c*fstride elements

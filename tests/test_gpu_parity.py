@@ -353,6 +353,62 @@ def test_fused_shell_block_depth(gpu_ctx, monkeypatch):
         assert np.array_equal(s.probes(), ref.probes())
 
 
+def test_tfsf_between_steps(gpu_ctx, monkeypatch):
+    """One TF/SF launch between fused steps (E corrections of step t, H
+    corrections of step t+1, aux line through its second set) against two
+    launches per step: fields and probes bit-identical.  Aux and analytic
+    incidence, FP32 and FP16, step counts that end a sequence on either aux
+    set, graph replay and eager launches, E probes on the TF/SF faces, H
+    probes beside them, a point source with probes, and H probes on H face
+    cells (the host keeps two launches there)."""
+    from luxum_amd.sim import Simulation
+
+    def run(p, steps, merged, graphs, w=None, probes=None):
+        monkeypatch.setenv("LX_FUSED", "1")
+        monkeypatch.setenv("LX_TFSF_MERGE", "1" if merged else "0")
+        monkeypatch.setenv("LX_NO_GRAPH", "0" if graphs else "1")
+        s = Simulation(p, gpu_ctx)
+        if w is not None:
+            s.set_materials(w.ids, w.eps_r)
+        if probes is not None:
+            s.set_probes(probes)
+        assert s.info().fused_step
+        s.step(steps)
+        s.sync()
+        return s
+
+    def same(label, p, steps, w=None, probes=None):
+        for graphs in (True, False):
+            say(f"case {label}, {steps} steps, graphs {graphs}")
+            a = run(p, steps, True, graphs, w, probes)
+            b = run(p, steps, False, graphs, w, probes)
+            for c in p.components():
+                assert np.array_equal(a.field(c), b.field(c)), f"{label} graphs {graphs} component {c}"
+            if probes is not None:
+                assert np.array_equal(a.probes(), b.probes()), f"{label} graphs {graphs} probes"
+            a.close()
+            b.close()
+
+    w = configs.config1(n=48, steps=200, pml=8, margin=6)
+    lo, hi, m = w.p.tf_lo[0], w.p.tf_hi[0], 24
+    pr = np.concatenate([w.probes, np.array([[LX_EZ, lo, m, m], [LX_EX, m, m, hi], [LX_HY, lo - 2, m, m],
+                                             [LX_HZ, m, m + 3, m]], dtype=np.int32)])
+    for steps in (37, 70, 1):
+        same("aux fp32", w.p, steps, w, pr)
+    w16 = configs.config1(n=48, steps=200, pml=8, margin=6, precision=LX_FP16)
+    same("aux fp16", w16.p, 45, w16, pr)
+    hface = np.concatenate([pr, np.array([[LX_HY, lo - 1, m, m]], dtype=np.int32)])
+    same("aux, H probe on an H face", w.p, 40, w, hface)
+    p = default_params(nx=40, ny=36, nz=44, use_pw=1, incident=LX_INC_ANALYTIC, off_pw=90.0,
+                       pw_sigma=(20 * 5e-8) ** 2, tf_lo=(8, 7, 9), tf_hi=(30, 27, 33), courant=0.5)
+    p.k_pw[:] = (0.6, 0.48, 0.64)
+    p.e_pw[:] = (0.6246950475544243, -0.7808688094430304, 0.0)
+    same("analytic", p, 41, None, np.array([[LX_EX, 8, 18, 20], [LX_HY, 20, 18, 20]], dtype=np.int32))
+    ps = default_params(nx=36, ny=34, nz=30, use_pml=1, pml_thickness=6 * 5e-8, use_ps=1, src_kind=LX_SRC_SOFT,
+                        src_amp=1.0, src_lambda0=20 * 5e-8, src_fwhm=3.0, src_delay=4.0, src_pos=(17, 16, 15))
+    same("point source", ps, 35, None, np.array([[LX_EZ, 17, 16, 15], [LX_HX, 17, 18, 15]], dtype=np.int32))
+
+
 def test_fused_shell_beside_interior(gpu_ctx, monkeypatch):
     """The CPML shell launch on a second stream, running at the same time as
     the interior launch, writes the same fields and probes as the shell
