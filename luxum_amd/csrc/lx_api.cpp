@@ -472,11 +472,11 @@ void launch_fused_step(lx_sim *s, const Grid<T> &g) {
     }
 }
 
-// Can the fused sequence run one TF/SF launch between steps?  Not when an H
-// probe lies on an H face cell (that launch records the H probes of a step
-// while it applies the next step's H corrections).
+// Can a fused sequence of this simulation run one TF/SF launch between
+// steps?  Not when an H probe lies on an H face cell (that launch records the
+// H probes of a step while it applies the next step's H corrections).
 bool tf_merge_ok(const lx_sim *s) {
-    if (!s->tf_merge || !s->fused) return false;
+    if (!s->tf_merge) return false;
     for (int q = 0; q < s->np; ++q) {
         const int *c = &s->probe_host[4 * (size_t)q];
         if (c[0] < 3) continue;

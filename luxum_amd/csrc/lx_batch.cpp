@@ -55,9 +55,11 @@ struct Group3 {
     int vec = 1;                              // x cells per lane of the fused step
     std::vector<lx_sim *> sims;
     void *d_sims = nullptr;                   // B3Sim<T>[sims]
-    int2 *d_map = nullptr;                    // block maps: interior, shell, H TF/SF, E TF/SF
-    int nblk[4] = {0, 0, 0, 0};
-    int off[4] = {0, 0, 0, 0};                // first entry of each map in d_map
+    int2 *d_map = nullptr;                    // block maps: interior, shell, H TF/SF, E TF/SF, between steps,
+                                              // between the chunk's last step and the next chunk
+    int nblk[6] = {0, 0, 0, 0, 0, 0};
+    int off[6] = {0, 0, 0, 0, 0, 0};          // first entry of each map in d_map
+    int merge = 0;                            // one TF/SF launch between steps (every member allows it)
     int par = 0;                              // buffer set holding the current state
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // one chunk starting at parity 0 / 1
     int stable = 0;
@@ -134,7 +136,8 @@ void free_group3(Group3 &g) {
     g.sims.clear();
     g.d_sims = nullptr;
     g.d_map = nullptr;
-    for (int k = 0; k < 4; ++k) g.nblk[k] = g.off[k] = 0;
+    for (int k = 0; k < 6; ++k) g.nblk[k] = g.off[k] = 0;
+    g.merge = 0;
 }
 
 bool batch3_ok(const lx_sim *s) {
@@ -207,7 +210,9 @@ lx_status build_group3(lx_batch *b, Group3 &g) {
     auto cells = [&](int q) { return (int64_t)g.sims[q]->N[0] * g.sims[q]->N[1] * g.sims[q]->N[2]; };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cells(x) > cells(y); });
     std::vector<B3Sim<T>> hs(n);
-    std::vector<int2> map[4];
+    std::vector<int2> map[6];
+    g.merge = 1;
+    for (const lx_sim *s : g.sims) g.merge = g.merge && tf_merge_ok(s);
     for (int q : order) {
         lx_sim *s = g.sims[q];
         B3Sim<T> &d = hs[q];
@@ -232,13 +237,15 @@ lx_status build_group3(lx_batch *b, Group3 &g) {
         d.use_aux = s->use_aux;
         d.probes = s->np > 0;
         const int extra = (d.use_aux || d.probes) ? 1 : 0;
-        const int nb[4] = {(d.nunits[0] + kB3Waves - 1) / kB3Waves, (d.nunits[1] + kB3Waves - 1) / kB3Waves,
-                           (d.totH + 255) / 256 + extra, (d.totE + 255) / 256 + extra};
-        for (int k = 0; k < 4; ++k)
+        const int bx = between_extra_blocks(d.use_aux ? d.aux.L : 0, d.probes);
+        const int nb[6] = {(d.nunits[0] + kB3Waves - 1) / kB3Waves, (d.nunits[1] + kB3Waves - 1) / kB3Waves,
+                           (d.totH + 255) / 256 + extra, (d.totE + 255) / 256 + extra,
+                           (d.totE + 255) / 256 + (d.totH + 255) / 256 + bx, (d.totE + 255) / 256 + bx};
+        for (int k = 0; k < 6; ++k)
             for (int j = 0; j < nb[k]; ++j) map[k].push_back(make_int2(q, j));
     }
     size_t total = 0;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 6; ++k) {
         g.off[k] = (int)total;
         g.nblk[k] = (int)map[k].size();
         total += map[k].size();
@@ -252,7 +259,7 @@ lx_status build_group3(lx_batch *b, Group3 &g) {
     st = desc_buf(b, gi, 4, sizeof(int2) * std::max<size_t>(total, 1), &p);
     if (st) return st;
     g.d_map = (int2 *)p;
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 6; ++k)
         if (!map[k].empty())
             HIP_TRY(hipMemcpy(g.d_map + g.off[k], map[k].data(), sizeof(int2) * map[k].size(), hipMemcpyHostToDevice));
     return LX_OK;
@@ -444,7 +451,7 @@ void enqueue3_t(const lx_batch *b, const Group3 &g, int nt, int par) {
     const bool fork = b->shell_side && g.nblk[0] && g.nblk[1];
     for (int q = 0; q < nt; ++q) {
         const int pq = (par + q) & 1;
-        launch_tfsf_batch<T>(sims, g.d_map + g.off[2], g.nblk[2], 1, pq, q, s);
+        if (!g.merge || q == 0) launch_tfsf_batch<T>(sims, g.d_map + g.off[2], g.nblk[2], 1, pq, q, s);
         if (fork) {
             (void)hipEventRecord(b->fork, s);
             (void)hipStreamWaitEvent(b->side, b->fork, 0);
@@ -456,7 +463,13 @@ void enqueue3_t(const lx_batch *b, const Group3 &g, int nt, int par) {
             launch_step3_batch<T>(sims, g.d_map + g.off[0], g.nblk[0], 0, pq, g.vec, s);
             launch_step3_batch<T>(sims, g.d_map + g.off[1], g.nblk[1], 1, pq, g.vec, s);
         }
-        launch_tfsf_batch<T>(sims, g.d_map + g.off[3], g.nblk[3], 0, pq, q, s);
+        if (!g.merge) {
+            launch_tfsf_batch<T>(sims, g.d_map + g.off[3], g.nblk[3], 0, pq, q, s);
+        } else if (q == nt - 1) {
+            launch_tfsf_between_batch<T>(sims, g.d_map + g.off[5], g.nblk[5], pq, q, 1, s);
+        } else {
+            launch_tfsf_between_batch<T>(sims, g.d_map + g.off[4], g.nblk[4], pq, q, 0, s);
+        }
     }
     launch_finish_batch<T>(sims, (int)g.sims.size(), nt, (par + nt) & 1, s);
 }

@@ -84,6 +84,12 @@ template <typename T>
 void launch_tfsf_batch(const B3Sim<T> *sims, const int2 *bmap, int nblocks, int pre, int par, int toff,
                        hipStream_t s);
 // Sequence end of every member after n steps: last E probes, step counter.
+// One TF/SF launch between steps toff and toff+1 of every member (the E
+// corrections of step toff, the H corrections of step toff+1 unless last);
+// bmap lists each member's E and H correction blocks and its extra block.
+template <typename T>
+void launch_tfsf_between_batch(const B3Sim<T> *sims, const int2 *bmap, int nblocks, int par, int toff, int last,
+                               hipStream_t s);
 template <typename T>
 void launch_finish_batch(const B3Sim<T> *sims, int nsims, int n, int par_end, hipStream_t s);
 

@@ -213,6 +213,12 @@ void launch_tfsf_hpre(const Grid<T> &g, const TfList &tf, const IncParams &inc, 
 // aout, the H probes of step t and (l.eprobes) the E probes of step t-1.
 // last: no H corrections (the sequence's final step).  Requires no H probe
 // on an H face cell.
+// Blocks after the face-cell blocks of such a launch: one thread per aux
+// node (L nodes, 0 without an aux line), at least one when there are probes.
+inline int between_extra_blocks(int L, bool probes) {
+    const int n = (L + 255) / 256;
+    return n > 0 ? n : (probes ? 1 : 0);
+}
 template <typename T>
 void launch_tfsf_between(const Grid<T> &g, const T *Fprev, const TfList &tfE, const TfList &tfH, const IncParams &inc,
                          const AuxLine &ain, const AuxLine &aout, const ProbeArgs &pr, Launch l, int last,

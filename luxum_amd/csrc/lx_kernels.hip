@@ -580,12 +580,12 @@ __global__ __launch_bounds__(256) void k_tfsf_hpre(Grid<T> g, const TfBox *__res
 // last: the sequence's final step: no H corrections, aout gets e_{t+1} and
 // h_{t+1/2} (the state the next sequence's k_tfsf_hpre expects).
 template <typename T>
-__global__ __launch_bounds__(256) void k_tfsf_between(Grid<T> g, const T *__restrict__ Fprev,
-                                                      const TfBox *__restrict__ tbE, int ntfE, int totE,
-                                                      const TfBox *__restrict__ tbH, int ntfH, int totH, IncParams inc,
-                                                      AuxLine ain, AuxLine aout, ProbeArgs pr, Launch l, int last) {
+__device__ __forceinline__ void between_block(const Grid<T> &g, const T *__restrict__ Fprev,
+                                              const TfBox *__restrict__ tbE, int ntfE, int totE,
+                                              const TfBox *__restrict__ tbH, int ntfH, int totH, const IncParams &inc,
+                                              const AuxLine &ain, const AuxLine &aout, const ProbeArgs &pr, Launch l,
+                                              int last, int bid) {
     const int nbE = (totE + 255) / 256, nbH = last ? 0 : (totH + 255) / 256;
-    const int bid = blockIdx.x;
     if (bid < nbE) {
         IncParams ie = inc;
         ie.e = ain.e;
@@ -597,16 +597,23 @@ __global__ __launch_bounds__(256) void k_tfsf_between(Grid<T> g, const T *__rest
         hpre_block<T, true>(g, tbH, ntfH, totH, inc, ain, pr, l, bid - nbE);
         return;
     }
+    // aux line: one thread per node over between_aux_blocks blocks; the H
+    // phase's e_{t+1}[m+1] is recomputed by the node's own thread (the same
+    // routine and inputs as its owner), so no block waits for another
     const int t = g.counter[0] + l.toff;
-    if (l.aux) {
-        for (int m = threadIdx.x; m < ain.L; m += blockDim.x) aout.e[m] = aux_e_next(ain, m, t);
-        __syncthreads();
-        if (!last) {
-            for (int m = threadIdx.x; m < ain.L - 1; m += blockDim.x) aout.h[m] = aux_h_next(ain.h, aout.e, ain.S, m);
-        } else if (aout.h != ain.h) {
-            for (int m = threadIdx.x; m < ain.L - 1; m += blockDim.x) aout.h[m] = ain.h[m];
+    const int xb = bid - nbE - nbH;
+    if (l.aux && xb < (ain.L + 255) / 256) {
+        const int m = xb * 256 + threadIdx.x;
+        if (m < ain.L) {
+            const double e0 = aux_e_next(ain, m, t);
+            aout.e[m] = e0;
+            if (m < ain.L - 1) {
+                if (!last) aout.h[m] = fma(-ain.S, aux_e_next(ain, m + 1, t) - e0, ain.h[m]);   // aux_h_next
+                else if (aout.h != ain.h) aout.h[m] = ain.h[m];
+            }
         }
     }
+    if (xb != 0) return;
     if (l.probes && t < pr.cap)
         for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
             if (pr.comp[q] >= 3)
@@ -615,6 +622,14 @@ __global__ __launch_bounds__(256) void k_tfsf_between(Grid<T> g, const T *__rest
         for (int q = threadIdx.x; q < pr.n; q += blockDim.x)
             if (pr.comp[q] < 3)
                 pr.rec[(long long)(t - 1) * pr.n + q] = (double)ld1(Fprev + pr.comp[q] * g.fstride + pr.idx[q]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_tfsf_between(Grid<T> g, const T *__restrict__ Fprev,
+                                                      const TfBox *__restrict__ tbE, int ntfE, int totE,
+                                                      const TfBox *__restrict__ tbH, int ntfH, int totH, IncParams inc,
+                                                      AuxLine ain, AuxLine aout, ProbeArgs pr, Launch l, int last) {
+    between_block<T>(g, Fprev, tbE, ntfE, totE, tbH, ntfH, totH, inc, ain, aout, pr, l, last, blockIdx.x);
 }
 
 // Batched TF/SF launch of one step over every member of a 3D batch: block b
@@ -635,6 +650,36 @@ __global__ __launch_bounds__(256) void k_tfsf_batch(const B3Sim<T> *__restrict__
     const Launch l{toff, d->use_aux, d->probes, d->probes};   // members have no point source
     if (PRE) hpre_block<T>(g, d->tfH, d->ntfH, d->totH, d->inc, d->aux, d->pr, l, mb.y);
     else tfsf_block<T, false>(g, d->tfE, d->ntfE, d->totE, d->inc, d->aux, d->pr, l, mb.y);
+}
+
+// The launch between steps toff and toff+1 of every member of a 3D batch
+// (k_tfsf_between per member): step toff read buffer set par and wrote set
+// par ^ 1; the member's aux line goes from set (toff & 1) to set
+// ((toff + 1) & 1), or to set 0 at the chunk's last step.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tfsf_between_batch(const B3Sim<T> *__restrict__ sims,
+                                                            const int2 *__restrict__ bmap, int par, int toff, int last) {
+    const int2 mb = bmap[blockIdx.x];
+    const B3Sim<T> *d = sims + mb.x;
+    Grid<T> g = d->g;
+    const T *Fprev = g.Fn;
+    if (par == 0) {
+        Fprev = g.F;
+        g.F = g.Fn;
+        g.psi = g.psin;
+    }
+    const Launch l{toff, d->use_aux, d->probes, d->probes && toff >= 1};
+    AuxLine ain = d->aux, aout = d->aux;
+    if (toff & 1) {
+        ain.e = d->aux.e2;
+        ain.h = d->aux.h2;
+    }
+    if (!last && !(toff & 1)) {
+        aout.e = d->aux.e2;
+        aout.h = d->aux.h2;
+    }
+    between_block<T>(g, Fprev, d->tfE, d->ntfE, d->totE, d->tfH, d->ntfH, d->totH, d->inc, ain, aout, d->pr, l, last,
+                     mb.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -845,7 +890,7 @@ void launch_tfsf_between(const Grid<T> &g, const T *Fprev, const TfList &tfE, co
                          const AuxLine &ain, const AuxLine &aout, const ProbeArgs &pr, Launch l, int last,
                          hipStream_t s) {
     const int nbE = (tfE.total + 255) / 256, nbH = last ? 0 : (tfH.total + 255) / 256;
-    const unsigned blocks = (unsigned)(nbE + nbH) + ((l.aux || l.probes || l.eprobes) ? 1u : 0u);
+    const unsigned blocks = (unsigned)(nbE + nbH + between_extra_blocks(l.aux ? ain.L : 0, l.probes || l.eprobes));
     if (blocks == 0) return;
     hipLaunchKernelGGL(k_tfsf_between<T>, dim3(blocks), dim3(256), 0, s, g, Fprev, tfE.dev, tfE.n, tfE.total, tfH.dev,
                        tfH.n, tfH.total, inc, ain, aout, pr, l, last);
@@ -861,6 +906,12 @@ void launch_tfsf_batch(const B3Sim<T> *sims, const int2 *bmap, int nblocks, int 
     if (nblocks <= 0) return;
     if (pre) hipLaunchKernelGGL((k_tfsf_batch<T, true>), dim3(nblocks), dim3(256), 0, s, sims, bmap, par, toff);
     else hipLaunchKernelGGL((k_tfsf_batch<T, false>), dim3(nblocks), dim3(256), 0, s, sims, bmap, par, toff);
+}
+template <typename T>
+void launch_tfsf_between_batch(const B3Sim<T> *sims, const int2 *bmap, int nblocks, int par, int toff, int last,
+                               hipStream_t s) {
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL(k_tfsf_between_batch<T>, dim3(nblocks), dim3(256), 0, s, sims, bmap, par, toff, last);
 }
 template <typename T>
 void launch_finish_batch(const B3Sim<T> *sims, int nsims, int n, int par_end, hipStream_t s) {
@@ -920,7 +971,8 @@ void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
     template void launch_monitor<T>(const Grid<T> &, int, double *, hipStream_t);                            \
     template void launch_gather_e2<T>(const Grid<T> &, int, int, int, double *, hipStream_t);              \
     template void launch_tfsf_batch<T>(const B3Sim<T> *, const int2 *, int, int, int, int, hipStream_t);   \
-    template void launch_finish_batch<T>(const B3Sim<T> *, int, int, int, hipStream_t);
+    template void launch_finish_batch<T>(const B3Sim<T> *, int, int, int, hipStream_t);                   \
+    template void launch_tfsf_between_batch<T>(const B3Sim<T> *, const int2 *, int, int, int, int, hipStream_t);
 LX_INST(float)
 LX_INST(__half)
 

@@ -113,6 +113,8 @@ ProbeArgs probe_args(const lx_sim *s);
 void drop_graph(lx_sim *s);
 bool fused_ok(const lx_sim *s);
 bool vec_ok(const lx_sim *s, int vec);
+// fused sequences may apply one TF/SF launch between steps (no H probe on an H face cell)
+bool tf_merge_ok(const lx_sim *s);
 // The side stream (snapshot copies) is taken from the context's pool on
 // first use.
 lx_status ensure_side_stream(lx_sim *s);
