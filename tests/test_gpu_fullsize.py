@@ -1,0 +1,123 @@
+"""GPU checks at the full sizes of the 3D workloads (256^3 and
+1280x1280x640), through properties that hold at any grid size: the FP64
+oracle needs minutes per step-thousand there, so it checks the same
+properties on small grids instead (tests/test_oracle.py) and the small-grid
+GPU runs directly.
+
+- Mirror symmetry: the 256^3 workload's grid, CPML 12 and aux TF/SF plane
+  wave (along +x, Ez polarised) with its eps 6 cube stretched over y and z
+  into a slab is symmetric about the y and z centre planes, so every
+  component is its own mirror image up to the parity the reflections fix.
+  A wrong index, sign or coefficient on one side of a slab, a TF/SF face or
+  a launch boundary breaks it.
+- Translation: a soft Ez dipole in the PhC-slab grid, placed 64 cells from
+  the CPML and 208 above the slab, stays in vacuum for 60 steps (the update
+  moves information one cell per step), so its field there is the translate
+  of the same dipole's field on a 160^3 vacuum grid, which the oracle
+  checks; everywhere else the 1.05e9-node grid must stay exactly zero.
+
+Both run on the configuration the autotuner picks, as the bench does.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import MIRROR_PARITY, mirror_asymmetry, rel_l2
+from luxum_amd import configs
+from luxum_amd.params import LX_EZ, LX_FP16, LX_FP32, LX_HY
+from oracle.lxoracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("prec", [LX_FP32, LX_FP16], ids=["fp32", "fp16"])
+def test_mirror_symmetry_256(gpu_ctx, prec):
+    from luxum_amd.sim import Simulation
+    n, steps = 256, 640
+    w = configs.config1(n=n, steps=steps, precision=prec)
+    ids = np.zeros_like(w.ids)
+    ids[n // 2 - n // 8:n // 2 + n // 8] = 1          # the cube's x extent, over all y and z
+    say(f"case mirror symmetry {n}^3 prec={prec} steps={steps}")
+    s = Simulation(w.p, gpu_ctx)
+    s.set_materials(ids, w.eps_r)
+    kz = s.autotune()
+    s.step(steps)
+    s.sync()
+    inf = s.info()
+    say(f"  autotuned kz={kz} fused={inf.fused_step} cells_per_lane={inf.cells_per_lane}")
+    worst = 0.0
+    for c in w.p.components():
+        f = s.field(c)
+        assert np.all(np.isfinite(f))
+        assert np.linalg.norm(f) > 0, c
+        py, pz = MIRROR_PARITY[c]
+        ay, az = mirror_asymmetry(f, 1, py), mirror_asymmetry(f, 2, pz)
+        say(f"  comp {c}: asymmetry y {ay:.3e} z {az:.3e}")
+        assert mirror_asymmetry(f, 1, -py) > 1.9 and mirror_asymmetry(f, 2, -pz) > 1.9
+        worst = max(worst, ay, az)
+    s.close()
+    assert worst <= 1e-6, worst
+
+
+@pytest.mark.parametrize("prec", [LX_FP32, LX_FP16], ids=["fp32", "fp16"])
+def test_dipole_translation_phc_full_size(gpu_ctx, prec):
+    from luxum_amd.sim import Simulation
+    steps, half = 60, 62                     # field support after `steps` steps: within `steps` cells
+    src_big, src_small = (1200, 1200, 560), (80, 80, 80)
+    shift = tuple(a - b for a, b in zip(src_big, src_small))
+
+    small = configs.config2(nxy=160, nz=160, steps=steps, precision=prec)
+    small.p.src_pos[:] = src_small
+    small.p.src_delay = 0.5      # envelope near its peak over the 60 steps (FP16 keeps ~1e-7 and up only)
+    vac = np.zeros_like(small.ids)
+    say(f"case dipole 160^3 vacuum prec={prec} steps={steps}")
+    ss = Simulation(small.p, gpu_ctx)
+    ss.set_materials(vac, small.eps_r)
+    ss.step(steps)
+    ss.sync()
+    o = OracleSim(small.p)
+    o.set_materials(vac, small.eps_r)
+    o.step(steps)
+    ref = {c: ss.field(c) for c in (LX_EZ, LX_HY)}
+    e_or = rel_l2(ref[LX_EZ], o.field(LX_EZ))
+    say(f"  160^3 GPU vs FP64 oracle: Ez rel L2 {e_or:.3e}")
+    assert e_or <= (1e-5 if prec == LX_FP32 else 5e-2)
+    o.close()
+    mon_small = ss.monitor()
+    ss.close()
+
+    big = configs.config2(steps=steps, precision=prec)
+    big.p.src_pos[:] = src_big
+    big.p.src_delay = small.p.src_delay
+    say(f"case dipole in the PhC slab grid {big.p.dims} prec={prec} steps={steps}")
+    sb = Simulation(big.p, gpu_ctx)
+    sb.set_materials(big.ids, big.eps_r)
+    del big
+    kz = sb.autotune()
+    sb.step(steps)
+    sb.sync()
+    inf = sb.info()
+    say(f"  autotuned kz={kz} fused={inf.fused_step} cells_per_lane={inf.cells_per_lane}")
+    mon_big = sb.monitor()
+    say(f"  monitor max|E|^2 / energy: {mon_big} vs 160^3 {mon_small}")
+    assert mon_big[0] > 0
+    assert mon_big[0] == pytest.approx(mon_small[0], rel=1e-12)
+    assert mon_big[1] == pytest.approx(mon_small[1], rel=1e-12)   # every cell of all six components
+    for c in (LX_EZ, LX_HY):
+        f = sb.field(c)
+        win = tuple(slice(a + b - half, a + b + half + 1) for a, b in zip(src_small, shift))
+        sw = tuple(slice(a - half, a + half + 1) for a in src_small)
+        d = float(np.max(np.abs(f[win] - ref[c][sw])))
+        nz_all, nz_win = int(np.count_nonzero(f)), int(np.count_nonzero(f[win]))
+        say(f"  comp {c}: max |window - 160^3| {d:.3e}, nonzero cells {nz_all} (window {nz_win})")
+        assert nz_win > 0
+        assert nz_all == nz_win                   # nothing outside the dipole's reach
+        assert rel_l2(f[win], ref[c][sw]) <= 1e-6      # the same arithmetic: equal in practice
+        del f
+    sb.close()

@@ -362,3 +362,45 @@ def test_hard_cw_point_source_semantics():
         hard = 3 * math.sin(prod * dt)
         assert s.field(LX_EZ)[6, 6, 6] == pytest.approx(hard + dt / eps0 * curl, rel=1e-12, abs=1e-22)
     assert abs(hard) > 1e-8
+
+
+# ---------------------------------------------------------------------------
+# mirror symmetry of the cube workload (used at full size on the GPU)
+# ---------------------------------------------------------------------------
+
+def test_slab_workload_mirror_parities():
+    """The cube workload's grid, CPML and aux TF/SF plane wave (along +x, Ez
+    polarised) with the eps 6 cube stretched over y and z into a slab: the
+    discrete problem is symmetric about the y and z centre planes, and the
+    FP64 oracle's fields are exact mirror images with the parities that the
+    reflections of E (polar) and H (axial) predict.  (The cube itself is not
+    symmetric on the grid: a node's material sets the coefficient of all
+    three E components at that array index, maxwell_three_dim_pml.cc:138-148,
+    whatever their stagger, so the cube is half a cell longer for the
+    staggered component along each axis.)  The same check flags one planted
+    wrong cell and a wrong parity."""
+    from conftest import MIRROR_PARITY, mirror_asymmetry
+    from luxum_amd import configs
+    w = configs.config1(n=32, steps=110, pml=6, margin=4)
+    ids = np.zeros_like(w.ids)
+    ids[12:20] = 1
+    o = O.OracleSim(w.p)
+    o.set_materials(ids, w.eps_r)
+    o.step(110)
+    for c in w.p.components():
+        f = o.field(c)
+        assert np.linalg.norm(f) > 0, c
+        py, pz = MIRROR_PARITY[c]
+        assert mirror_asymmetry(f, 1, py) == 0.0, (c, "y")
+        assert mirror_asymmetry(f, 2, pz) == 0.0, (c, "z")
+        assert mirror_asymmetry(f, 1, -py) > 1.9, (c, "y parity flipped")
+        assert mirror_asymmetry(f, 2, -pz) > 1.9, (c, "z parity flipped")
+        g = f.copy()
+        g[5, 7, 9] += 1e-3 * np.abs(f).max()      # one wrong cell off the centre planes
+        assert mirror_asymmetry(g, 1, py) > 1e-5
+    o2 = O.OracleSim(w.p)                         # the cube: not a mirror image
+    o2.set_materials(w.ids, w.eps_r)
+    o2.step(110)
+    assert mirror_asymmetry(o2.field(2), 2, +1) > 1e-3
+    o.close()
+    o2.close()
