@@ -9,8 +9,12 @@ waveguide (soft Ez dipole source) instead, main line only.
 
 Prints ONE JSON line: Gcell-updates/s (whole job), the roofline of the
 dominant kernel (algorithmic bytes per launch / average launch duration,
-timed with HIP events on the simulation's stream) and a CPU baseline (the
-FP64 reference-structured oracle on a bounded sample of the same workload).
+timed with HIP events on the simulation's stream), that kernel's HBM bytes
+per launch ("traffic": FETCH_SIZE and WRITE_SIZE from two rocprofv3 --pmc
+runs of a short child bench with the same tuned configuration, each scaled
+by what the counter reads for a plain device copy of known size) and a CPU
+baseline (the FP64 reference-structured oracle on a bounded sample of the
+same workload).
 
 More objects ride on the same line:
   "batched": 256 seeded 2D photonic-crystal simulations of sizes
@@ -325,6 +329,83 @@ def snapshot_section(args, w, ctx, sim):
     return out
 
 
+def _pmc_pass(counter, args, tune, outdir):
+    """One `rocprofv3 --pmc <counter>` run of a short bench.py child (same
+    workload, the tuned configuration pinned through LX_TUNE, plus a device
+    copy of known size); returns {dispatch id: (kernel name, value)}."""
+    import csv
+    import glob
+    import signal
+    import subprocess
+    prof = None
+    for cand in (os.environ.get("ROCPROFV3"), "/opt/rocm/bin/rocprofv3"):
+        if cand and os.path.exists(cand):
+            prof = cand
+            break
+    if prof is None:
+        raise RuntimeError("rocprofv3 not found")
+    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", outdir, "-o", counter.lower(), "--",
+           sys.executable, os.path.join(ROOT, "bench.py"), "--workload", args.workload, "--n", str(args.n),
+           "--steps", "4", "--warmup", "1", "--kernel-steps", "2", "--copy-calib", str(args.traffic_copy_mib),
+           "--no-batch", "--no-serve", "--no-cpu", "--no-snap", "--no-traffic"]
+    if args.fp16:
+        cmd.append("--fp16")
+    env = dict(os.environ, LX_TUNE=tune, TMPDIR="/tmp")
+    print(f"bench: traffic pass {counter} (LX_TUNE={tune})", file=sys.stderr, flush=True)
+    p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                         start_new_session=True)
+    try:
+        _, err = p.communicate(timeout=args.traffic_timeout)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.communicate()
+        raise RuntimeError(f"rocprofv3 --pmc {counter} timed out")
+    rows = {}
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") != counter:
+                    continue
+                d = r["Dispatch_Id"]
+                name, v = rows.get(d, (r["Kernel_Name"], 0.0))
+                rows[d] = (name, v + float(r["Counter_Value"]))
+    if not rows:
+        # the profiler can exit non-zero after complete results; only missing rows are a failure
+        raise RuntimeError(f"rocprofv3 --pmc {counter}: no counter rows (exit {p.returncode}): "
+                           f"{err.decode(errors='replace')[-400:]}")
+    return rows
+
+
+def traffic_section(args, info, dom_pattern):
+    """HBM bytes per launch of the dominant kernel: FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch) from two rocprofv3 --pmc runs, each scaled by what the
+    same counter reads for a plain device copy of known size in the same run."""
+    import re
+    import shutil
+    import tempfile
+    import statistics
+    nw = info.waves if info.waves in (4, 8) else 4
+    tune = f"{int(info.fused_step)},{nw},{info.cells_per_lane},{info.kz},{int(info.shell_concurrent)}"
+    copy_bytes = args.traffic_copy_mib * (1 << 20)
+    pat = re.compile(dom_pattern)
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        outdir = tempfile.mkdtemp(prefix="lx_pmc_", dir="/tmp")
+        try:
+            rows = _pmc_pass(counter, args, tune, outdir)
+        finally:
+            shutil.rmtree(outdir, ignore_errors=True)
+        dom = [v for name, v in rows.values() if pat.search(name)]
+        cp = [v for name, v in rows.values() if "copy" in name.lower()]
+        if not dom or not cp:
+            raise RuntimeError(f"{counter}: no dispatch of the dominant kernel or of the copy kernel")
+        factor = copy_bytes / (max(cp) * 1024.0)
+        res[counter] = (statistics.median(dom) * 1024.0 * factor, factor, len(dom))
+    rd, wr = res["FETCH_SIZE"], res["WRITE_SIZE"]
+    return {"read": round(rd[0]), "write": round(wr[0]), "fetch_factor": round(rd[1], 3),
+            "write_factor": round(wr[1], 3), "dispatches": rd[2], "copy_MiB": args.traffic_copy_mib}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -355,6 +436,10 @@ def main():
     ap.add_argument("--snap-every", type=int, default=50)
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (PMC traffic calibration)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes that measure the dominant kernel's HBM bytes")
+    ap.add_argument("--traffic-copy-mib", type=int, default=512)
+    ap.add_argument("--traffic-timeout", type=float, default=240.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -413,14 +498,15 @@ def main():
         # fused H+E step: the interior-box launch (ms_h) and, with CPML, the
         # shell launch around it (ms_e); the box launch is the dominant kernel
         dom, dbytes, dms = "k_step3 interior box (fused H+E step)", info.core_bytes, ms_h
+        dom_pat = r"k_step3<[^<>,]+, [01], \d+, \d+>"
         fused_frac = info.bytes_per_step / ((ms_h + ms_e) * 1e-3) / 1e9 / HBM_PEAK_GBS
     else:
         bytes_h = info.cells * 9 * s + slab * 16
         bytes_e = info.cells * (9 * s + 1) + slab * 16
         if ms_e >= ms_h:
-            dom, dbytes, dms = "e_update_3d", bytes_e, ms_e
+            dom, dbytes, dms, dom_pat = "e_update_3d", bytes_e, ms_e, r"k_e3<"
         else:
-            dom, dbytes, dms = "h_update_3d", bytes_h, ms_h
+            dom, dbytes, dms, dom_pat = "h_update_3d", bytes_h, ms_h, r"k_h3<"
     achieved = dbytes / (dms * 1e-3) / 1e9
     step_gbs = info.bytes_per_step / (ms_step * 1e-3) / 1e9
     mx, _ = sim.monitor()
@@ -453,6 +539,14 @@ def main():
     if not args.no_snap and args.workload == "cube":
         out["snapshots"] = snapshot_section(args, w, ctx, sim)
     sim.close()
+    if not args.no_traffic:
+        try:
+            t = traffic_section(args, info, dom_pat)
+            out["roofline"]["traffic"] = t["read"] + t["write"]
+            out["roofline"]["traffic_detail"] = t
+            print(f"bench: dominant kernel HBM traffic {t}", file=sys.stderr, flush=True)
+        except Exception as exc:   # traffic stays null; the reason goes to stderr
+            print(f"bench: traffic not measured: {exc}", file=sys.stderr, flush=True)
     if not args.no_batch:
         out["batched"] = batch_section(args)
         if args.batch3d_sims > 0:

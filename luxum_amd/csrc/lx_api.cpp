@@ -1269,6 +1269,32 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
             return LX_OK;
         }
     }
+    // LX_TUNE=fused,nw,vec,kz,shell_concurrent pins the result without timing
+    // (a profiling process repeats another process's choice)
+    if (const char *pin = std::getenv("LX_TUNE"); pin && !s->planar) {
+        int f = 0, nw = 0, vec = 0, kz = 0, conc = 0;
+        if (std::sscanf(pin, "%d,%d,%d,%d,%d", &f, &nw, &vec, &kz, &conc) != 5 || kz < 1 || (nw != 4 && nw != 8) ||
+            !vec_ok(s, vec))
+            return fail(LX_EINVAL, "LX_TUNE: expected fused,nw,vec,kz,shell_concurrent valid for this grid, got '%s'",
+                        pin);
+        s->kz = kz;
+        s->nw = nw;
+        s->vec = vec;
+        s->kzs = 0;
+        s->kconc = conc ? 1 : 0;
+        if (f && fused_ok(s)) {
+            lx_status st = ensure_fused_buffers(s);
+            if (st) return st;
+            s->fused = 1;
+        } else {
+            s->fused = 0;
+            if (s->F_alt) release_fused_buffers(s);
+        }
+        s->gen++;
+        drop_graph(s);
+        if (kz_out) *kz_out = s->kz;
+        return LX_OK;
+    }
     if (!s->planar) {
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
