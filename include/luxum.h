@@ -108,6 +108,11 @@ typedef struct lx_params {
     double src_lambda0, src_fwhm, src_delay;
     int32_t snapshot_every;     /* 0: none */
     int32_t monitor_every;      /* 0: none; else FP64 max|E|^2 + NaN guard every n steps */
+    double storage_scale;       /* fields are stored times this power of two and every output divides it
+                                   out (results equal the unscaled run's wherever no value leaves the
+                                   storage type's normal range); 0: automatic, FP32 1, FP16 the power of
+                                   two nearest 1/(largest source amplitude), so that binary16's normal
+                                   range (6.1e-5 .. 65504) holds the field */
 } lx_params;
 
 typedef struct lx_ctx lx_ctx;

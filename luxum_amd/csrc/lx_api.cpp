@@ -348,7 +348,7 @@ lx_status build_tfsf(lx_sim *s) {
                     bx.comp = tgt % 3;
                     bx.curl = which - 1;
                     bx.fa = a;
-                    bx.dir = dir;
+                    bx.dir = dir * s->sscale;
                     bx.lo[a] = bx.hi[a] = idx;
                     bx.off[a] = (float)(coord - idx);
                     for (int b : {b0, b1}) {
@@ -843,6 +843,18 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
     s->p = *p;
     s->planar = p->is_planar;
     s->fp16 = p->precision == LX_FP16;
+    s->sscale = p->storage_scale;
+    if (s->sscale == 0.0) {
+        // automatic: FP16 storage keeps the largest source amplitude near 1
+        double amp = 0;
+        if (p->use_pw) amp = std::max(amp, std::sqrt(p->e_pw[0] * p->e_pw[0] + p->e_pw[1] * p->e_pw[1] +
+                                                     p->e_pw[2] * p->e_pw[2]));
+        if (p->use_ps)
+            amp = std::max(amp, std::fabs(p->src_kind == LX_SRC_HARD_CW ? (double)(int)p->src_amp : p->src_amp));
+        s->sscale = 1.0;
+        if (s->fp16 && amp > 0 && std::isfinite(amp))
+            s->sscale = std::ldexp(1.0, std::max(-60, std::min(60, -(int)std::lround(std::log2(amp)))));
+    }
     s->pml = p->use_pml;
     s->N[0] = p->nx; s->N[1] = p->is_planar ? 1 : p->ny; s->N[2] = p->nz;
     s->px = (s->N[0] + 63) / 64 * 64;
@@ -943,6 +955,7 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
             ps.amp = p->src_amp;
             ps.omega = 0;
         }
+        ps.amp *= s->sscale;
         ps.dt = s->dt;
         ps.c = s->c0;
         ps.lambda0 = p->src_lambda0;
@@ -1077,7 +1090,7 @@ lx_status lx_sim_sync(lx_sim *s) {
 }
 
 static lx_status gather(lx_sim *s, int comp, double *dst_dev, int ext[3]) {
-    const double scale = comp >= 3 ? 1.0 / s->eta0 : 1.0;
+    const double scale = (comp >= 3 ? 1.0 / s->eta0 : 1.0) / s->sscale;
     if (s->fp16) launch_gather(make_grid<__half>(s), comp, ext[0], ext[1], ext[2], scale, dst_dev, s->cs);
     else launch_gather(make_grid<float>(s), comp, ext[0], ext[1], ext[2], scale, dst_dev, s->cs);
     return check_launch();
@@ -1091,8 +1104,9 @@ lx_status gather_output(lx_sim *s, int what, int comp, double *dst_dev, const in
         int e[3] = {ext[0], ext[1], ext[2]};
         return gather(s, comp, dst_dev, e);
     }
-    if (s->fp16) launch_gather_e2(make_grid<__half>(s), ext[0], ext[1], ext[2], dst_dev, s->cs);
-    else launch_gather_e2(make_grid<float>(s), ext[0], ext[1], ext[2], dst_dev, s->cs);
+    const double e2scale = 1.0 / (s->sscale * s->sscale);
+    if (s->fp16) launch_gather_e2(make_grid<__half>(s), ext[0], ext[1], ext[2], e2scale, dst_dev, s->cs);
+    else launch_gather_e2(make_grid<float>(s), ext[0], ext[1], ext[2], e2scale, dst_dev, s->cs);
     return check_launch();
 }
 }  // namespace lx
@@ -1161,6 +1175,8 @@ lx_status lx_sim_read_probes(lx_sim *s, double *dst, size_t n, int32_t *n_steps)
     if (!s->np) return LX_OK;
     if (n < m || !dst) return fail(LX_EINVAL, "probe buffer too small: need %zu", m);
     HIP_TRY(hipMemcpy(dst, s->prec, sizeof(double) * m, hipMemcpyDeviceToHost));
+    if (s->sscale != 1.0)
+        for (size_t q = 0; q < m; ++q) dst[q] /= s->sscale;
     return LX_OK;
 }
 
@@ -1177,8 +1193,9 @@ lx_status lx_sim_monitor(lx_sim *s, double *max_e2, double *energy) {
     HIP_TRY(hipMemcpyAsync(out, s->mon, sizeof(double) * 3, hipMemcpyDeviceToHost, s->cs));
     HIP_TRY(hipStreamSynchronize(s->cs));
     const double vol = s->p.dx * s->p.dz * (s->planar ? 1.0 : s->p.dy);
-    if (max_e2) *max_e2 = out[0];
-    if (energy) *energy = 0.5 * s->eps0 * out[1] * vol;
+    const double s2 = s->sscale * s->sscale;
+    if (max_e2) *max_e2 = out[0] / s2;
+    if (energy) *energy = 0.5 * s->eps0 * out[1] * vol / s2;
     if (out[2] > 0) return fail(LX_EUNSTABLE, "non-finite field values after %d steps", s->steps);
     return LX_OK;
 }

@@ -248,7 +248,7 @@ void launch_gather(const Grid<T> &g, int comp, int ex, int ey, int ez, double sc
 // |E|^2 at equal storage indices, FP64, into the Ez-shaped host layout
 // (ex, ey, ez = Ez extents), zero outside the reference's loop range.
 template <typename T>
-void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double *dst, hipStream_t s);
+void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double scale, double *dst, hipStream_t s);
 
 // FP64 monitor: out[0] = max |E|^2 (reference index range), out[1] = sum of
 // E^2 + Hs^2, out[2] = non-finite count.

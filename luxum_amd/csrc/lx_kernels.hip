@@ -751,7 +751,7 @@ __global__ __launch_bounds__(256) void k_gather(Grid<T> g, const T *F, int ex, i
             if (E2) {
                 if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
                     const double a = ld1(g.F + o), b = ld1(g.F + g.fstride + o), c = ld1(g.F + 2 * g.fstride + o);
-                    v = c * c + b * b + a * a;
+                    v = (c * c + b * b + a * a) * scale;
                 }
             } else {
                 v = (double)ld1(F + o) * scale;
@@ -929,9 +929,9 @@ void launch_gather(const Grid<T> &g, int comp, int ex, int ey, int ez, double sc
     hipLaunchKernelGGL((k_gather<T, false>), grd, blk, 0, s, g, F, ex, ey, ez, scale, dst);
 }
 template <typename T>
-void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double *dst, hipStream_t s) {
+void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double scale, double *dst, hipStream_t s) {
     dim3 blk(256), grd((ex + 63) / 64, ey, (ez + 63) / 64);
-    hipLaunchKernelGGL((k_gather<T, true>), grd, blk, 0, s, g, (const T *)g.F, ex, ey, ez, 1.0, dst);
+    hipLaunchKernelGGL((k_gather<T, true>), grd, blk, 0, s, g, (const T *)g.F, ex, ey, ez, scale, dst);
 }
 template <typename T>
 void launch_monitor(const Grid<T> &g, int planar, double *out, hipStream_t s) {
@@ -969,7 +969,7 @@ void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
     template void launch_point_source<T>(const Grid<T> &, const PointSrc &, const ProbeArgs &, int, hipStream_t); \
     template void launch_gather<T>(const Grid<T> &, int, int, int, int, double, double *, hipStream_t);      \
     template void launch_monitor<T>(const Grid<T> &, int, double *, hipStream_t);                            \
-    template void launch_gather_e2<T>(const Grid<T> &, int, int, int, double *, hipStream_t);              \
+    template void launch_gather_e2<T>(const Grid<T> &, int, int, int, double, double *, hipStream_t);              \
     template void launch_tfsf_batch<T>(const B3Sim<T> *, const int2 *, int, int, int, int, hipStream_t);   \
     template void launch_finish_batch<T>(const B3Sim<T> *, int, int, int, hipStream_t);                   \
     template void launch_tfsf_between_batch<T>(const B3Sim<T> *, const int2 *, int, int, int, int, hipStream_t);

@@ -174,6 +174,7 @@ extern "C" lx_status lx_params_default(lx_params *p) {
     p->a_max = 0.05;
     p->a_m = 1.0;
     p->precision = LX_FP32;
+    p->storage_scale = 0.0;
     p->incident = LX_INC_ANALYTIC;
     p->pw_sigma = std::pow(10, -4) / 0.5;   // maxwell.cc:71
     p->pw_wave = LX_WAVE_GAUSSIAN_SINE;
@@ -268,6 +269,7 @@ extern "C" lx_status lx_params_load_yaml(const char *path, lx_params *p) {
         TRY(opt_double(d6, "source delay", &p->src_delay));
         TRY(opt_int(d6, "snapshot every", &p->snapshot_every));
         TRY(opt_int(d6, "monitor every", &p->monitor_every));
+        TRY(opt_double(d6, "storage scale", &p->storage_scale));
     }
     return lx_params_check(p);
 }
@@ -296,6 +298,12 @@ extern "C" lx_status lx_params_check(const lx_params *p) {
     if (!(p->courant > 0)) return lx::fail(LX_EINVAL, "Courant factor must be positive");
     if (p->num_timesteps < 0) return lx::fail(LX_EINVAL, "Number of timesteps must be >= 0");
     if (p->precision != LX_FP32 && p->precision != LX_FP16) return lx::fail(LX_EINVAL, "bad precision");
+    if (p->storage_scale != 0.0) {
+        int ex = 0;
+        if (!(p->storage_scale > 0) || !std::isfinite(p->storage_scale) || std::frexp(p->storage_scale, &ex) != 0.5 ||
+            ex < -60 || ex > 62)
+            return lx::fail(LX_EINVAL, "storage scale must be 0 (automatic) or a power of two in [2^-61, 2^61]");
+    }
     if (p->use_pml) {
         for (int a = 0; a < 3; ++a) {
             if (p->is_planar && a == 1) continue;

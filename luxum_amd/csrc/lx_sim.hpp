@@ -48,6 +48,8 @@ struct lx_sim {
     size_t esize = 4;
     // constants (maxwell.cc:63-84)
     double eps0 = 0, mu0 = 0, c0 = 0, dt = 0, eta0 = 0, sigma_max = 0;
+    // fields are stored times sscale (a power of two): sources inject sscale*amplitude, outputs divide it out
+    double sscale = 1;
     hipStream_t cs = nullptr, ss = nullptr;
     // fused step: the CPML shell launch runs on ks beside the interior launch
     // on cs (both read one buffer set and write disjoint cells of the other)

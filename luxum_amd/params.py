@@ -53,6 +53,7 @@ class LxParams(C.Structure):
         ("src_lambda0", C.c_double), ("src_fwhm", C.c_double), ("src_delay", C.c_double),
         ("snapshot_every", C.c_int32),
         ("monitor_every", C.c_int32),
+        ("storage_scale", C.c_double),
     ]
 
     def copy(self) -> "LxParams":
@@ -122,6 +123,7 @@ def default_params(**over) -> LxParams:
     p.src_delay = 4.0
     p.snapshot_every = 0
     p.monitor_every = 0
+    p.storage_scale = 0.0
     for k, v in over.items():
         if not hasattr(p, k):
             raise AttributeError(k)

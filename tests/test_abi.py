@@ -130,3 +130,15 @@ def test_footprint_estimate():
     bad.nx = 0
     with pytest.raises(LuxumError):
         footprint(bad, True, 10, 0)
+
+
+def test_storage_scale_validation():
+    """storage_scale: 0 (automatic) or a power of two; anything else is
+    rejected before a simulation is created."""
+    from luxum_amd.params import default_params
+    from luxum_amd.sim import params_check
+    for ok in (0.0, 1.0, 0.25, 2.0 ** 20):
+        params_check(default_params(storage_scale=ok))
+    for bad in (3.0, -2.0, float("inf"), 2.0 ** 70):
+        with pytest.raises(Exception):
+            params_check(default_params(storage_scale=bad))

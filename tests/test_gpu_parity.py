@@ -511,3 +511,52 @@ def test_fused_two_cells_per_lane(gpu_ctx, monkeypatch):
     e, pe = rel(a, b, p.components()), rel_l2(a.probes(), b.probes())
     say(f"fp16 vec4 vs vec1 fused (cpml aux, 133 steps): fields {e:.2e} probes {pe:.2e}")
     assert e <= 1e-6 and pe <= 1e-6
+
+
+def test_storage_scale(gpu_ctx):
+    """Fields stored times a power of two give the unscaled run's results
+    (TF/SF, probes, monitor: the same arithmetic, scaled exactly), and the
+    automatic FP16 scale keeps a source of amplitude 1e-6 resolvable, which
+    unscaled binary16 (subnormal below 6.1e-5) is not."""
+    from luxum_amd.sim import Simulation
+
+    def run(p, w, steps):
+        s = Simulation(p, gpu_ctx)
+        s.set_materials(w.ids, w.eps_r)
+        s.set_probes(w.probes)
+        s.step(steps)
+        s.sync()
+        return s
+
+    w = configs.config1(n=48, steps=160, pml=8, margin=6)
+    a = run(w.p, w, 160)
+    w.p.storage_scale = 1024.0
+    b = run(w.p, w, 160)
+    for c in w.p.components():
+        assert rel_l2(b.field(c), a.field(c)) <= 1e-12, c
+    assert rel_l2(b.probes(), a.probes()) <= 1e-12
+    ma, mb = a.monitor(), b.monitor()
+    say(f"scaled FP32 run: monitor {mb} vs {ma}")
+    assert mb[0] == pytest.approx(ma[0], rel=1e-12) and mb[1] == pytest.approx(ma[1], rel=1e-12)
+
+    small = configs.config2(nxy=96, nz=96, steps=40, precision=LX_FP16)
+    small.p.src_pos[:] = (48, 48, 48)
+    small.p.src_delay = 0.5
+    small.p.src_amp = 1e-6
+    small.ids[:] = 0
+    o = OracleSim(small.p)
+    o.set_materials(small.ids, small.eps_r)
+    o.step(40)
+    ref = o.field(LX_EZ)
+    o.close()
+    errs = {}
+    for scale in (0.0, 1.0):        # automatic (2^20 here), none
+        small.p.storage_scale = scale
+        s = Simulation(small.p, gpu_ctx)
+        s.set_materials(small.ids, small.eps_r)
+        s.step(40)
+        s.sync()
+        errs[scale] = rel_l2(s.field(LX_EZ), ref)
+        s.close()
+    say(f"FP16 dipole of amplitude 1e-6 vs FP64 oracle: automatic scale {errs[0.0]:.3e}, unscaled {errs[1.0]:.3e}")
+    assert errs[0.0] <= 5e-2
