@@ -290,10 +290,59 @@ def test_aux_line_propagates_at_grid_speed():
     assert np.abs(e).max() == pytest.approx(1.0, rel=0.05)
 
 
+def _analytic_g(p, x, y, z, tp):
+    """The reference's incident scalar exp(-((k.r) + off*dx - t'*c*dt)^2/sigma)
+    at node coordinates (x, y, z) in cells and time level t' (maxwell.cc:206,
+    289, 604, 712)."""
+    _, _, c0 = consts()
+    dt = p.courant * p.dx / c0
+    pos = x * p.dx * p.k_pw[0] + y * p.dy * p.k_pw[1] + z * p.dz * p.k_pw[2]
+    return np.exp(-(pos + p.off_pw * p.dx - tp * c0 * dt) ** 2 / p.pw_sigma)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_tfsf_analytic_time_level_3d():
+    """Analytic injection along +x (E along z, H_dir = -y) with a resolved
+    pulse in vacuum.  The reference samples E_inc at t-1/2 for the curl-E
+    correction of step t (maxwell.cc:604) and H_inc at t for the curl-H
+    correction (:712), a consistent half-step delay: after n steps the TF
+    region holds Ez^n = g(x, n-1/2) and Hy^{n-1/2} = -g(x+1/2, n-1)/(c mu0).
+    Measured 3.6e-5 and 3.4e-5 (grid dispersion of a 20-cell pulse), while
+    a level off by half a step is 1.2e-2 away.  An E_inc level of t (no
+    delay) leaks 4e-3 of the pulse into the SF region; a correct injection
+    leaks 1.3e-5."""
+    n = 200
+    p = default_params(nx=80, ny=24, nz=24, use_pml=1, pml_thickness=6 * DX, use_pw=1, incident=LX_INC_ANALYTIC,
+                       off_pw=60.0, pw_sigma=(20 * DX) ** 2, tf_lo=(10, 8, 8), tf_hi=(69, 15, 15))
+    p.k_pw[:] = (1.0, 0.0, 0.0)
+    p.e_pw[:] = (0.0, 0.0, 1.0)
+    s = O.OracleSim(p, threads=4)
+    s.step(n)
+    _, mu0, c0 = consts()
+    x = np.arange(12, 68)
+    ez = s.field(LX_EZ)[12:68, 11, 11]
+    hy = s.field(LX_HY)[12:68, 11, 11]
+    assert _rel(ez, _analytic_g(p, x, 0, 0, n - 0.5)) < 2e-4
+    assert _rel(hy, -_analytic_g(p, x + 0.5, 0, 0, n - 1.0) / (c0 * mu0)) < 2e-4
+    for off in (-0.5, +0.5):
+        assert _rel(ez, _analytic_g(p, x, 0, 0, n - 0.5 + off)) > 5e-3
+        assert _rel(hy, -_analytic_g(p, x + 0.5, 0, 0, n - 1.0 + off) / (c0 * mu0)) > 5e-3
+    E = [s.field(c) for c in (LX_EX, LX_EY, LX_EZ)]
+    tfmax = max(np.abs(f[12:68, 9:15, 9:15]).max() for f in E)
+    sfmax = max(np.abs(np.concatenate([f[:8].ravel(), f[72:].ravel()])).max() for f in E)
+    assert tfmax > 0.9
+    assert sfmax < 2e-4 * tfmax
+
+
 def test_tfsf_analytic_oblique_leakage_bound():
     """Reference analytic injection with an oblique k: the SF region sees only
-    the grid-dispersion mismatch of a resolved pulse (a wrong face set or sign
-    leaks O(1))."""
+    the grid-dispersion mismatch of a resolved pulse, measured 1.3e-5 of the
+    TF amplitude.  A wrong face set or sign leaks O(1), E_inc taken at t
+    instead of t-1/2 leaks 4.1e-3 and H_inc on the face instead of half a
+    cell outside it 2.9e-2."""
     p = default_params(nx=44, ny=40, nz=40, use_pml=1, pml_thickness=6 * DX, use_pw=1, incident=LX_INC_ANALYTIC,
                        off_pw=100.0, pw_sigma=(20 * DX) ** 2, tf_lo=(12, 11, 11), tf_hi=(31, 28, 28))
     p.k_pw[:] = (0.5773502691896258,) * 3
@@ -304,7 +353,33 @@ def test_tfsf_analytic_oblique_leakage_bound():
     tfmax = max(np.abs(f[14:29, 13:26, 13:26]).max() for f in E)
     sfmax = max(np.abs(np.concatenate([f[:9].ravel(), f[35:].ravel()])).max() for f in E)
     assert tfmax > 0.3
-    assert sfmax < 0.05 * tfmax
+    assert sfmax < 2e-4 * tfmax
+
+
+def test_tfsf_analytic_2d_oblique():
+    """2D analytic injection with an oblique k in the x-z plane: the TF
+    region holds the negated incident wave (maxwell.cc:206 vs :289),
+    Ey^n = -g(x, z, n-1/2), to 1.4e-4 (measured; half a step off is 1.1e-2
+    away), and the SF region outside the CPML sees 6e-5 of it."""
+    n = 260
+    p = default_params(is_planar=1, nx=90, ny=1, nz=90, dy=0.0, use_pml=1, pml_thickness=8 * DX, use_pw=1,
+                       incident=LX_INC_ANALYTIC, off_pw=80.0, pw_sigma=(20 * DX) ** 2, tf_lo=(14, 0, 14),
+                       tf_hi=(75, 0, 75), src_comp=LX_EY)
+    p.k_pw[:] = (0.6, 0.0, 0.8)
+    p.e_pw[:] = (0.0, 1.0, 0.0)
+    s = O.OracleSim(p, threads=4)
+    s.step(n)
+    ey = s.field(LX_EY)[:, 0, :]
+    i = np.arange(18, 72)[:, None]
+    k = np.arange(18, 72)[None, :]
+    tf = ey[18:72, 18:72]
+    assert _rel(tf, -_analytic_g(p, i, 0, k, n - 0.5)) < 1e-3
+    for off in (-0.5, +0.5):
+        assert _rel(tf, -_analytic_g(p, i, 0, k, n - 0.5 + off)) > 5e-3
+    m = np.zeros_like(ey, bool)
+    m[8:82, 8:82] = True
+    m[12:78, 12:78] = False
+    assert np.abs(ey[m]).max() < 5e-4 * np.abs(tf).max()
 
 
 def test_tfsf_2d_negated_wave():
