@@ -174,6 +174,14 @@ void lx_sim_destroy(lx_sim *sim);
 lx_status lx_sim_set_materials(lx_sim *sim, const uint8_t *ids, const double *eps_r,
                                int32_t n_ids);
 
+/* Replaces the per-component Cb set-up of maxwell_three_dim_pml.cc:109-128,
+ * where Cb->Vx, Cb->Vy and Cb->Vz come from three files and may differ:
+ * ids as above; eps_r3[3*id + c] is the relative permittivity that E
+ * component c (LX_EX, LX_EY, LX_EZ) at storage index (i,j,k) takes when
+ * ids[i,j,k] == id.  Planar runs use the LX_EY column. */
+lx_status lx_sim_set_materials3(lx_sim *sim, const uint8_t *ids, const double *eps_r3,
+                                int32_t n_ids);
+
 /* Probe points (no reference counterpart; the reference only prints max|E|^2):
  * comp_ijk holds n quadruples (component, i, j, k).  Values are recorded after
  * every timestep, FP64, SI units. */
@@ -203,7 +211,9 @@ lx_status lx_sim_read_probes(lx_sim *sim, double *dst, size_t n, int32_t *n_step
 
 /* FP64 max over cells of Ex^2+Ey^2+Ez^2 at equal indices (the reference
  * monitor, maxwell_three_dim_pml.cc:266-276; 2D: max Ey^2,
- * maxwell_two_dim_pml.cc:181) and total discrete field energy. Synchronous. */
+ * maxwell_two_dim_pml.cc:181) and the total discrete field energy
+ * sum(eps0*eps_r*E^2 + mu0*H^2)/2 * cell volume over every stored value
+ * (eps_r of each E component from its material table).  Synchronous. */
 lx_status lx_sim_monitor(lx_sim *sim, double *max_e2, double *energy);
 
 lx_status lx_sim_get_info(lx_sim *sim, lx_sim_info *info);

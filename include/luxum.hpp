@@ -17,9 +17,11 @@
 //   solve3d           Ez_data2.h5    |E|^2, every 10 steps  (maxwell_three_dim.cc:77,192)
 //   solve_planar_pml  Ey_data_2d.h5  Ey,    every 10 steps  (maxwell_two_dim_pml.cc:123,184)
 //   solve_planar      Ey_data_2d.h5  Ey,    every 20 steps from t = 100 (maxwell_two_dim.cc:73,140)
-// solve3d_pml reads Cb from airplane_100_{x,y,z}.h5 (dataset "data",
-// maxwell_three_dim_pml.cc:109-125) when those files exist; the device keeps
-// one material per node, taken from the x file (eps_r = 1 + 1e6*data).
+// solve3d_pml reads the Cb of Ex, Ey and Ez from airplane_100_{x,y,z}.h5
+// (dataset "data", shaped like the component, eps_r = 1 + 1e6*data,
+// maxwell_three_dim_pml.cc:109-128) when the files exist (none: vacuum); each
+// component keeps its own permittivities, so three different files give the
+// reference's three different Cb arrays.
 // Errors throw luxum::error instead of printing and calling exit().
 #ifndef LUXUM_HPP
 #define LUXUM_HPP
@@ -29,6 +31,7 @@
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <array>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -216,7 +219,7 @@ class maxwell {
     // (maxwell_two_dim.cc:128-137); the plain solvers also report the time.
     void solve3d() { run("Ez_data2.h5", LX_OUT_E2, LX_EZ, 10, 0, false, true); }
     void solve3d_pml() {
-        load_cb("airplane_100_x.h5");
+        load_cb();
         run("Ez_data.h5", LX_OUT_E2, LX_EZ, 3, 0, false, false);
     }
     void solve_planar() { run("Ey_data_2d.h5", LX_OUT_FIELD, LX_EY, 20, 100, true, true); }
@@ -259,32 +262,52 @@ class maxwell {
         if (timed && mpi.rank == 0) *out << "Time taken: " << sec << " sec" << std::endl;
     }
 
-    // Cb of the x component from an HDF5 file (eps_r = 1 + 1e6*data, as
-    // maxwell_three_dim_pml.cc:121), one material per node.
-    void load_cb(const char *file) {
-        std::ifstream probe(file);
-        if (!probe.good()) return;   // no file: vacuum
-        fileio rd(file);
-        int64_t d[3];
-        const std::vector<double> data = rd.readHDF5("data", d);
-        const int Nx = inputData.Nx, Ny = gridData.isPlanar ? 1 : inputData.Ny, Nz = inputData.Nz;
+    // Cb of Ex, Ey and Ez from airplane_100_{x,y,z}.h5, dataset "data" of
+    // each component's global shape (maxwell_three_dim_pml.cc:109-128,
+    // fileio.cc:224-247), eps_r = 1 + 1e6*data (:121-123).  Node (i,j,k)
+    // carries one material id for the triple of permittivities that Ex, Ey
+    // and Ez take at that index (1 where a component has no entry there);
+    // at most 256 distinct triples.  No file: vacuum; some but not all three,
+    // or a dataset of the wrong shape: an error (the reference aborts).
+    void load_cb() {
+        static const char *files[3] = {"airplane_100_x.h5", "airplane_100_y.h5", "airplane_100_z.h5"};
+        int present = 0;
+        for (const char *f : files) present += std::ifstream(f).good() ? 1 : 0;
+        if (present == 0) return;
+        const int Nx = inputData.Nx, Ny = inputData.Ny, Nz = inputData.Nz;
+        const int N[3] = {Nx, Ny, Nz};
+        std::vector<double> data[3];
+        for (int c = 0; c < 3; ++c) {
+            if (!std::ifstream(files[c]).good())
+                throw error(LX_EIO, std::string(files[c]) + " not found (the other Cb inputs exist)");
+            int64_t d[3];
+            data[c] = fileio(files[c]).readHDF5("data", d);
+            for (int a = 0; a < 3; ++a)
+                if (d[a] != N[a] - (a == c ? 1 : 0))
+                    throw error(LX_EINVAL, std::string(files[c]) + ": dataset \"data\" does not have the component's shape");
+        }
         std::vector<uint8_t> ids((size_t)Nx * Ny * Nz, 0);
-        std::map<double, int> table;
+        std::map<std::array<double, 3>, int> table;
         std::vector<double> eps;
         for (int i = 0; i < Nx; ++i)
             for (int j = 0; j < Ny; ++j)
                 for (int k = 0; k < Nz; ++k) {
-                    const bool in = i < d[0] && j < d[1] && k < d[2];
-                    const double e = in ? 1.0 + 1e6 * data[((size_t)i * d[1] + j) * d[2] + k] : 1.0;
+                    std::array<double, 3> e{1.0, 1.0, 1.0};
+                    const int ijk[3] = {i, j, k};
+                    for (int c = 0; c < 3; ++c) {
+                        if (ijk[c] == N[c] - 1) continue;   // past the component's staggered extent
+                        const int n1 = N[1] - (c == 1 ? 1 : 0), n2 = N[2] - (c == 2 ? 1 : 0);
+                        e[c] = 1.0 + 1e6 * data[c][((size_t)i * n1 + j) * n2 + k];
+                    }
                     auto it = table.find(e);
                     if (it == table.end()) {
-                        if (eps.size() == 256) throw error(LX_EINVAL, std::string(file) + " holds more than 256 materials");
-                        it = table.emplace(e, (int)eps.size()).first;
-                        eps.push_back(e);
+                        if (eps.size() == 3 * 256) throw error(LX_EINVAL, "airplane_100_{x,y,z}.h5 hold more than 256 materials");
+                        it = table.emplace(e, (int)(eps.size() / 3)).first;
+                        eps.insert(eps.end(), e.begin(), e.end());
                     }
                     ids[((size_t)i * Ny + j) * Nz + k] = (uint8_t)it->second;
                 }
-        check(lx_sim_set_materials(sim_, ids.data(), eps.data(), (int32_t)eps.size()));
+        check(lx_sim_set_materials3(sim_, ids.data(), eps.data(), (int32_t)(eps.size() / 3)));
     }
 
     lx_ctx *ctx_ = nullptr;

@@ -22,7 +22,7 @@ STATUS_NAMES = {0: "LX_OK", 1: "LX_EINVAL", 2: "LX_ENOMEM", 3: "LX_EDEVICE", 4: 
 EXPORTS = (
     "lx_params_default", "lx_params_load_yaml", "lx_params_check",
     "lx_ctx_create", "lx_ctx_destroy", "lx_ctx_mem_info",
-    "lx_sim_create", "lx_sim_destroy", "lx_sim_set_materials", "lx_sim_set_probes",
+    "lx_sim_create", "lx_sim_destroy", "lx_sim_set_materials", "lx_sim_set_materials3", "lx_sim_set_probes",
     "lx_sim_step", "lx_sim_sync", "lx_sim_read_field", "lx_sim_snapshot_async",
     "lx_sim_read_probes", "lx_sim_monitor", "lx_sim_get_info", "lx_sim_stream",
     "lx_sim_time_kernels", "lx_sim_autotune",
@@ -126,6 +126,7 @@ def lib():
         "lx_sim_create": ([vp, P, C.POINTER(vp)], st),
         "lx_sim_destroy": ([vp], None),
         "lx_sim_set_materials": ([vp, vp, vp, C.c_int32], st),
+        "lx_sim_set_materials3": ([vp, vp, vp, C.c_int32], st),
         "lx_sim_set_probes": ([vp, vp, C.c_int32], st),
         "lx_sim_step": ([vp, C.c_int32], st),
         "lx_sim_sync": ([vp], st),

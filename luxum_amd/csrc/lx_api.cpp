@@ -439,7 +439,7 @@ lx_status build_incident(lx_sim *s) {
 
 void fill_cb_vacuum(lx_sim *s) {
     const double v = s->c0 * s->dt / s->p.dx;
-    for (int q = 0; q < 256; ++q) s->cb_host[q] = (float)v;
+    for (int q = 0; q < 3 * 256; ++q) s->cb_host[q] = (float)v;
 }
 
 ProbeArgs probe_args(const lx_sim *s) {
@@ -891,7 +891,7 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
     if ((st = dalloc(s, &q, ncell))) return bail(st);
     s->mat = (uint8_t *)q;
     launch_fill_u8(s->mat, 0, ncell, s->cs);
-    if ((st = dalloc(s, &q, 256 * sizeof(float)))) return bail(st);
+    if ((st = dalloc(s, &q, sizeof(s->cb_host)))) return bail(st);
     s->cb = (float *)q;
     fill_cb_vacuum(s);
     he = hipMemcpy(s->cb, s->cb_host, sizeof(s->cb_host), hipMemcpyHostToDevice);
@@ -969,23 +969,28 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
     return LX_OK;
 }
 
-lx_status lx_sim_set_materials(lx_sim *s, const uint8_t *ids, const double *eps_r, int32_t n_ids) {
-    if (!s || !ids || !eps_r) return fail(LX_EINVAL, "lx_sim_set_materials: null pointer");
-    if (n_ids < 1 || n_ids > 256) return fail(LX_EINVAL, "n_ids must be in [1, 256]");
-    for (int q = 0; q < n_ids; ++q)
-        if (!(eps_r[q] > 0)) return fail(LX_EINVAL, "eps_r must be positive");
+}  // extern "C"
+
+// Material ids plus per-id permittivities of the three E components
+// (eps3[3*id + c]); the isotropic and the per-component entry points share it.
+static lx_status set_materials(lx_sim *s, const uint8_t *ids, const double *eps3, int32_t n_ids, const char *fn) {
+    if (!s || !ids || !eps3) return fail(LX_EINVAL, "%s: null pointer", fn);
+    if (n_ids < 1 || n_ids > 256) return fail(LX_EINVAL, "%s: n_ids must be in [1, 256]", fn);
+    for (int q = 0; q < 3 * n_ids; ++q)
+        if (!(eps3[q] > 0)) return fail(LX_EINVAL, "%s: eps_r must be positive", fn);
     HIP_TRY(hipSetDevice(s->ctx->device));
     member_enter(s);
     const int Nx = s->N[0], Ny = s->N[1], Nz = s->N[2];
     const size_t n = (size_t)Nx * Ny * Nz;
     uint8_t top = 0;
     for (size_t q = 0; q < n; ++q) top = std::max(top, ids[q]);
-    if (top >= n_ids) return fail(LX_EINVAL, "material id %d >= n_ids", (int)top);
-    for (int q = 0; q < 256; ++q) {
-        const double er = q < n_ids ? eps_r[q] : 1.0;
-        const double Cb = s->dt / (s->eps0 * er);             // maxwell_three_dim_pml.cc:122-128
-        s->cb_host[q] = (float)(Cb / (s->eta0 * s->p.dx));
-    }
+    if (top >= n_ids) return fail(LX_EINVAL, "%s: material id %d >= n_ids", fn, (int)top);
+    for (int c = 0; c < 3; ++c)
+        for (int q = 0; q < 256; ++q) {
+            const double er = q < n_ids ? eps3[3 * q + c] : 1.0;
+            const double Cb = s->dt / (s->eps0 * er);             // maxwell_three_dim_pml.cc:122-128
+            s->cb_host[c * 256 + q] = (float)(Cb / (s->eta0 * s->p.dx));
+        }
     // upload in the caller's C order and transpose on the device into the
     // x-fastest layout (the padding columns keep id 0)
     void *raw = nullptr;
@@ -999,6 +1004,19 @@ lx_status lx_sim_set_materials(lx_sim *s, const uint8_t *ids, const double *eps_
     HIP_TRY(hipStreamSynchronize(s->cs));
     dfree(s, raw);
     return st;
+}
+
+extern "C" {
+
+lx_status lx_sim_set_materials(lx_sim *s, const uint8_t *ids, const double *eps_r, int32_t n_ids) {
+    if (!eps_r || n_ids < 1 || n_ids > 256) return set_materials(s, ids, eps_r, n_ids, "lx_sim_set_materials");
+    std::vector<double> eps3(3 * (size_t)n_ids);
+    for (int q = 0; q < n_ids; ++q) eps3[3 * q] = eps3[3 * q + 1] = eps3[3 * q + 2] = eps_r[q];
+    return set_materials(s, ids, eps3.data(), n_ids, "lx_sim_set_materials");
+}
+
+lx_status lx_sim_set_materials3(lx_sim *s, const uint8_t *ids, const double *eps_r3, int32_t n_ids) {
+    return set_materials(s, ids, eps_r3, n_ids, "lx_sim_set_materials3");
 }
 
 lx_status lx_sim_set_probes(lx_sim *s, const int32_t *cijk, int32_t n) {
@@ -1185,8 +1203,9 @@ lx_status lx_sim_monitor(lx_sim *s, double *max_e2, double *energy) {
     HIP_TRY(hipSetDevice(s->ctx->device));
     member_enter(s);
     launch_zero_bytes(s->mon, 32, s->cs);
-    if (s->fp16) launch_monitor(make_grid<__half>(s), s->planar, s->mon, s->cs);
-    else launch_monitor(make_grid<float>(s), s->planar, s->mon, s->cs);
+    const double cb0 = s->c0 * s->dt / s->p.dx;   // vacuum coefficient: eps_r = cb0/cb
+    if (s->fp16) launch_monitor(make_grid<__half>(s), s->planar, cb0, s->mon, s->cs);
+    else launch_monitor(make_grid<float>(s), s->planar, cb0, s->mon, s->cs);
     lx_status st = check_launch();
     if (st) return st;
     double out[4] = {0, 0, 0, 0};

@@ -556,7 +556,12 @@ void lx_batch_destroy(lx_batch *b) {
     for (auto &row : b->desc)
         for (void *p : row)
             if (p) (void)hipFree(p);
-    for (lx_sim *s : b->sims) s->batch = nullptr;
+    // the batch's last step is complete (stream synchronised above): members
+    // no longer wait for it, and its event is about to be destroyed
+    for (lx_sim *s : b->sims) {
+        s->batch = nullptr;
+        s->batch_wait = nullptr;
+    }
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->pre) (void)hipEventDestroy(b->pre);
     if (b->fork) (void)hipEventDestroy(b->fork);
@@ -583,6 +588,7 @@ lx_status lx_batch_remove(lx_batch *b, lx_sim *s) {
     HIP_TRY(hipStreamSynchronize(b->stream));
     b->sims.erase(it);
     s->batch = nullptr;
+    s->batch_wait = nullptr;   // the batch's steps so far are complete; later ones are not the member's
     b->dirty = true;
     return LX_OK;
 }

@@ -21,7 +21,7 @@ struct BSim {
     void *F[2];                       // field buffers (storage type), ping-pong
     float *psi[2];                    // CPML psi, ping-pong
     const uint8_t *mat;
-    const float *cb;
+    const float *cb;                  // [3][256]; planar members use the Ey table
     const float *tab;
     int tn;
     int slo[3], shi[3], sw[3];

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
     const bool colin = i >= 0 && i < nx;
     const bool pml = !INNER && S.pml;
 
-    for (int q = tid; q < 256; q += NT) scb[q] = S.cb[q];
+    for (int q = tid; q < 256; q += NT) scb[q] = S.cb[256 + q];   // the Ey table
     if (tid == 0) snprb = 0;
     if (!INNER) {
         for (int q = tid; q < kRH; q += NT) {

@@ -12,7 +12,11 @@
 //   * H is stored scaled, Hs = eta0*H, and lengths are measured in dx, so every
 //     update coefficient is O(1) (lx_api.cpp has the derivation).
 //   * Materials: one uint8 id per node shared by the E components at that
-//     storage index; cb[id] = c*dt/(dx*eps_r) lives in a 256-entry table.
+//     storage index; each E component c has its own 256-entry table,
+//     cb[c*256 + id] = c*dt/(dx*eps_r,c), so that the three components may
+//     see different permittivities as the reference's per-component Cb
+//     arrays do (maxwell_three_dim_pml.cc:109-128).  Planar runs use the Ey
+//     table (c = 1).
 //   * CPML psi lives only in the boundary slabs: per axis a, the cells with
 //     index < slo[a] or >= shi[a]; four FP32 arrays per axis (psi of the two H
 //     and the two E components that carry a derivative along a).
@@ -111,7 +115,7 @@ struct Grid {
     long long fstride;        // elements between components
     T *F;
     const uint8_t *mat;
-    const float *cb;          // [256]
+    const float *cb;          // [3][256]: Ex, Ey, Ez tables
     const float *tab;
     int tn;                   // table row length
     float *psi;
@@ -253,7 +257,7 @@ void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double scale, do
 // FP64 monitor: out[0] = max |E|^2 (reference index range), out[1] = sum of
 // E^2 + Hs^2, out[2] = non-finite count.
 template <typename T>
-void launch_monitor(const Grid<T> &g, int planar, double *out, hipStream_t s);
+void launch_monitor(const Grid<T> &g, int planar, double cb0, double *out, hipStream_t s);
 
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 // material ids, host C order (i, j, k) -> device layout (k, j, x)

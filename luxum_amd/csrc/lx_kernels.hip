@@ -158,13 +158,15 @@ __global__ __launch_bounds__(256) void k_h3(Grid<T> g) {
 }
 
 // ---------------------------------------------------------------------------
-// 3D E update:  E += cb[id] * ((d1*ke_a1 - d2*ke_a2) + (psi1 - psi2)),
+// 3D E update:  E_c += cb[c][id] * ((d1*ke_a1 - d2*ke_a2) + (psi1 - psi2)),
 // interior only (PEC walls come from the curl ranges, vfield.cc:242-268).
+// Each E component has its own coefficient table (the reference's per-
+// component Cb arrays, maxwell_three_dim_pml.cc:109-128).
 // ---------------------------------------------------------------------------
 template <typename T, bool PML>
 __global__ __launch_bounds__(256) void k_e3(Grid<T> g) {
-    __shared__ float cbs[256];
-    cbs[threadIdx.x] = g.cb[threadIdx.x];
+    __shared__ float cbs[3 * 256];
+    for (int q = threadIdx.x; q < 3 * 256; q += 256) cbs[q] = g.cb[q];
     __syncthreads();
     const int tile = xcd_remap(blockIdx.x, g.ntiles);
     const int tx = tile % g.ntx, rr = tile / g.ntx, ty = rr % g.nty, tz = rr / g.nty;
@@ -222,7 +224,6 @@ __global__ __launch_bounds__(256) void k_e3(Grid<T> g) {
         const float bez = sz ? tab[(2 * 6 + TK_BE) * tn + k] : 0.f, cez = sz ? tab[(2 * 6 + TK_CE) * tn + k] : 0.f;
         if (on) {
             const uchar4 id = *reinterpret_cast<const uchar4 *>(g.mat + o);
-            const float4 cb = make_float4(cbs[id.x], cbs[id.y], cbs[id.z], cbs[id.w]);
             // Ex: d1 = dHz/dy (y, psi y slot0), d2 = dHy/dz (z, psi z slot1)
             if (kin && m_ex) {
                 const float4 hzj = ld4(Hz + o - dj);
@@ -240,6 +241,7 @@ __global__ __launch_bounds__(256) void k_e3(Grid<T> g) {
                     st4(psz1 + zoff, p);
                     acc = acc - p;
                 }
+                const float4 cb = make_float4(cbs[id.x], cbs[id.y], cbs[id.z], cbs[id.w]);
                 const float4 e = ld4(Ex + o);
                 st4(Ex + o, sel(m_ex, e + cb * acc, e));
             }
@@ -261,6 +263,7 @@ __global__ __launch_bounds__(256) void k_e3(Grid<T> g) {
                     st4(psx1 + xoff, p);
                     acc = acc - p;
                 }
+                const float4 cb = make_float4(cbs[256 + id.x], cbs[256 + id.y], cbs[256 + id.z], cbs[256 + id.w]);
                 const float4 e = ld4(Ey + o);
                 st4(Ey + o, sel(m_ey, e + cb * acc, e));
             }
@@ -283,6 +286,7 @@ __global__ __launch_bounds__(256) void k_e3(Grid<T> g) {
                     st4(psy1 + yoff, p);
                     acc = acc - p;
                 }
+                const float4 cb = make_float4(cbs[512 + id.x], cbs[512 + id.y], cbs[512 + id.z], cbs[512 + id.w]);
                 const float4 e = ld4(Ez + o);
                 st4(Ez + o, sel(m_ez, e + cb * acc, e));
             }
@@ -352,7 +356,7 @@ __global__ __launch_bounds__(256) void k_h2(Grid<T> g) {
 template <typename T, bool PML>
 __global__ __launch_bounds__(256) void k_e2(Grid<T> g) {
     __shared__ float cbs[256];
-    cbs[threadIdx.x] = g.cb[threadIdx.x];
+    cbs[threadIdx.x] = g.cb[256 + threadIdx.x];   // the Ey table
     __syncthreads();
     const int tile = xcd_remap(blockIdx.x, g.ntiles);
     const int tx = tile % g.ntx, tz = tile / g.ntx;
@@ -495,7 +499,7 @@ __device__ __forceinline__ void tfsf_block(const Grid<T> &g, const TfBox *__rest
         st1(F + o, ld1(F + o) - upd);
     } else {
         T *F = g.F + c * g.fstride;
-        const float cb = g.cb[g.mat[o]];
+        const float cb = g.cb[c * 256 + g.mat[o]];
         st1(F + o, ld1(F + o) + cb * upd);
     }
 }
@@ -772,8 +776,11 @@ __device__ __forceinline__ void atomic_max_pos(double *addr, double v) {
     atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
 }
 
+// max|E|^2 at equal indices, the discrete energy sum of eps_r*E^2 + (eta0*H)^2
+// (eps_r of each E component from its coefficient table: cb0/cb = eps_r with
+// cb0 = c*dt/dx) and the count of non-finite values.
 template <typename T>
-__global__ __launch_bounds__(256) void k_monitor(Grid<T> g, int planar, double *out) {
+__global__ __launch_bounds__(256) void k_monitor(Grid<T> g, int planar, double cb0, double *out) {
     __shared__ double smax[256];
     __shared__ double ssum[256];
     __shared__ int sbad[256];
@@ -789,7 +796,7 @@ __global__ __launch_bounds__(256) void k_monitor(Grid<T> g, int planar, double *
         if (planar) {
             const double e = ld1(g.F + fs + o);
             mx = e * e;
-            sum = e * e;
+            sum = e * e * (cb0 / (double)g.cb[256 + g.mat[o]]);
             const double hx = ld1(g.F + 3 * fs + o), hz = ld1(g.F + 5 * fs + o);
             sum += hx * hx + hz * hz;
             bad = !isfinite(e) || !isfinite(hx) || !isfinite(hz);
@@ -797,7 +804,9 @@ __global__ __launch_bounds__(256) void k_monitor(Grid<T> g, int planar, double *
             const double ex = ld1(g.F + o), ey = ld1(g.F + fs + o), ez = ld1(g.F + 2 * fs + o);
             const double hx = ld1(g.F + 3 * fs + o), hy = ld1(g.F + 4 * fs + o), hz = ld1(g.F + 5 * fs + o);
             if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) mx = ex * ex + ey * ey + ez * ez;
-            sum = ex * ex + ey * ey + ez * ez + hx * hx + hy * hy + hz * hz;
+            const int id = g.mat[o];
+            sum = ex * ex * (cb0 / (double)g.cb[id]) + ey * ey * (cb0 / (double)g.cb[256 + id]) +
+                  ez * ez * (cb0 / (double)g.cb[512 + id]) + hx * hx + hy * hy + hz * hz;
             bad = !isfinite(ex) || !isfinite(ey) || !isfinite(ez) || !isfinite(hx) || !isfinite(hy) || !isfinite(hz);
         }
     }
@@ -934,9 +943,9 @@ void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double scale, do
     hipLaunchKernelGGL((k_gather<T, true>), grd, blk, 0, s, g, (const T *)g.F, ex, ey, ez, scale, dst);
 }
 template <typename T>
-void launch_monitor(const Grid<T> &g, int planar, double *out, hipStream_t s) {
+void launch_monitor(const Grid<T> &g, int planar, double cb0, double *out, hipStream_t s) {
     dim3 grd((g.nx + 255) / 256, g.ny, g.nz);
-    hipLaunchKernelGGL(k_monitor<T>, grd, dim3(256), 0, s, g, planar, out);
+    hipLaunchKernelGGL(k_monitor<T>, grd, dim3(256), 0, s, g, planar, cb0, out);
 }
 void launch_ids_to_device(const uint8_t *src, uint8_t *dst, int nx, int ny, int nz, int px, long long plane,
                           hipStream_t s) {
@@ -968,7 +977,7 @@ void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
                                       const ProbeArgs &, Launch, hipStream_t);                              \
     template void launch_point_source<T>(const Grid<T> &, const PointSrc &, const ProbeArgs &, int, hipStream_t); \
     template void launch_gather<T>(const Grid<T> &, int, int, int, int, double, double *, hipStream_t);      \
-    template void launch_monitor<T>(const Grid<T> &, int, double *, hipStream_t);                            \
+    template void launch_monitor<T>(const Grid<T> &, int, double, double *, hipStream_t);                            \
     template void launch_gather_e2<T>(const Grid<T> &, int, int, int, double, double *, hipStream_t);              \
     template void launch_tfsf_batch<T>(const B3Sim<T> *, const int2 *, int, int, int, int, hipStream_t);   \
     template void launch_finish_batch<T>(const B3Sim<T> *, int, int, int, hipStream_t);                   \

@@ -99,7 +99,7 @@ struct lx_sim {
     int gK = 0;
     int graphs = 1;
     int steps = 0;
-    float cb_host[256];
+    float cb_host[3 * 256];   // Ex, Ey, Ez coefficient tables
 };
 
 

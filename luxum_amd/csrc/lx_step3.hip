@@ -214,8 +214,19 @@ __device__ __forceinline__ vf<V> curl2(const vf<V> &d1, const vf<V> &k1, bool s1
     return r;
 }
 
-// One unit (x chunk, row, z block) of the step; cbs is the coefficient table
-// of the grid's materials in LDS.  PM: CPML axes the launch handles (bit 0:
+// Coefficient of one E component for the lane's V cells: tab is that
+// component's 256-entry table in LDS (the reference keeps one Cb array per
+// component, maxwell_three_dim_pml.cc:109-128).
+template <int V>
+__device__ __forceinline__ vf<V> cbv(const float *tab, const unsigned (&id)[V]) {
+    vf<V> r;
+#pragma unroll
+    for (int e = 0; e < V; ++e) r.v[e] = tab[id[e]];
+    return r;
+}
+
+// One unit (x chunk, row, z block) of the step; cbs holds the coefficient
+// tables of Ex, Ey and Ez (256 entries each) of the grid's materials in LDS.  PM: CPML axes the launch handles (bit 0:
 // x slab, 1: y, 2: z).  A CPML grid runs two launches: the units whose rows
 // and planes lie outside the y and z slabs (PM = 1, x slab only, far fewer
 // registers) and the rest (PM = 7).
@@ -506,23 +517,20 @@ __device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, 
 
         // ---------------- E half step, row j
         const F hyp = xprev<V>(hy, hyl, lane), hzp = xprev<V>(hz, hzl, lane);
-        F cb;
-#pragma unroll
-        for (int e = 0; e < V; ++e) cb.v[e] = cbs[id[e]];
         F ex = ex0, ey = ey0, ez = ez0;
         if (kine) {
             {                // Ex: d1 = dHz/dy, d2 = dHy/dz
                 const F acc = curl2<PML, V>(hz - hzj, vkey, sy, ey2, vbey, vcey, hy - hym, vkez, sz, ez3, vbez, vcez);
-                ex = vsel<V>(m_ex, ex0 + cb * acc, ex0);
+                ex = vsel<V>(m_ex, ex0 + cbv<V>(cbs, id) * acc, ex0);
             }
             {                // Ey: d1 = dHx/dz, d2 = dHz/dx
                 const F acc = curl2<PML, V>(hx - hxm, vkez, sz, ez2, vbez, vcez, hz - hzp, kex, sx, ex3, bex, cex);
-                ey = vsel<V>(m_ey, ey0 + cb * acc, ey0);
+                ey = vsel<V>(m_ey, ey0 + cbv<V>(cbs + 256, id) * acc, ey0);
             }
         }
         if (kin) {           // Ez: d1 = dHy/dx, d2 = dHx/dy
             const F acc = curl2<PML, V>(hy - hyp, kex, sx, ex2, bex, cex, hx - hxj, vkey, sy, ey3, vbey, vcey);
-            ez = vsel<V>(m_ez, ez0 + cb * acc, ez0);
+            ez = vsel<V>(m_ez, ez0 + cbv<V>(cbs + 512, id) * acc, ez0);
         }
         stv<T, V>(Fnk, rj, ex);
         stv<T, V>(Fnk + fs, rj, ey);
@@ -547,8 +555,8 @@ __device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, 
 
 template <typename T, int PM, int NW, int V>
 __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nunits) {
-    __shared__ float cbs[256];
-    for (int q = threadIdx.x; q < 256; q += 64 * NW) cbs[q] = g.cb[q];
+    __shared__ float cbs[3 * 256];
+    for (int q = threadIdx.x; q < 3 * 256; q += 64 * NW) cbs[q] = g.cb[q];
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int unit = xcd_remap(blockIdx.x, (nunits + NW - 1) / NW) * NW + w;
@@ -565,7 +573,7 @@ template <typename T, int PM, int NW, int V>
 __global__ __launch_bounds__(64 * NW) void k_step3_batch(const B3Sim<T> *__restrict__ sims,
                                                          const int2 *__restrict__ bmap, int nblocks, int part,
                                                          int par) {
-    __shared__ float cbs[256];
+    __shared__ float cbs[3 * 256];
     const int2 mb = bmap[xcd_remap(blockIdx.x, nblocks)];
     const B3Sim<T> *d = sims + mb.x;
     Grid<T> g = d->g;
@@ -577,7 +585,7 @@ __global__ __launch_bounds__(64 * NW) void k_step3_batch(const B3Sim<T> *__restr
         g.psi = g.psin;
         g.psin = q;
     }
-    for (int q = threadIdx.x; q < 256; q += 64 * NW) cbs[q] = g.cb[q];
+    for (int q = threadIdx.x; q < 3 * 256; q += 64 * NW) cbs[q] = g.cb[q];
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int unit = mb.y * NW + w;

@@ -72,12 +72,19 @@ class Simulation:
 
     # set-up --------------------------------------------------------------
     def set_materials(self, ids: np.ndarray, eps_r) -> None:
+        """eps_r: one relative permittivity per id (every E component), or an
+        (n_ids, 3) array of the permittivities that Ex, Ey and Ez take."""
         ids = np.ascontiguousarray(ids, dtype=np.uint8)
         eps = np.ascontiguousarray(eps_r, dtype=np.float64)
         nx, ny, nz = self.p.dims
         if ids.size != nx * ny * nz:
             raise ValueError(f"ids has {ids.size} entries, grid has {nx * ny * nz}")
-        check(lib().lx_sim_set_materials(self._h, ids.ctypes.data, eps.ctypes.data, eps.size))
+        if eps.ndim == 2:
+            if eps.shape[1] != 3:
+                raise ValueError("per-component eps_r must have shape (n_ids, 3)")
+            check(lib().lx_sim_set_materials3(self._h, ids.ctypes.data, eps.ctypes.data, eps.shape[0]))
+        else:
+            check(lib().lx_sim_set_materials(self._h, ids.ctypes.data, eps.ctypes.data, eps.size))
 
     def set_probes(self, probes) -> None:
         arr = np.ascontiguousarray(np.asarray(probes, dtype=np.int32).reshape(-1, 4))

@@ -494,8 +494,10 @@ void lxo_destroy(lxo_sim *s) {
     free(s);
 }
 
-int lxo_set_materials(lxo_sim *s, const uint8_t *ids, const double *eps_r, int n_ids) {
-    int Nx = s->N[0], Ny = s->N[1], Nz = s->N[2];
+/* Per-component Cb = dt/(eps0*eps_r) (maxwell_three_dim_pml.cc:121-128):
+ * component c at array index (i,j,k) takes eps3[3*ids[i,j,k] + c]. */
+int lxo_set_materials3(lxo_sim *s, const uint8_t *ids, const double *eps3, int n_ids) {
+    int Ny = s->N[1], Nz = s->N[2];
     for (int c = 0; c < 3; ++c) {
         fld *f = &s->Cb[c];
         if (!f->v) continue;
@@ -504,11 +506,20 @@ int lxo_set_materials(lxo_sim *s, const uint8_t *ids, const double *eps_r, int n
                 for (int k = 0; k < f->n[2]; ++k) {
                     int id = ids[((size_t)i * Ny + j) * Nz + k];
                     if (id >= n_ids) return -1;
-                    AT(*f, i, j, k) = s->dt / (s->eps0 * eps_r[id]);
+                    AT(*f, i, j, k) = s->dt / (s->eps0 * eps3[3 * id + c]);
                 }
     }
-    (void)Nx;
     return 0;
+}
+
+int lxo_set_materials(lxo_sim *s, const uint8_t *ids, const double *eps_r, int n_ids) {
+    if (n_ids < 1) return -1;
+    double *eps3 = (double *)malloc(sizeof(double) * 3 * (size_t)n_ids);
+    if (!eps3) return -1;
+    for (int q = 0; q < n_ids; ++q) eps3[3 * q] = eps3[3 * q + 1] = eps3[3 * q + 2] = eps_r[q];
+    int rc = lxo_set_materials3(s, ids, eps3, n_ids);
+    free(eps3);
+    return rc;
 }
 
 int lxo_set_probes(lxo_sim *s, const int32_t *cijk, int n) {

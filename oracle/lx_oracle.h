@@ -21,6 +21,8 @@ enum { LXO_SIGMA = 0, LXO_KAPPA = 1, LXO_ALPHA = 2, LXO_B = 3, LXO_C = 4 };
 int lxo_create(const lx_params *p, int threads, lxo_sim **out);
 void lxo_destroy(lxo_sim *s);
 int lxo_set_materials(lxo_sim *s, const uint8_t *ids, const double *eps_r, int n_ids);
+/* Per-component permittivities: E component c takes eps3[3*id + c]. */
+int lxo_set_materials3(lxo_sim *s, const uint8_t *ids, const double *eps3, int n_ids);
 int lxo_set_probes(lxo_sim *s, const int32_t *comp_ijk, int n);
 int lxo_step(lxo_sim *s, int nsteps);
 int lxo_read_field(lxo_sim *s, int comp, double *dst, size_t n);

@@ -36,6 +36,7 @@ def lib():
         L.lxo_create.argtypes = [P, C.c_int, C.POINTER(vp)]
         L.lxo_destroy.argtypes = [vp]
         L.lxo_set_materials.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int]
+        L.lxo_set_materials3.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int]
         L.lxo_set_probes.argtypes = [vp, C.c_void_p, C.c_int]
         L.lxo_step.argtypes = [vp, C.c_int]
         L.lxo_read_field.argtypes = [vp, C.c_int, C.c_void_p, C.c_size_t]
@@ -91,10 +92,16 @@ class OracleSim:
             pass
 
     def set_materials(self, ids: np.ndarray, eps_r):
+        """eps_r: one permittivity per id, or an (n_ids, 3) array of the
+        permittivities that Ex, Ey and Ez take for each id."""
         ids = np.ascontiguousarray(ids, dtype=np.uint8)
         eps = np.ascontiguousarray(eps_r, dtype=np.float64)
         assert ids.size == self.p.dims[0] * self.p.dims[1] * self.p.dims[2]
-        rc = lib().lxo_set_materials(self._h, ids.ctypes.data, eps.ctypes.data, eps.size)
+        if eps.ndim == 2:
+            assert eps.shape[1] == 3
+            rc = lib().lxo_set_materials3(self._h, ids.ctypes.data, eps.ctypes.data, eps.shape[0])
+        else:
+            rc = lib().lxo_set_materials(self._h, ids.ctypes.data, eps.ctypes.data, eps.size)
         if rc != 0:
             raise ValueError("bad material ids")
 

@@ -53,15 +53,20 @@ def test_luxum_app_matches_api(gpu_ctx, tmp_path, name):
     p = load_yaml(path)
     mats = None
     if name == "solve3d_pml":
-        # Cb inputs of maxwell_three_dim_pml.cc:109-125: dataset "data" per
-        # component, eps_r = 1 + 1e6*data; a block of eps_r = 4 in the TF box
+        # Cb inputs of maxwell_three_dim_pml.cc:109-128: dataset "data" of each
+        # component's shape, eps_r = 1 + 1e6*data, three different files: a
+        # block in the TF box with eps_r 4 for Ex, 2 for Ey and 8 for Ez
         from luxum_amd.sim import h5_write
         nx, ny, nz = p.dims
-        data = np.zeros((nx - 1, ny, nz))
-        data[40:60, 45:55, 38:62] = 3e-6
-        for c in "xyz":
-            h5_write(str(tmp_path / f"airplane_100_{c}.h5"), "data", data)
-        mats = data
+        for c, ax in enumerate("xyz"):
+            shape = [nx, ny, nz]
+            shape[c] -= 1
+            data = np.zeros(shape)
+            data[40:60, 45:55, 38:62] = (3e-6, 1e-6, 7e-6)[c]
+            h5_write(str(tmp_path / f"airplane_100_{ax}.h5"), "data", data)
+        ids = np.zeros((nx, ny, nz), np.uint8)
+        ids[40:60, 45:55, 38:62] = 1
+        mats = (ids, np.array([[1.0, 1.0, 1.0], [4.0, 2.0, 8.0]]))
     r = subprocess.run([APP], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     say(r.stdout[-300:] + r.stderr[-300:])
     assert r.returncode == 0, r.stderr
@@ -72,10 +77,7 @@ def test_luxum_app_matches_api(gpu_ctx, tmp_path, name):
 
     s = Simulation(p, gpu_ctx)
     if mats is not None:
-        nx, ny, nz = p.dims
-        ids = np.zeros((nx, ny, nz), np.uint8)
-        ids[: nx - 1] = (mats != 0)
-        s.set_materials(ids, [1.0, 1.0 + mats.max() * 1e6])
+        s.set_materials(*mats)
     planar = p.is_planar
     steps_done = 0
     mx_after = [0.0]                      # max after k steps
@@ -99,3 +101,63 @@ def test_luxum_app_matches_api(gpu_ctx, tmp_path, name):
     with pytest.raises(Exception):
         h5_read(str(tmp_path / fname), h5_dataset_name(max(expect) + every))
     assert max(mx_after) > 0
+
+
+def test_luxum_app_three_material_files_vs_oracle(gpu_ctx, tmp_path):
+    """solve3d_pml with three different airplane_100_{x,y,z}.h5 files: the
+    last |E|^2 dataset the app writes against the FP64 oracle run with the
+    same per-component permittivities (|E|^2 relative L2 <= 2e-5, twice the
+    field tolerance)."""
+    from luxum_amd.sim import h5_dataset_name, h5_read, h5_write, load_yaml
+    from oracle.lxoracle import OracleSim
+    path = reference_yaml("parameters_3d.yaml", tmp_path / "parameters.yaml", **{"Number of timesteps": 31})
+    p = load_yaml(path)
+    nx, ny, nz = p.dims
+    say("luxum app: three different Cb files vs oracle, 31 steps")
+    for c, ax in enumerate("xyz"):
+        shape = [nx, ny, nz]
+        shape[c] -= 1
+        data = np.zeros(shape)
+        data[30:70, 35:65, 28:72] = (3e-6, 1e-6, 7e-6)[c]
+        h5_write(str(tmp_path / f"airplane_100_{ax}.h5"), "data", data)
+    r = subprocess.run([APP], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = h5_read(str(tmp_path / "Ez_data.h5"), h5_dataset_name(30))      # state after 31 steps
+    ids = np.zeros((nx, ny, nz), np.uint8)
+    ids[30:70, 35:65, 28:72] = 1
+    o = OracleSim(p)
+    o.set_materials(ids, np.array([[1.0, 1.0, 1.0], [4.0, 2.0, 8.0]]))
+    o.step(31)
+
+    class _O:
+        def field(self, c):
+            return o.field(c)
+    want = e2(_O(), p)
+    e = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+    iso = OracleSim(p)
+    iso.set_materials(ids, [1.0, 4.0])
+    iso.step(31)
+
+    class _I:
+        def field(self, c):
+            return iso.field(c)
+    d = float(np.linalg.norm(e2(_I(), p) - want) / np.linalg.norm(want))
+    say(f"app |E|^2 vs oracle {e:.2e}; isotropic eps 4 vs per-component {d:.2e}")
+    assert e <= 2e-5
+    assert d > 1e-3
+
+
+def test_luxum_app_partial_material_files_fail(tmp_path):
+    """Two of the three Cb files: the app reports an error instead of
+    silently using vacuum for the missing component (the reference aborts
+    when it cannot read its input)."""
+    from luxum_amd.sim import h5_write, load_yaml
+    path = reference_yaml("parameters_3d.yaml", tmp_path / "parameters.yaml", **{"Number of timesteps": 3})
+    p = load_yaml(path)
+    nx, ny, nz = p.dims
+    h5_write(str(tmp_path / "airplane_100_x.h5"), "data", np.zeros((nx - 1, ny, nz)))
+    h5_write(str(tmp_path / "airplane_100_y.h5"), "data", np.zeros((nx, ny - 1, nz)))
+    r = subprocess.run([APP], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    say(r.stdout[-200:] + r.stderr[-200:])
+    assert r.returncode != 0
+    assert "airplane_100_z.h5" in r.stderr

@@ -272,3 +272,77 @@ def test_batch_3d_fp16_four_cells(gpu_ctx):
         say(f"{w.name}: fields {e:.2e} probes {pe:.2e}")
         assert e <= 1e-6 and pe <= 1e-6
     b.close()
+
+
+@pytest.mark.parametrize("how", ["remove", "close"])
+def test_batch_member_after_unsynced_remove_or_close(gpu_ctx, how):
+    """A member leaves the batch right after a batch step, without a sync:
+    it must not wait on the batch's (destroyed, or later unrelated) step
+    event, and its own next steps continue from the batch's state."""
+    from luxum_amd.sim import Batch
+    ws = [configs.box3d(64, 40, 36, steps=60), configs.box3d(56, 48, 40, steps=60)]
+    members = [make(w, gpu_ctx) for w in ws]
+    b = Batch(gpu_ctx)
+    for s in members:
+        b.add(s)
+    say(f"batch of 2 3D members: step 20, {how} member 0 without sync, member 0 steps 7 alone")
+    b.step(20)
+    if how == "remove":
+        b.remove(members[0])
+        b.step(9)                     # the batch goes on with member 1
+    else:
+        b.close()
+    members[0].step(7)
+    members[0].sync()
+    ref = make(ws[0], gpu_ctx)
+    ref.step(27)
+    ref.sync()
+    e = field_err(members[0], ref)
+    say(f"member 0 after {how}: vs single {e:.2e}")
+    assert e <= 1e-6
+    if how == "remove":
+        b.sync()
+        ref1 = make(ws[1], gpu_ctx)
+        ref1.step(29)
+        ref1.sync()
+        assert field_err(members[1], ref1) <= 1e-6
+        b.close()
+
+
+def test_batch_members_with_storage_scale(gpu_ctx):
+    """Members stored times a power of two in the tiled 2D kernel and the
+    batched fused 3D step give the results of the same batch unscaled (FP32:
+    the scale is exact; the tiled kernel's own TF/SF and probe records divide
+    it out), and an FP16 3D member with a scale equals its single-simulation
+    run with the same scale."""
+    from luxum_amd.sim import Batch
+
+    def run_batch(ws, steps):
+        ms = [make(w, gpu_ctx) for w in ws]
+        b = Batch(gpu_ctx)
+        for m in ms:
+            b.add(m)
+        b.step(steps)
+        b.sync()
+        return b, ms
+
+    w2, w3 = phc(160, 3), configs.box3d(64, 40, 36, steps=120)
+    say("batch unscaled, then scaled 1024 (2D tiled, 3D FP32) with a scaled FP16 3D member: 120 steps")
+    b0, plain = run_batch([w2, w3], 120)
+    for w in (w2, w3):
+        w.p.storage_scale = 1024.0
+    w16 = configs.box3d(72, 40, 36, precision=LX_FP16, steps=120)
+    w16.p.storage_scale = 2.0 ** 12
+    b1, scaled = run_batch([w2, w3, w16], 120)
+    for w, s, r in zip((w2, w3), scaled, plain):
+        e, pe = field_err(s, r), rel_l2(s.probes(), r.probes())
+        say(f"{w.name}: scaled vs unscaled batch fields {e:.2e} probes {pe:.2e}")
+        assert e <= 1e-12 and pe <= 1e-12
+    r16 = make(w16, gpu_ctx)
+    r16.step(120)
+    r16.sync()
+    e, pe = field_err(scaled[2], r16), rel_l2(scaled[2].probes(), r16.probes())
+    say(f"{w16.name}: scaled FP16 batch vs scaled FP16 single fields {e:.2e} probes {pe:.2e}")
+    assert e <= 1e-6 and pe <= 1e-6
+    b0.close()
+    b1.close()

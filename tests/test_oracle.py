@@ -131,6 +131,49 @@ def test_hand_computed_single_step_3d():
     np.testing.assert_allclose(s.field(LX_EZ), ez, rtol=1e-12, atol=1e-12)
 
 
+def test_per_component_cb_single_step():
+    """Per-component materials (maxwell_three_dim_pml.cc:109-128: Cb->Vx,
+    Cb->Vy, Cb->Vz from three files): after one plain step from a known H
+    field, E_c changed by Cb_c(i,j,k)*curl_c H, where Cb_c takes column c of
+    the permittivity triple of the node's id, and the isotropic entry point
+    equals the per-component one with three equal columns bit for bit."""
+    p = default_params(nx=5, ny=6, nz=7)
+    eps0, mu0, c0 = consts()
+    dt = 0.5 * DX / c0
+    rng = np.random.default_rng(3)
+    ids = rng.integers(0, 3, size=(5, 6, 7)).astype(np.uint8)
+    eps3 = np.array([[1.0, 2.0, 3.0], [4.0, 5.0, 6.0], [7.5, 1.5, 2.5]])
+    H0 = [rng.standard_normal(p.field_shape(c)) for c in (LX_HX, LX_HY, LX_HZ)]
+    s = O.OracleSim(p, threads=1)
+    s.set_materials(ids, eps3)
+    for c in range(3):
+        s.set_field(3 + c, H0[c])
+    s.step(1)
+    Hx, Hy, Hz = [s.field(3 + c) for c in range(3)]     # H after the H half step (E was zero: unchanged)
+    for c in range(3):
+        np.testing.assert_array_equal(s.field(3 + c), H0[c])
+    ex = np.zeros(p.field_shape(LX_EX))
+    ey = np.zeros(p.field_shape(LX_EY))
+    ez = np.zeros(p.field_shape(LX_EZ))
+    ex[:, 1:-1, 1:-1] = (Hz[:, 1:, 1:-1] - Hz[:, :-1, 1:-1]) / DX - (Hy[:, 1:-1, 1:] - Hy[:, 1:-1, :-1]) / DX
+    ey[1:-1, :, 1:-1] = (Hx[1:-1, :, 1:] - Hx[1:-1, :, :-1]) / DX - (Hz[1:, :, 1:-1] - Hz[:-1, :, 1:-1]) / DX
+    ez[1:-1, 1:-1, :] = (Hy[1:, 1:-1, :] - Hy[:-1, 1:-1, :]) / DX - (Hx[1:-1, 1:, :] - Hx[1:-1, :-1, :]) / DX
+    for c, curl in enumerate((ex, ey, ez)):
+        sh = curl.shape
+        cb = dt / (eps0 * eps3[ids[: sh[0], : sh[1], : sh[2]], c])
+        np.testing.assert_allclose(s.field(c), cb * curl, rtol=1e-12, atol=1e-20)
+    a = O.OracleSim(p, threads=1)
+    b = O.OracleSim(p, threads=1)
+    a.set_materials(ids, eps3[:, 1])
+    b.set_materials(ids, np.repeat(eps3[:, 1:2], 3, axis=1))
+    for o in (a, b):
+        for c in range(3):
+            o.set_field(3 + c, H0[c])
+        o.step(3)
+    for c in range(6):
+        np.testing.assert_array_equal(a.field(c), b.field(c))
+
+
 @pytest.mark.parametrize("pml", [0, 1])
 def test_cavity_eigenmode_follows_yee_dispersion(pml):
     """TM110 mode of the PEC box (PML flag on but zero thickness effect is not
