@@ -48,25 +48,124 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(args):
-    """FP64 oracle (reference loop structure, OpenMP) on a bounded sample:
-    the full 256^3 grid of the same workload for a few timesteps."""
-    from luxum_amd import configs
+def _usable_cores():
+    """Host cores this process may run on: the affinity mask, capped by the
+    cgroup's CPU quota (a container's CPU share), and the machine's count."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, -(-int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return n, os.cpu_count() or n
+
+
+def _cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_run(w, steps, threads, monitor=False, snap_every=0, snap_comp=None, e2=False, tmpdir=None):
+    """Time `steps` oracle timesteps after one untimed step.  monitor: the
+    reference's per-step max|E|^2 (serial, as blitz::max); snap_every: an
+    HDF5 snapshot every that many steps in the reference's layout (|E|^2 of
+    the 3D solver or the field of the 2D one), written like fileio."""
+    import numpy as np
+    from luxum_amd.params import LX_EX, LX_EY, LX_EZ
+    from luxum_amd.sim import h5_write
     from oracle.lxoracle import OracleSim
-    threads = min(os.cpu_count() or 1, 16)
-    w = configs.config1(n=args.n, steps=args.cpu_steps)
     o = OracleSim(w.p, threads=threads)
     o.set_materials(w.ids, w.eps_r)
     o.set_probes(w.probes)
     o.step(1)
+    path = os.path.join(tmpdir, "snap.h5") if snap_every else None
+    nsnap = 0
     t0 = time.perf_counter()
-    o.step(args.cpu_steps)
+    for t in range(steps):
+        o.step(1)
+        if monitor:
+            o.max_e2()
+        if snap_every and t % snap_every == 0:
+            if e2:
+                ex, ey, ez = o.field(LX_EX), o.field(LX_EY), o.field(LX_EZ)
+                nx, ny, nz = w.p.dims
+                out = np.zeros((nx, ny, nz - 1))
+                c = (slice(0, nx - 1), slice(0, ny - 1), slice(0, nz - 1))
+                out[c] = ez[c] ** 2 + ey[c] ** 2 + ex[c] ** 2
+            else:
+                out = o.field(snap_comp)
+            h5_write(path, t, out, append=nsnap > 0)
+            nsnap += 1
     dt = time.perf_counter() - t0
     o.close()
-    gcells = w.cells * args.cpu_steps / dt / 1e9
-    return {"value": round(gcells, 5), "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 reference-structured oracle, {args.n}^3 grid of the same workload, "
-                      f"{args.cpu_steps} timesteps ({dt:.1f} s), {threads} OpenMP threads"}
+    if path and os.path.exists(path):
+        os.remove(path)
+    return {"steps": steps, "seconds": round(dt, 3), "gcell_per_s": round(w.cells * steps / dt / 1e9, 5),
+            "snapshots": nsnap}
+
+
+def cpu_child(args):
+    """Runs in a child process whose OpenMP settings the parent chose: the
+    FP64 reference-structured oracle on configuration 1 (the main line's
+    workload) and configuration 0 (512^2 2D CPML, the designated CPU case),
+    compute-only and as the reference times its loop (per-step monitor and
+    snapshots at the reference's cadences).  Prints one JSON object."""
+    import tempfile
+    from luxum_amd import configs
+    from luxum_amd.params import LX_EY
+    th = args.cpu_child
+    one = th == 1
+    w1 = configs.config1(n=args.n, steps=4000)
+    w0 = configs.config0(n=512, steps=4000)
+    res = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        res["config1"] = _oracle_run(w1, args.cpu_steps_1t if one else args.cpu_steps, th)
+        res["config0"] = _oracle_run(w0, 1000, th)
+        if not one:
+            # maxwell_three_dim_pml.cc:266-281: |E|^2 + max every step, |E|^2 snapshot every 3 steps
+            res["config1_reference_timed"] = _oracle_run(w1, args.cpu_ref_steps, th, monitor=True, snap_every=3,
+                                                         e2=True, tmpdir=tmp)
+            # maxwell_two_dim_pml.cc:179-185: max Ey^2 every step, Ey snapshot every 10 steps
+            res["config0_reference_timed"] = _oracle_run(w0, 1000, th, monitor=True, snap_every=10,
+                                                         snap_comp=LX_EY, tmpdir=tmp)
+    print(json.dumps(res), flush=True)
+
+
+def cpu_baseline(args):
+    """The FP64 reference-structured oracle on the host cores, in child
+    processes with OMP_PROC_BIND=close: all usable cores, then one thread."""
+    import subprocess
+    cores, nproc = _usable_cores()
+    out = {}
+    for th in (cores, 1):
+        env = dict(os.environ, OMP_NUM_THREADS=str(th), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", str(th), "--n", str(args.n),
+               "--cpu-steps", str(args.cpu_steps), "--cpu-steps-1t", str(args.cpu_steps_1t),
+               "--cpu-ref-steps", str(args.cpu_ref_steps)]
+        print(f"bench: CPU baseline, {th} thread(s)", file=sys.stderr, flush=True)
+        r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError(f"CPU baseline child failed ({r.returncode}): {r.stderr[-500:]}")
+        out[th] = json.loads(r.stdout.strip().splitlines()[-1])
+    a, b = out[cores], out[1]
+    c1 = a["config1"]
+    return {"value": c1["gcell_per_s"], "unit": "Gcell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 reference-structured oracle (separate curl and update passes, full-domain psi, per-cell "
+                      f"Cb, serial TF/SF, OpenMP schedule(dynamic)), {args.n}^3 grid of the main line's workload, "
+                      f"{c1['steps']} timesteps ({c1['seconds']} s), compute only, {cores} threads, "
+                      f"OMP_PROC_BIND=close",
+            "cpu_model": _cpu_model(), "nproc": nproc, "threads": cores,
+            "one_thread": {"config1": b["config1"], "config0": b["config0"]},
+            "config0": {"workload": "2D TM 512^2, CPML 16, TF/SF aux-1D plane wave, eps_r 10.89 square, 4 probes, "
+                                    "1000 steps", "compute_only": a["config0"],
+                        "reference_timed": a["config0_reference_timed"]},
+            "config1_reference_timed": a["config1_reference_timed"]}
 
 
 def batch_section(args):
@@ -417,7 +516,11 @@ def main():
                          "sections: its FP64 CPU state would not fit in host memory)")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--fp16", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=150)
+    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps-1t", type=int, default=5, help="config-1 steps of the one-thread CPU run")
+    ap.add_argument("--cpu-ref-steps", type=int, default=30,
+                    help="config-1 steps of the CPU run with the reference's monitor and |E|^2 output")
+    ap.add_argument("--cpu-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-steps", type=int, default=50)
     ap.add_argument("--batch-sims", type=int, default=256)
@@ -441,6 +544,9 @@ def main():
     ap.add_argument("--traffic-copy-mib", type=int, default=512)
     ap.add_argument("--traffic-timeout", type=float, default=240.0)
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_child(args)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the whole domain is resident on a single MI355X)", file=sys.stderr)

@@ -53,9 +53,20 @@
 /* storage                                                                 */
 /* ---------------------------------------------------------------------- */
 
+/* Storage type of fields, split curls and psi.  The checker is built with
+ * double; a second build (-DLXO_STORAGE_FLOAT, liblxoracle_f32s.so) rounds
+ * every stored value to float while keeping FP64 arithmetic, the precision
+ * floor of any FP32-storage implementation of this algorithm, which tests
+ * use to tell storage rounding from kernel error. */
+#ifdef LXO_STORAGE_FLOAT
+typedef float store_t;
+#else
+typedef double store_t;
+#endif
+
 typedef struct {
     int n[3];
-    double *v;
+    store_t *v;
 } fld;
 
 #define AT(f, i, j, k) ((f).v[((size_t)(i) * (size_t)(f).n[1] + (size_t)(j)) * (size_t)(f).n[2] + (size_t)(k)])
@@ -110,7 +121,7 @@ int lxo_field_size(const lx_params *p, int comp, int ext[3]) {
 static int falloc(fld *f, const int ext[3]) {
     size_t n = (size_t)ext[0] * ext[1] * ext[2];
     f->n[0] = ext[0]; f->n[1] = ext[1]; f->n[2] = ext[2];
-    f->v = (double *)calloc(n ? n : 1, sizeof(double));
+    f->v = (store_t *)calloc(n ? n : 1, sizeof(store_t));
     return f->v ? 0 : -1;
 }
 
@@ -345,7 +356,7 @@ static void pw_apply(lxo_sim *s, fld *c1, fld *c2, int h_update, int t) {
                             /* 2D E terms carry an explicit (-1) (maxwell.cc:563-588) */
                             val = g * (-1) * dir / s->d[a] / s->c / s->mu0;
                         }
-                        double *q = &AT(*arr, ijk[0], ijk[1], ijk[2]);
+                        store_t *q = &AT(*arr, ijk[0], ijk[1], ijk[2]);
                         if (sgn < 0) *q = *q - val;
                         else *q = *q + val;
                     }
@@ -934,7 +945,7 @@ int lxo_read_field(lxo_sim *s, int comp, double *dst, size_t n) {
     if (!f->v) return -1;
     size_t m = (size_t)f->n[0] * f->n[1] * f->n[2];
     if (n != m) return -1;
-    memcpy(dst, f->v, sizeof(double) * m);
+    for (size_t q = 0; q < m; ++q) dst[q] = f->v[q];
     return 0;
 }
 
@@ -944,7 +955,7 @@ int lxo_set_field(lxo_sim *s, int comp, const double *src, size_t n) {
     if (!f->v) return -1;
     size_t m = (size_t)f->n[0] * f->n[1] * f->n[2];
     if (n != m) return -1;
-    memcpy(f->v, src, sizeof(double) * m);
+    for (size_t q = 0; q < m; ++q) f->v[q] = (store_t)src[q];
     return 0;
 }
 

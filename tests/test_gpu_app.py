@@ -110,24 +110,24 @@ def test_luxum_app_three_material_files_vs_oracle(gpu_ctx, tmp_path):
     field tolerance)."""
     from luxum_amd.sim import h5_dataset_name, h5_read, h5_write, load_yaml
     from oracle.lxoracle import OracleSim
-    path = reference_yaml("parameters_3d.yaml", tmp_path / "parameters.yaml", **{"Number of timesteps": 31})
+    path = reference_yaml("parameters_3d.yaml", tmp_path / "parameters.yaml", **{"Number of timesteps": 100})
     p = load_yaml(path)
     nx, ny, nz = p.dims
-    say("luxum app: three different Cb files vs oracle, 31 steps")
+    say("luxum app: three different Cb files vs oracle, 100 steps")
     for c, ax in enumerate("xyz"):
         shape = [nx, ny, nz]
         shape[c] -= 1
         data = np.zeros(shape)
         data[30:70, 35:65, 28:72] = (3e-6, 1e-6, 7e-6)[c]
         h5_write(str(tmp_path / f"airplane_100_{ax}.h5"), "data", data)
-    r = subprocess.run([APP], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    r = subprocess.run([APP, "--print-every", "50"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    got = h5_read(str(tmp_path / "Ez_data.h5"), h5_dataset_name(30))      # state after 31 steps
+    got = h5_read(str(tmp_path / "Ez_data.h5"), h5_dataset_name(99))      # state after 100 steps
     ids = np.zeros((nx, ny, nz), np.uint8)
     ids[30:70, 35:65, 28:72] = 1
     o = OracleSim(p)
     o.set_materials(ids, np.array([[1.0, 1.0, 1.0], [4.0, 2.0, 8.0]]))
-    o.step(31)
+    o.step(100)
 
     class _O:
         def field(self, c):
@@ -136,7 +136,7 @@ def test_luxum_app_three_material_files_vs_oracle(gpu_ctx, tmp_path):
     e = float(np.linalg.norm(got - want) / np.linalg.norm(want))
     iso = OracleSim(p)
     iso.set_materials(ids, [1.0, 4.0])
-    iso.step(31)
+    iso.step(100)
 
     class _I:
         def field(self, c):

@@ -522,3 +522,24 @@ def test_slab_workload_mirror_parities():
     assert mirror_asymmetry(o2.field(2), 2, +1) > 1e-3
     o.close()
     o2.close()
+
+
+def test_fp32_storage_build_is_the_same_algorithm():
+    """The FP32-storage build of the oracle (every stored value rounded to
+    float, FP64 arithmetic) follows the FP64 checker to FP32 rounding on the
+    cube workload, and is not the FP64 build (its fields differ)."""
+    from luxum_amd import configs
+    w = configs.config1(n=32, steps=80, pml=6, margin=4)
+    runs = []
+    for storage in ("fp64", "fp32"):
+        o = O.OracleSim(w.p, threads=4, storage=storage)
+        o.set_materials(w.ids, w.eps_r)
+        o.step(80)
+        runs.append([o.field(c) for c in range(6)])
+        o.close()
+    num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(*runs))
+    den = sum(float(np.sum(a ** 2)) for a in runs[0])
+    err = (num / den) ** 0.5
+    assert 1e-9 < err < 1e-6
+    for a, b in zip(*runs):
+        assert np.all(b == b.astype(np.float32))     # every stored value is a float
