@@ -16,7 +16,13 @@ GPU runs directly.
   of the same dipole's field on a 160^3 vacuum grid, which the oracle
   checks; everywhere else the 1.05e9-node grid must stay exactly zero.
 
-Both run on the configuration the autotuner picks, as the bench does.
+- Configuration 2 at its full size and length (4000 steps) with the dipole
+  on the slab's mid-plane: exact z-mirror parity of every component after
+  the field has crossed the rods, the slab and every CPML face.
+- Configuration 0 at its stated size (512^2, 1000 steps) directly against
+  the FP64 oracle, alone and as a batch member.
+
+All run on the configuration the autotuner picks, as the bench does.
 """
 import sys
 
@@ -121,3 +127,97 @@ def test_dipole_translation_phc_full_size(gpu_ctx, prec):
         assert rel_l2(f[win], ref[c][sw]) <= 1e-6      # the same arithmetic: equal in practice
         del f
     sb.close()
+
+
+def test_config0_full_size_vs_oracle(gpu_ctx):
+    """Configuration 0 at its stated size: 2D TM 512^2, CPML 16, TF/SF plane
+    wave, the n = 3.3 square, 1000 steps; Ey snapshots every 100 steps and
+    the 4 probes against the FP64 oracle (relative L2 <= 1e-5), through the
+    single-simulation kernels and as a member of a batch (the temporally
+    tiled kernel)."""
+    from luxum_amd.sim import Batch, Simulation
+    from luxum_amd.params import LX_EY, LX_HX, LX_HZ
+    w = configs.config0()
+    say("case config 0: 512^2 x 1000 steps, single and batched, vs FP64 oracle")
+    single = Simulation(w.p, gpu_ctx)
+    member = Simulation(w.p, gpu_ctx)
+    other = Simulation(configs.phc2d(384, 5, steps=1000).p, gpu_ctx)     # a second, different member
+    for s in (single, member):
+        s.set_materials(w.ids, w.eps_r)
+        s.set_probes(w.probes)
+    b = Batch(gpu_ctx)
+    b.add(member)
+    b.add(other)
+    o = OracleSim(w.p)
+    o.set_materials(w.ids, w.eps_r)
+    o.set_probes(w.probes)
+    worst = {"single": 0.0, "batched": 0.0}
+    for q in range(10):
+        single.step(100)
+        b.step(100)
+        o.step(100)
+        single.sync()
+        b.sync()
+        ref = o.field(LX_EY)
+        for name, s in (("single", single), ("batched", member)):
+            e = rel_l2(s.field(LX_EY), ref)
+            worst[name] = max(worst[name], e)
+        say(f"  t={100 * (q + 1)}: Ey rel L2 single {rel_l2(single.field(LX_EY), ref):.2e} "
+            f"batched {rel_l2(member.field(LX_EY), ref):.2e}")
+    for name, s in (("single", single), ("batched", member)):
+        h = max(rel_l2(s.field(c), o.field(c)) for c in (LX_HX, LX_HZ))
+        pe = rel_l2(s.probes(), o.probes())
+        say(f"  {name}: worst Ey snapshot {worst[name]:.2e}, final H {h:.2e}, probes {pe:.2e}")
+        assert worst[name] <= 1e-5 and h <= 1e-5 and pe <= 1e-5
+    b.close()
+    for s in (single, member, other):
+        s.close()
+
+
+@pytest.mark.parametrize("prec", [LX_FP32, LX_FP16], ids=["fp32", "fp16"])
+def test_phc_slab_full_size_z_mirror(gpu_ctx, prec):
+    """Configuration 2 at its full size and length: 1280 x 1280 x 640, rods
+    eps 12 in the 64-cell slab, the removed row, CPML 16, 4000 steps, with
+    the soft Ez dipole moved half a cell, onto the slab's mid-plane (Ez index
+    319 of 639, position 319.5, the grid's centre plane).  The grid, CPML,
+    slab and source are then symmetric under z -> 639 - z (nodes) /
+    638 - m (staggered), so every component equals +/- its mirror image with
+    the parities of a z-directed dipole on the plane (Ex, Ey, Hz odd; Ez,
+    Hx, Hy even), exactly, while the field has crossed the rods, the slab
+    and the CPML on every face.  A wrong index, coefficient or sign on one
+    side of a slab, a z block, a launch boundary or the z CPML breaks it."""
+    from luxum_amd.sim import Simulation
+    steps = 4000
+    w = configs.config2(steps=steps, precision=prec)
+    w.p.src_pos[2] = 319
+    w.probes[:, 3] = 319
+    say(f"case PhC slab full size z-mirror prec={prec} steps={steps}")
+    s = Simulation(w.p, gpu_ctx)
+    s.set_materials(w.ids, w.eps_r)
+    del w
+    kz = s.autotune()
+    s.step(steps)
+    s.sync()
+    inf = s.info()
+    say(f"  autotuned kz={kz} fused={inf.fused_step} cells_per_lane={inf.cells_per_lane}")
+    mx, en = s.monitor()
+    say(f"  monitor after {steps} steps: max|E|^2 {mx:.3e} energy {en:.3e}")
+    worst = 0.0
+    for c in range(6):
+        f = s.field(c)
+        assert np.all(np.isfinite(f))
+        pz = MIRROR_PARITY[c][1]
+        nrm = float(np.linalg.norm(f))
+        assert nrm > 0, c
+        # the field reached the CPML of every face
+        for a in range(3):
+            edge = np.take(f, range(0, 8), axis=a)
+            assert np.abs(edge).max() > 1e-12 * np.abs(f).max(), (c, a)
+        asym = float(np.linalg.norm(f - pz * f[:, :, ::-1])) / nrm
+        flip = float(np.linalg.norm(f + pz * f[:, :, ::-1])) / nrm
+        say(f"  comp {c}: asymmetry {asym:.3e} (flipped parity {flip:.3f})")
+        assert flip > 1.9
+        worst = max(worst, asym)
+        del f
+    s.close()
+    assert worst == 0.0, worst
