@@ -231,6 +231,12 @@ def batch_section(args):
             "speedup": round(t_single / t_batch, 2),
             "rel_l2_vs_one_at_a_time": (num / den) ** 0.5 if den > 0 else None,
             "l2_sample_sims": len(sample)})
+    if not args.no_traffic:
+        try:
+            out["traffic"] = batch_traffic(args, cells, out["gcell_per_s"])
+            print(f"bench: batched 2D HBM traffic {out['traffic']}", file=sys.stderr, flush=True)
+        except Exception as exc:   # the traffic stays unmeasured; the reason goes to stderr
+            print(f"bench: batched 2D traffic not measured: {exc}", file=sys.stderr, flush=True)
     return out
 
 
@@ -312,9 +318,10 @@ def _serve_stream(ctx, stream):
                "makespan_s": round(st.makespan, 3), "gcell_per_s": round(st.cell_updates / st.makespan / 1e9, 2),
                "completed": st.completed, "failed": st.failed, "peak_active": st.peak_active,
                "peak_reserved_GB": round(st.peak_bytes / 1e9, 2), "ticks": st.ticks}
-        eff = st.alg_bytes / st.makespan / 1e9
-        res["effective_GBps"] = round(eff, 1)
-        res["effective_roofline_frac"] = round(eff / HBM_PEAK_GBS, 4)
+        # the half-step method's algorithmic bytes over the makespan: a work
+        # rate, not a bandwidth (the tiled 2D kernel keeps fields in registers
+        # for 8 steps and the fused 3D step reads each value once)
+        res["alg_GBps_per_makespan"] = round(st.alg_bytes / st.makespan / 1e9, 1)
         out[mode] = res
         print(f"bench: serving {mode}: {res}", file=sys.stderr, flush=True)
         srv.close()
@@ -428,10 +435,11 @@ def snapshot_section(args, w, ctx, sim):
     return out
 
 
-def _pmc_pass(counter, args, tune, outdir):
+def _pmc_pass(counter, args, tune, outdir, child_args=None):
     """One `rocprofv3 --pmc <counter>` run of a short bench.py child (same
     workload, the tuned configuration pinned through LX_TUNE, plus a device
-    copy of known size); returns {dispatch id: (kernel name, value)}."""
+    copy of known size; or the child arguments given); returns
+    {dispatch id: (kernel name, value)}."""
     import csv
     import glob
     import signal
@@ -444,13 +452,20 @@ def _pmc_pass(counter, args, tune, outdir):
     if prof is None:
         raise RuntimeError("rocprofv3 not found")
     cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", outdir, "-o", counter.lower(), "--",
-           sys.executable, os.path.join(ROOT, "bench.py"), "--workload", args.workload, "--n", str(args.n),
-           "--steps", "4", "--warmup", "1", "--kernel-steps", "2", "--copy-calib", str(args.traffic_copy_mib),
-           "--no-batch", "--no-serve", "--no-cpu", "--no-snap", "--no-traffic"]
-    if args.fp16:
-        cmd.append("--fp16")
-    env = dict(os.environ, LX_TUNE=tune, TMPDIR="/tmp")
-    print(f"bench: traffic pass {counter} (LX_TUNE={tune})", file=sys.stderr, flush=True)
+           sys.executable, os.path.join(ROOT, "bench.py")]
+    if child_args is None:
+        cmd += ["--workload", args.workload, "--n", str(args.n), "--steps", "4", "--warmup", "1", "--kernel-steps",
+                "2", "--copy-calib", str(args.traffic_copy_mib), "--no-batch", "--no-serve", "--no-cpu", "--no-snap",
+                "--no-traffic"]
+        if args.fp16:
+            cmd.append("--fp16")
+    else:
+        cmd += child_args
+    env = dict(os.environ, TMPDIR="/tmp")
+    if tune:
+        env["LX_TUNE"] = tune
+    print(f"bench: traffic pass {counter} ({'LX_TUNE=' + tune if tune else ' '.join(child_args)})", file=sys.stderr,
+          flush=True)
     p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
                          start_new_session=True)
     try:
@@ -473,6 +488,71 @@ def _pmc_pass(counter, args, tune, outdir):
         raise RuntimeError(f"rocprofv3 --pmc {counter}: no counter rows (exit {p.returncode}): "
                            f"{err.decode(errors='replace')[-400:]}")
     return rows
+
+
+def batch_child(args):
+    """A short run of the batched 2D workload for the PMC passes of
+    batch_traffic: the same 256 members, --batch-child steps, and a device
+    copy of --traffic-copy-mib MiB (the counters' calibration)."""
+    import torch
+    from luxum_amd import configs
+    from luxum_amd.sim import Batch, Context, Simulation
+    a = torch.ones(args.traffic_copy_mib * (1 << 18), device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    ctx = Context(0)
+    sizes = configs.batch_sizes(args.batch_sims)
+    ws = [configs.phc2d(n, i, steps=args.batch_steps) for i, n in enumerate(sizes)]
+    sims = []
+    bt = Batch(ctx)
+    for w in ws:
+        s = Simulation(w.p, ctx)
+        s.set_materials(w.ids, w.eps_r)
+        s.set_probes(w.probes)
+        sims.append(s)
+        bt.add(s)
+    bt.step(args.batch_child)
+    bt.sync()
+    bt.close()
+
+
+def batch_traffic(args, cells, gcell_per_s):
+    """HBM bytes per cell-update of the batched 2D step: FETCH_SIZE and
+    WRITE_SIZE summed over every dispatch of the batch's kernels (the tiled
+    kernel of each slab class and the aux-line kernel) in a short child run
+    of the same members, each counter scaled by what it reads for a device
+    copy of known size; and the HBM bandwidth those bytes take at the
+    measured cell rate, as a fraction of the peak."""
+    import re
+    import shutil
+    import tempfile
+    steps = args.batch_traffic_steps
+    child = ["--batch-child", str(steps), "--batch-sims", str(args.batch_sims), "--batch-steps", str(args.batch_steps),
+             "--traffic-copy-mib", str(args.traffic_copy_mib)]
+    pat = re.compile(r"k_tile2d|k_aux_chunk")
+    copy_bytes = args.traffic_copy_mib * (1 << 20)
+    tot = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        outdir = tempfile.mkdtemp(prefix="lx_pmc_", dir="/tmp")
+        try:
+            rows = _pmc_pass(counter, args, None, outdir, child_args=child)
+        finally:
+            shutil.rmtree(outdir, ignore_errors=True)
+        ks = [v for name, v in rows.values() if pat.search(name)]
+        cp = [v for name, v in rows.values() if "copy" in name.lower()]
+        if not ks or not cp:
+            raise RuntimeError(f"{counter}: no dispatch of the batch's kernels or of the copy kernel")
+        factor = copy_bytes / (max(cp) * 1024.0)
+        tot[counter] = (sum(ks) * 1024.0 * factor, factor, len(ks))
+    cu = cells * steps
+    rd, wr = tot["FETCH_SIZE"][0] / cu, tot["WRITE_SIZE"][0] / cu
+    return {"bytes_per_cell_update": round(rd + wr, 3), "read_per_cell_update": round(rd, 3),
+            "write_per_cell_update": round(wr, 3), "steps": steps, "dispatches": tot["FETCH_SIZE"][2],
+            "fetch_factor": round(tot["FETCH_SIZE"][1], 3), "write_factor": round(tot["WRITE_SIZE"][1], 3),
+            "hbm_GBps": round((rd + wr) * gcell_per_s, 1),
+            "hbm_frac": round((rd + wr) * gcell_per_s / HBM_PEAK_GBS, 4)}
 
 
 def traffic_section(args, info, dom_pattern):
@@ -521,6 +601,7 @@ def main():
     ap.add_argument("--cpu-ref-steps", type=int, default=30,
                     help="config-1 steps of the CPU run with the reference's monitor and |E|^2 output")
     ap.add_argument("--cpu-child", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--batch-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-steps", type=int, default=50)
     ap.add_argument("--batch-sims", type=int, default=256)
@@ -543,9 +624,14 @@ def main():
                     help="skip the two rocprofv3 --pmc passes that measure the dominant kernel's HBM bytes")
     ap.add_argument("--traffic-copy-mib", type=int, default=512)
     ap.add_argument("--traffic-timeout", type=float, default=240.0)
+    ap.add_argument("--batch-traffic-steps", type=int, default=48, dest="batch_traffic_steps",
+                    help="steps of the batched 2D child run under rocprofv3 --pmc")
     args = ap.parse_args()
     if args.cpu_child:
         cpu_child(args)
+        return
+    if args.batch_child:
+        batch_child(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "

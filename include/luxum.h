@@ -137,6 +137,9 @@ typedef struct lx_sim_info {
                                    (autotuned); else kz */
     int32_t shell_concurrent;   /* fused step on a CPML grid: 1 when the shell launch runs on a second
                                    stream beside the interior launch (autotuned); else 0 */
+    int32_t shell_pipelined;    /* 1 when, in addition, the shell launch of a step may overlap the TF/SF
+                                   launch before it (every face cell, probe and the point source lie two
+                                   cells inside the interior box); else 0 */
 } lx_sim_info;
 
 /* ---- parameters ------------------------------------------------------- */

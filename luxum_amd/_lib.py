@@ -60,6 +60,7 @@ class SimInfo(C.Structure):
         ("cells_per_lane", C.c_int32),
         ("shell_kz", C.c_int32),
         ("shell_concurrent", C.c_int32),
+        ("shell_pipelined", C.c_int32),
     ]
 
 

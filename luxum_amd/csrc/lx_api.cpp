@@ -491,6 +491,33 @@ bool tf_merge_ok(const lx_sim *s) {
     return true;
 }
 
+// Can the CPML shell launch of step q+1 start before the TF/SF launch that
+// follows step q's interior launch?  The shell reads and writes only shell
+// cells and the interior box's cells next to them (the rows and planes it
+// recomputes); the launch between steps writes TF/SF face cells and reads the
+// probe cells, and the point source writes its cell.  When all of these lie
+// at least two cells inside the interior box, the two touch disjoint cells,
+// and the shell's only orders are the interior launch before it (whose output
+// it reads) and the interior launch after it (which reads its output).
+// LX_SHELL_PIPE=0 keeps the shell launch between the same events as the
+// interior launch.
+bool shell_pipe_ok(const lx_sim *s) {
+    if (!s->ks || !s->kconc || !s->pml || !s->shell_pipe) return false;
+    UnitMap um[2];
+    int nu[2];
+    step3_unit_maps(s->N[0], s->N[1], s->N[2], s->kz, s->kzs, s->pml, s->slo, s->shi, s->vec, um, nu);
+    const int jlo = um[0].jlo + 2, jhi = um[0].jhi - 3;
+    const int zlo = um[0].tlo * um[0].kz + 2, zhi = std::min(um[0].thi * um[0].kz, s->N[2]) - 3;
+    auto inside = [&](int j, int k) { return j >= jlo && j <= jhi && k >= zlo && k <= zhi; };
+    for (const TfList *L : {&s->tfE, &s->tfH})
+        for (int b = 0; b < L->n; ++b)
+            if (!inside(L->b[b].lo[1], L->b[b].lo[2]) || !inside(L->b[b].hi[1], L->b[b].hi[2])) return false;
+    for (int q = 0; q < s->np; ++q)
+        if (!inside(s->probe_host[4 * (size_t)q + 2], s->probe_host[4 * (size_t)q + 3])) return false;
+    if (s->p.use_ps && !inside(s->p.src_pos[1], s->p.src_pos[2])) return false;
+    return true;
+}
+
 // One launch sequence of n timesteps: per step [point source] H launch,
 // E launch; then the sequence-end kernel.  Kernels read the step index as
 // counter[0] + toff, so the same sequence can be captured once and replayed.
@@ -510,6 +537,15 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
         T *buf[2] = {(T *)s->F, (T *)s->F_alt};
         float *pbuf[2] = {s->psi, s->psi_alt};
         const bool merge = corr && !ev && tf_merge_ok(s);
+        // pipelined shell: the shell of step q waits for the interior launch
+        // of step q-1 (event kfork) and the interior launch of step q for the
+        // shell of step q-1 (event kjoin), so a shell overlaps the interior
+        // launch of its own step and the TF/SF launch before it
+        const bool pipe = merge && shell_pipe_ok(s);
+        if (pipe) {
+            (void)hipEventRecord(s->kfork, s->cs);
+            (void)hipStreamWaitEvent(s->ks, s->kfork, 0);
+        }
         AuxLine aset[2] = {s->aux, s->aux};
         aset[1].e = s->aux.e2;
         aset[1].h = s->aux.h2;
@@ -523,7 +559,16 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
             if (s->p.use_ps) launch_point_source(gq, s->ps, pr, q, s->cs);
             if (corr && (!merge || q == 0)) launch_tfsf_hpre(gq, s->tfH, s->inc, s->aux, pr, l, s->cs);
             // timed parts: [0,1] interior box, [2,3] CPML shell
-            if (!ev) {
+            if (pipe) {
+                if (q >= 1) {
+                    (void)hipStreamWaitEvent(s->cs, s->kjoin, 0);   // shell of step q-1
+                    (void)hipStreamWaitEvent(s->ks, s->kfork, 0);   // interior of step q-1
+                }
+                launch_step3(gq, s->nw, s->vec, 1, s->ks);
+                (void)hipEventRecord(s->kjoin, s->ks);
+                launch_step3(gq, s->nw, s->vec, 0, s->cs);
+                (void)hipEventRecord(s->kfork, s->cs);
+            } else if (!ev) {
                 launch_fused_step(s, gq);
             } else {
                 if (ev) (void)hipEventRecord((*ev)[4 * q + 0], s->cs);
@@ -545,6 +590,7 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
                 launch_tfsf(gn, s->tfE, s->inc, s->aux, pr, l, 1, s->cs);
             }
         }
+        if (pipe) (void)hipStreamWaitEvent(s->cs, s->kjoin, 0);   // the last shell
         Grid<T> gf = g;
         gf.F = buf[n & 1];
         launch_finish(gf, pr, n, s->cs);
@@ -864,6 +910,8 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
     s->graphs = !(ng && *ng && *ng != '0');
     const char *mg = std::getenv("LX_TFSF_MERGE");   // 0: two TF/SF launches per fused step
     s->tf_merge = !(mg && *mg == '0');
+    const char *sp = std::getenv("LX_SHELL_PIPE");   // 0: the shell launch between the interior launch's events
+    s->shell_pipe = !(sp && *sp == '0');
     // constants exactly as maxwell.cc:63-84
     s->eps0 = 8.85 * std::pow(10, -12);
     s->mu0 = M_PI * 4 * std::pow(10, -7);
@@ -1246,6 +1294,8 @@ lx_status lx_sim_get_info(lx_sim *s, lx_sim_info *info) {
     info->cells_per_lane = s->fused ? s->vec : 1;
     info->shell_kz = s->fused && s->pml && s->kzs > 0 ? s->kzs : s->kz;
     info->shell_concurrent = s->fused && s->ks && s->kconc ? 1 : 0;
+    info->shell_pipelined = s->fused && shell_pipe_ok(s) && (s->tfH.n > 0 || s->use_aux || s->np > 0) &&
+                            tf_merge_ok(s) ? 1 : 0;
     if (s->fused && s->pml) {
         int rows = 0, planes = 0;
         step3_core_extent(s->N[1], s->N[2], s->kz, s->slo, s->shi, &rows, &planes);

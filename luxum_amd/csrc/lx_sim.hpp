@@ -57,6 +57,7 @@ struct lx_sim {
     hipEvent_t kfork = nullptr, kjoin = nullptr;
     int kconc = 1;              // use ks (when it exists); autotuned
     int tf_merge = 1;           // fused step: one TF/SF launch between steps (LX_TFSF_MERGE=0: two per step)
+    int shell_pipe = 1;         // fused step: shell launch pipelined past the TF/SF launch (LX_SHELL_PIPE=0: not)
     std::vector<std::pair<void *, size_t>> allocs;   // device blocks and their sizes
     int64_t bytes = 0;
     void *F = nullptr;          // six components, component c at F + c*fstride elements
@@ -117,6 +118,8 @@ bool fused_ok(const lx_sim *s);
 bool vec_ok(const lx_sim *s, int vec);
 // fused sequences may apply one TF/SF launch between steps (no H probe on an H face cell)
 bool tf_merge_ok(const lx_sim *s);
+// the CPML shell launch may overlap the TF/SF launch before it (disjoint cells)
+bool shell_pipe_ok(const lx_sim *s);
 // The side stream (snapshot copies) is taken from the context's pool on
 // first use.
 lx_status ensure_side_stream(lx_sim *s);

@@ -179,22 +179,31 @@ def test_phc_slab_full_size_z_mirror(gpu_ctx, prec):
     """Configuration 2 at its full size and length: 1280 x 1280 x 640, rods
     eps 12 in the 64-cell slab, the removed row, CPML 16, 4000 steps, with
     the soft Ez dipole moved half a cell, onto the slab's mid-plane (Ez index
-    319 of 639, position 319.5, the grid's centre plane).  The grid, CPML,
-    slab and source are then symmetric under z -> 639 - z (nodes) /
-    638 - m (staggered), so every component equals +/- its mirror image with
-    the parities of a z-directed dipole on the plane (Ex, Ey, Hz odd; Ez,
-    Hx, Hy even), exactly, while the field has crossed the rods, the slab
-    and the CPML on every face.  A wrong index, coefficient or sign on one
-    side of a slab, a z block, a launch boundary or the z CPML breaks it."""
+    319 of 639, position 319.5, the grid's centre plane).  Nodes map as
+    k -> 639 - k and the z-staggered Ez as m -> 638 - m.  A node's material
+    sets its three E components at that array index (as the reference's Cb
+    arrays do), so Ez at the slab's top node plane 351 (position 351.5) lies
+    half a cell outside the slab, whose bottom face Ez at 287 (287.5) mirrors:
+    per-component permittivities give the rod nodes of plane 351 eps 12 for
+    Ex and Ey and 1 for Ez, so the structure seen by every component is
+    symmetric.  Every component then equals +/- its mirror image with the
+    parities of a z-directed dipole on the plane (Ex, Ey, Hz odd; Ez, Hx, Hy
+    even), exactly, while the field has crossed the rods, the slab and the
+    CPML on every face.  A wrong index, coefficient or sign on one side of a
+    slab, a z block, a launch boundary or the z CPML breaks it."""
     from luxum_amd.sim import Simulation
     steps = 4000
     w = configs.config2(steps=steps, precision=prec)
     w.p.src_pos[2] = 319
-    w.probes[:, 3] = 319
+    top = 640 // 2 + 64 // 2 - 1                  # the slab's top node plane, 351
+    ids = w.ids
+    plane = ids[:, :, top]
+    plane[plane == 1] = 2
+    eps3 = np.array([[1.0, 1.0, 1.0], [12.0, 12.0, 12.0], [12.0, 12.0, 1.0]])
     say(f"case PhC slab full size z-mirror prec={prec} steps={steps}")
     s = Simulation(w.p, gpu_ctx)
-    s.set_materials(w.ids, w.eps_r)
-    del w
+    s.set_materials(ids, eps3)
+    del w, ids, plane
     kz = s.autotune()
     s.step(steps)
     s.sync()

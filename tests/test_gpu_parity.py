@@ -560,3 +560,38 @@ def test_storage_scale(gpu_ctx):
         s.close()
     say(f"FP16 dipole of amplitude 1e-6 vs FP64 oracle: automatic scale {errs[0.0]:.3e}, unscaled {errs[1.0]:.3e}")
     assert errs[0.0] <= 5e-2
+
+
+def test_fused_shell_pipelined(gpu_ctx, monkeypatch):
+    """The CPML shell launch of a step overlapping the TF/SF launch before it
+    (only the interior launch before and after it order the shell) writes
+    the same fields and probes as the shell between the interior launch's
+    events: FP32 and FP16, an odd step count, graph replay and eager."""
+    from luxum_amd.sim import Simulation
+
+    def run(w, pipe, graphs):
+        monkeypatch.setenv("LX_FUSED", "1" if w.p.precision == configs.LX_FP32 else "4")
+        monkeypatch.setenv("LX_SHELL_STREAM", "1")
+        monkeypatch.setenv("LX_SHELL_PIPE", "1" if pipe else "0")
+        monkeypatch.setenv("LX_NO_GRAPH", "0" if graphs else "1")
+        s = Simulation(w.p, gpu_ctx)
+        s.set_materials(w.ids, w.eps_r)
+        s.set_probes(w.probes)
+        assert s.info().shell_concurrent == 1
+        assert s.info().shell_pipelined == int(pipe)
+        s.step(45)
+        s.step(30)
+        s.sync()
+        return s
+
+    for prec in (configs.LX_FP32, LX_FP16):
+        w = configs.config1(n=96, steps=75, pml=8, margin=10, precision=prec)
+        ref = run(w, False, True)
+        for graphs in (True, False):
+            say(f"case {prec} shell pipelined past the TF/SF launch, graphs {graphs}")
+            s = run(w, True, graphs)
+            for c in w.p.components():
+                assert np.array_equal(s.field(c), ref.field(c)), f"{prec} graphs {graphs} component {c}"
+            assert np.array_equal(s.probes(), ref.probes())
+            s.close()
+        ref.close()
