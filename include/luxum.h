@@ -203,6 +203,16 @@ lx_status lx_sim_sync(lx_sim *sim);
  * n must equal the component's element count.  Synchronous. */
 lx_status lx_sim_read_field(lx_sim *sim, int32_t comp, double *dst, size_t n);
 
+/* No counterpart in the reference: lx_sim_read_field for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops).  Same arguments and the same values; calls no hipMalloc/hipFree:
+ * the component is gathered in slices of whole x indices through a fixed
+ * 32 MiB device array per GPU (held from library load until the process
+ * ends) and copied slice by slice into dst.  LX_EINVAL when one x slice
+ * (ny*nz values of the component's extents) exceeds 4194304 values.  Calls
+ * that overlap on one GPU share that array and race.  Synchronous. */
+lx_status lx_sim_read_field_noalloc(lx_sim *sim, int32_t comp, double *dst, size_t n);
+
 /* Replaces fileio::writeHDF5's data path (lib/io/fileio.cc:184-212): enqueue
  * a snapshot of `comp` on the side stream into dst (pinned or pageable host
  * memory, reference layout, FP64).  The copy reflects the state after the

@@ -23,7 +23,7 @@ EXPORTS = (
     "lx_params_default", "lx_params_load_yaml", "lx_params_check",
     "lx_ctx_create", "lx_ctx_destroy", "lx_ctx_mem_info",
     "lx_sim_create", "lx_sim_destroy", "lx_sim_set_materials", "lx_sim_set_materials3", "lx_sim_set_probes",
-    "lx_sim_step", "lx_sim_sync", "lx_sim_read_field", "lx_sim_snapshot_async",
+    "lx_sim_step", "lx_sim_sync", "lx_sim_read_field", "lx_sim_read_field_noalloc", "lx_sim_snapshot_async",
     "lx_sim_read_probes", "lx_sim_monitor", "lx_sim_get_info", "lx_sim_stream",
     "lx_sim_time_kernels", "lx_sim_autotune",
     "lx_batch_create", "lx_batch_destroy", "lx_batch_add", "lx_batch_remove",
@@ -132,6 +132,7 @@ def lib():
         "lx_sim_step": ([vp, C.c_int32], st),
         "lx_sim_sync": ([vp], st),
         "lx_sim_read_field": ([vp, C.c_int32, vp, C.c_size_t], st),
+        "lx_sim_read_field_noalloc": ([vp, C.c_int32, vp, C.c_size_t], st),
         "lx_sim_snapshot_async": ([vp, C.c_int32, vp, C.c_size_t], st),
         "lx_sim_read_probes": ([vp, vp, C.c_size_t, C.POINTER(C.c_int32)], st),
         "lx_sim_monitor": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], st),

@@ -832,6 +832,11 @@ lx_status lx_ctx_create(int32_t device, lx_ctx **out) {
     HIP_TRY(hipMemGetInfo(&fr, &tot));
     c->total_mem = (int64_t)tot;
     c->cache_limit = std::min<int64_t>(16LL << 30, c->total_mem / 16);
+    lx_status st = noalloc_init(device);   // once per device: the address of its readback scratch
+    if (st) {
+        delete c;
+        return st;
+    }
     *out = c;
     return LX_OK;
 }

@@ -133,4 +133,7 @@ lx_status ensure_fused_buffers(lx_sim *s);
 // |E|^2 (LX_OUT_E2) into dst_dev in the FP64 host layout of extents ext
 lx_status gather_output(lx_sim *s, int what, int comp, double *dst_dev, const int ext[3]);
 void release_fused_buffers(lx_sim *s);
+// Look up the device's allocation-free readback scratch (lx_noalloc.hip);
+// called with the device current.
+lx_status noalloc_init(int device);
 }  // namespace lx
