@@ -222,11 +222,14 @@ def test_phc_slab_full_size_z_mirror(gpu_ctx, prec):
         for a in range(3):
             edge = np.take(f, range(0, 8), axis=a)
             assert np.abs(edge).max() > 1e-12 * np.abs(f).max(), (c, a)
-        asym = float(np.linalg.norm(f - pz * f[:, :, ::-1])) / nrm
-        flip = float(np.linalg.norm(f + pz * f[:, :, ::-1])) / nrm
+        # every 5th x plane (1.3 GB of FP64 per component instead of 8.4)
+        g = f[::5]
+        ng = float(np.linalg.norm(g))
+        asym = float(np.linalg.norm(g - pz * g[:, :, ::-1])) / ng
+        flip = float(np.linalg.norm(g + pz * g[:, :, ::-1])) / ng
         say(f"  comp {c}: asymmetry {asym:.3e} (flipped parity {flip:.3f})")
         assert flip > 1.9
         worst = max(worst, asym)
-        del f
+        del f, g
     s.close()
     assert worst == 0.0, worst
