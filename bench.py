@@ -307,12 +307,14 @@ def _serve_stream(ctx, stream):
     import numpy as np
     from luxum_amd.server import Server
     out = {}
+    series = {}
     for mode in ("batched", "plain"):
         srv = Server(ctx, hbm_cap=200 * 10 ** 9, plain=(mode == "plain"))
         ids = [srv.submit(w.p, w.ids, w.eps_r, steps, arr, w.probes) for w, steps, arr in stream]
         srv.run()
         st = srv.stats()
         lat = np.array([srv.result(i).latency for i in ids])
+        series[mode] = [srv.probes(i) for i in ids]
         res = {"solves_per_s": round(st.completed / st.makespan, 3),
                "p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
                "makespan_s": round(st.makespan, 3), "gcell_per_s": round(st.cell_updates / st.makespan / 1e9, 2),
@@ -326,6 +328,18 @@ def _serve_stream(ctx, stream):
         print(f"bench: serving {mode}: {res}", file=sys.stderr, flush=True)
         srv.close()
     out["speedup_solves_per_s"] = round(out["batched"]["solves_per_s"] / out["plain"]["solves_per_s"], 2)
+    # each request's probe series served in the mix against the same request
+    # run alone (the plain server runs every request by itself), by storage
+    # precision: the mix runs 2D requests in the tiled kernel, whose rounding
+    # differs from the single-simulation kernels'
+    from luxum_amd.params import LX_FP16
+    errs = {"fp32": [], "fp16": []}
+    for (w, _, _), a, b in zip(stream, series["batched"], series["plain"]):
+        nb = float(np.linalg.norm(b))
+        e = float(np.linalg.norm(a - b)) / nb if nb > 0 else float(np.linalg.norm(a))
+        errs["fp16" if w.p.precision == LX_FP16 else "fp32"].append(e)
+    out["rel_l2_mix_vs_alone"] = {k: {"max": max(v), "median": float(np.median(v)), "requests": len(v)}
+                                  for k, v in errs.items() if v}
     return out
 
 
@@ -490,6 +504,46 @@ def _pmc_pass(counter, args, tune, outdir, child_args=None):
     return rows
 
 
+def fp16_section(args, ctx):
+    """The main line's workload with E and H stored in FP16 (FP32
+    arithmetic): the same timed run (autotune, warmup, --steps steps), the
+    whole step's and the interior launch's fraction of the HBM roofline (the
+    algorithmic bytes count each stored value at 2 B)."""
+    from luxum_amd import configs
+    from luxum_amd.params import LX_FP16
+    from luxum_amd.sim import Simulation
+    total = max(args.steps + args.warmup + args.kernel_steps, 2000)
+    w = configs.config1(n=args.n, steps=total, precision=LX_FP16)
+    sim = Simulation(w.p, ctx)
+    sim.set_materials(w.ids, w.eps_r)
+    sim.set_probes(w.probes)
+    sim.autotune()
+    sim.step(args.warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    elapsed = time.perf_counter() - t0
+    ms_h, ms_e, _ = sim.time_kernels(args.kernel_steps)
+    info = sim.info()
+    sim.close()
+    ms_step = elapsed * 1e3 / args.steps
+    step_gbs = info.bytes_per_step / (ms_step * 1e-3) / 1e9
+    core_gbs = info.core_bytes / (ms_h * 1e-3) / 1e9
+    print(f"bench: FP16 storage {ms_step:.4f} ms/step", file=sys.stderr, flush=True)
+    return {"workload": "the main line's workload, E and H stored in FP16 (FP32 arithmetic, automatic "
+                        "power-of-two storage scale)",
+            "value": round(w.cells * args.steps / elapsed / 1e9, 3), "unit": "Gcell-updates/s",
+            "steps": args.steps, "ms_per_step": round(ms_step, 5), "bytes_per_step": info.bytes_per_step,
+            "step_GBps": round(step_gbs, 1), "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+            "roofline": {"bound": "hbm", "kernel": "k_step3 interior box (fused H+E step), FP16 storage",
+                         "achieved": round(core_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(core_gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": info.core_bytes,
+                         "ms_per_launch": round(ms_h, 5)},
+            "ms_shell": round(ms_e, 5), "kz": info.kz, "cells_per_lane": info.cells_per_lane,
+            "shell_concurrent": bool(info.shell_concurrent), "shell_pipelined": bool(info.shell_pipelined)}
+
+
 def batch_child(args):
     """A short run of the batched 2D workload for the PMC passes of
     batch_traffic: the same 256 members, --batch-child steps, and a device
@@ -616,6 +670,7 @@ def main():
     ap.add_argument("--serve2d-rate", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-snap", action="store_true", help="skip the snapshot-output section")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-storage run of the main line's workload")
     ap.add_argument("--snap-steps", type=int, default=400)
     ap.add_argument("--snap-every", type=int, default=50)
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -728,9 +783,12 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": dbytes, "ms_per_launch": round(dms, 5)},
     }
+    out["config"]["shell_pipelined"] = bool(info.shell_pipelined)
     if not args.no_snap and args.workload == "cube":
         out["snapshots"] = snapshot_section(args, w, ctx, sim)
     sim.close()
+    if not args.no_fp16 and args.workload == "cube" and not args.fp16:
+        out["fp16"] = fp16_section(args, ctx)
     if not args.no_traffic:
         try:
             t = traffic_section(args, info, dom_pat)

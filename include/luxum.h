@@ -270,7 +270,10 @@ lx_status lx_batch_sync(lx_batch *b);
  * of their device footprints stays within an HBM cap and the number of
  * active requests within a slot limit, and advances every active request
  * together in ticks (LX_SERVE_BATCHED: one lx_batch; a tick is the smallest
- * remaining step count of the active requests, at most 128 steps).  Finished
+ * remaining step count of the active requests, at most 128 steps).  3D
+ * requests are admitted in arrival order while the 3D cells already active
+ * stay below 8 Mi cells (LX_SERVE_3D_CELLS overrides; 0: no limit), so large
+ * 3D grids, each of which fills the device, run one after another.  Finished
  * requests leave at the end of their tick; their probe series (and fields,
  * when asked) stay readable.  LX_SERVE_PLAIN runs the requests one at a time
  * in arrival order with the single-simulation kernels (the per-request

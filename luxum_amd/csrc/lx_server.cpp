@@ -9,6 +9,14 @@
 // memory released).  Requests that arrive while a tick runs join at the next
 // tick boundary, so a long request never blocks a short one behind it.
 //
+// 3D requests also wait for the device's 3D capacity: they are admitted in
+// arrival order while the 3D cells already active stay below k3dCells (a 3D
+// grid of that size fills the device on its own), so large 3D requests run
+// one after another at full speed instead of sharing every tick with each
+// other, which would stretch the latency of every early arrival; small 3D
+// grids still batch together, and 2D requests fill in first fit around them.
+// LX_SERVE_3D_CELLS overrides the limit (0: none).
+//
 // A tick is the smallest remaining step count of the active requests, capped
 // at kTick: every request ends exactly on a tick boundary.  Batched mode
 // steps one lx_batch (planar requests in one tiled launch per chunk, the
@@ -21,6 +29,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <new>
@@ -36,6 +45,7 @@ using lx::fail;
 namespace {
 
 constexpr int kTick = 128;
+constexpr int64_t k3dCells = int64_t(8) << 20;
 constexpr int kPrewarm = 64;   // pooled streams created with the server
 
 struct Req {
@@ -73,6 +83,7 @@ struct lx_server {
     std::vector<Req *> active;
     std::vector<Req *> retiring;  // finished on the device, results not yet read back
     int64_t used = 0;
+    int64_t cells3d_cap = 0;      // 3D cells that may be active together (0: no limit)
     lx_server_stats st{};
     Clock::time_point t0;
 };
@@ -134,17 +145,31 @@ void fail_request(lx_server *s, Req *r) {
     s->st.failed++;
 }
 
-// admit arrived requests that fit, first fit in arrival order
+// admit arrived requests that fit, first fit in arrival order; 3D requests
+// in strict arrival order within the 3D cell limit
 lx_status admit_arrived(lx_server *s) {
     const double now = since(s);
+    int64_t cells3d = 0;
+    for (const Req *r : s->active)
+        if (!r->p.is_planar) cells3d += r->cells;
+    bool hold3d = false;   // an earlier 3D request waits: later ones wait behind it
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         Req *r = *it;
         if (r->arrival > now) break;
         if ((int)s->active.size() >= s->max_active) break;
-        if (s->used + r->footprint > s->cap) {
+        const bool is3d = !r->p.is_planar;
+        if (is3d && s->mode == LX_SERVE_BATCHED &&
+            (hold3d || (s->cells3d_cap > 0 && cells3d > 0 && cells3d + r->cells > s->cells3d_cap))) {
+            hold3d = true;
             ++it;
             continue;
         }
+        if (s->used + r->footprint > s->cap) {
+            if (is3d) hold3d = true;
+            ++it;
+            continue;
+        }
+        if (is3d) cells3d += r->cells;
         s->used += r->footprint;
         r->state = LX_REQ_ACTIVE;   // reserved
         lx_status st = admit(s, r);
@@ -237,6 +262,8 @@ lx_status lx_server_create(lx_ctx *ctx, int32_t mode, int64_t hbm_cap_bytes, int
     s->mode = mode;
     s->cap = hbm_cap_bytes;
     s->max_active = mode == LX_SERVE_PLAIN ? 1 : max_active;
+    s->cells3d_cap = k3dCells;
+    if (const char *e = std::getenv("LX_SERVE_3D_CELLS")) s->cells3d_cap = std::max<int64_t>(0, std::atoll(e));
     if (hipSetDevice(ctx->device) != hipSuccess) {
         delete s;
         return fail(LX_EDEVICE, "hipSetDevice(%d) failed", ctx->device);
