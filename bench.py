@@ -541,7 +541,8 @@ def fp16_section(args, ctx):
                          "frac": round(core_gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": info.core_bytes,
                          "ms_per_launch": round(ms_h, 5)},
             "ms_shell": round(ms_e, 5), "kz": info.kz, "cells_per_lane": info.cells_per_lane,
-            "shell_concurrent": bool(info.shell_concurrent), "shell_pipelined": bool(info.shell_pipelined)}
+            "shell_concurrent": bool(info.shell_concurrent), "shell_pipelined": bool(info.shell_pipelined),
+            "interior_rows": info.interior_rows}
 
 
 def batch_child(args):
@@ -618,7 +619,8 @@ def traffic_section(args, info, dom_pattern):
     import tempfile
     import statistics
     nw = info.waves if info.waves in (4, 8) else 4
-    tune = f"{int(info.fused_step)},{nw},{info.cells_per_lane},{info.kz},{int(info.shell_concurrent)}"
+    tune = (f"{int(info.fused_step)},{nw},{info.cells_per_lane},{info.kz},{int(info.shell_concurrent)},"
+            f"{info.interior_rows}")
     copy_bytes = args.traffic_copy_mib * (1 << 20)
     pat = re.compile(dom_pattern)
     res = {}
@@ -745,7 +747,10 @@ def main():
         # fused H+E step: the interior-box launch (ms_h) and, with CPML, the
         # shell launch around it (ms_e); the box launch is the dominant kernel
         dom, dbytes, dms = "k_step3 interior box (fused H+E step)", info.core_bytes, ms_h
-        dom_pat = r"k_step3<[^<>,]+, [01], \d+, \d+>"
+        dom_pat = r"k_step3r?<[^<>,]+, [01], \d+, \d+>"
+        if info.interior_rows:
+            dom = (f"k_step3r interior box (fused H+E step, {info.interior_rows} rows per block sharing neighbour "
+                   f"rows through LDS)")
         fused_frac = info.bytes_per_step / ((ms_h + ms_e) * 1e-3) / 1e9 / HBM_PEAK_GBS
     else:
         bytes_h = info.cells * 9 * s + slab * 16
@@ -776,7 +781,7 @@ def main():
                    "step_GBps": round(step_gbs, 1), "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
                    "ms_h": round(ms_h, 5), "ms_e": round(ms_e, 5), "max_E2": mx, "kz": info.kz,
                    "waves_per_block": info.waves, "cells_per_lane": info.cells_per_lane, "shell_kz": info.shell_kz,
-                   "shell_concurrent": bool(info.shell_concurrent),
+                   "shell_concurrent": bool(info.shell_concurrent), "interior_rows": info.interior_rows,
                    "fused_step": bool(info.fused_step),
                    "fused_step_roofline_frac": None if fused_frac is None else round(fused_frac, 4)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

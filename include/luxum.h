@@ -140,6 +140,9 @@ typedef struct lx_sim_info {
     int32_t shell_pipelined;    /* 1 when, in addition, the shell launch of a step may overlap the TF/SF
                                    launch before it (every face cell, probe and the point source lie two
                                    cells inside the interior box); else 0 */
+    int32_t interior_rows;      /* fused step: rows per block of the interior launch that share their
+                                   neighbour rows through LDS (3 or 7, autotuned); 0: one unit per
+                                   wavefront, neighbour rows read and recomputed */
 } lx_sim_info;
 
 /* ---- parameters ------------------------------------------------------- */

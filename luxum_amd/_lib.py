@@ -61,6 +61,7 @@ class SimInfo(C.Structure):
         ("shell_kz", C.c_int32),
         ("shell_concurrent", C.c_int32),
         ("shell_pipelined", C.c_int32),
+        ("interior_rows", C.c_int32),
     ]
 
 

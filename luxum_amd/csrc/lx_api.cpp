@@ -73,6 +73,7 @@ Grid<T> make_grid(const lx_sim *s) {
     g.pml = s->pml;
     g.kz = s->kz;
     g.kzs = s->kzs;
+    g.rows = s->rows;
     g.ntx = (g.nx + 255) / 256;
     if (s->planar) {
         g.nty = (g.nz + kBY - 1) / kBY;
@@ -974,6 +975,12 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
                 s->kz = 8;
                 s->nw = 4;
                 s->vec = 1;
+                // LX_ROWS: rows per block of the interior launch sharing neighbours through LDS
+                // (3 or 7; 0: one unit per wavefront)
+                if (const char *rw = std::getenv("LX_ROWS"); rw && *rw) {
+                    const int r = std::atoi(rw);
+                    s->rows = (r == 3 || r == 7) ? r : 0;
+                }
                 if ((f == '4' || (f == 'd' && s->fp16)) && vec_ok(s, 4)) s->vec = 4;
                 else if ((f == '2' || (f == 'd' && s->fp16)) && vec_ok(s, 2)) s->vec = 2;
                 const char *sk = std::getenv("LX_SHELL_KZ");   // z-block depth of the CPML shell launch
@@ -1299,6 +1306,7 @@ lx_status lx_sim_get_info(lx_sim *s, lx_sim_info *info) {
     info->cells_per_lane = s->fused ? s->vec : 1;
     info->shell_kz = s->fused && s->pml && s->kzs > 0 ? s->kzs : s->kz;
     info->shell_concurrent = s->fused && s->ks && s->kconc ? 1 : 0;
+    info->interior_rows = s->fused ? s->rows : 0;
     info->shell_pipelined = s->fused && shell_pipe_ok(s) && (s->tfH.n > 0 || s->use_aux || s->np > 0) &&
                             tf_merge_ok(s) ? 1 : 0;
     if (s->fused && s->pml) {
@@ -1336,7 +1344,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
     member_enter(s);
     using Key = std::tuple<int, int, int, int, int, int, int>;
     static std::mutex mu;
-    static std::map<Key, std::array<int, 6>> cache;   // (kz, nw, fused, vec, shell kz, shell stream)
+    static std::map<Key, std::array<int, 7>> cache;   // (kz, nw, fused, vec, shell kz, shell stream, rows)
     const Key key{s->N[0], s->N[1], s->N[2], s->fp16, s->pml, s->planar, 0};
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -1347,6 +1355,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
             s->vec = it->second[3];
             s->kzs = it->second[4];
             s->kconc = it->second[5];
+            s->rows = it->second[6];
             if (it->second[2] && fused_ok(s)) {
                 lx_status st = ensure_fused_buffers(s);
                 if (st) return st;
@@ -1360,14 +1369,16 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
             return LX_OK;
         }
     }
-    // LX_TUNE=fused,nw,vec,kz,shell_concurrent pins the result without timing
-    // (a profiling process repeats another process's choice)
+    // LX_TUNE=fused,nw,vec,kz,shell_concurrent[,rows] pins the result without
+    // timing (a profiling process repeats another process's choice)
     if (const char *pin = std::getenv("LX_TUNE"); pin && !s->planar) {
-        int f = 0, nw = 0, vec = 0, kz = 0, conc = 0;
-        if (std::sscanf(pin, "%d,%d,%d,%d,%d", &f, &nw, &vec, &kz, &conc) != 5 || kz < 1 || (nw != 4 && nw != 8) ||
-            !vec_ok(s, vec))
-            return fail(LX_EINVAL, "LX_TUNE: expected fused,nw,vec,kz,shell_concurrent valid for this grid, got '%s'",
-                        pin);
+        int f = 0, nw = 0, vec = 0, kz = 0, conc = 0, rows = 0;
+        const int nf = std::sscanf(pin, "%d,%d,%d,%d,%d,%d", &f, &nw, &vec, &kz, &conc, &rows);
+        if ((nf != 5 && nf != 6) || kz < 1 || (nw != 4 && nw != 8) || !vec_ok(s, vec) ||
+            (rows != 0 && rows != 3 && rows != 7))
+            return fail(LX_EINVAL,
+                        "LX_TUNE: expected fused,nw,vec,kz,shell_concurrent[,rows] valid for this grid, got '%s'", pin);
+        s->rows = rows;
         s->kz = kz;
         s->nw = nw;
         s->vec = vec;
@@ -1458,10 +1469,14 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         const int can_fuse = fused_ok(s) && ensure_fused_buffers(s) == LX_OK;
         s->kzs = 0;
         s->kconc = 1;
-        struct Cand { int nw, vec, kz; float ms; int conc; };
+        // rows: the interior launch with 3 or 7 rows per block sharing their
+        // neighbours through LDS (0: one unit per wavefront)
+        struct Cand { int nw, vec, kz; float ms; int conc; int rows; };
         std::vector<Cand> tried;
-        for (Cand c : {Cand{0, 1, 0, 0, 1}, Cand{8, 1, 0, 0, 1}, Cand{4, 1, 0, 0, 1}, Cand{4, 2, 0, 0, 1},
-                       Cand{8, 2, 0, 0, 1}, Cand{4, 4, 0, 0, 1}}) {
+        for (Cand c : {Cand{0, 1, 0, 0, 1, 0}, Cand{8, 1, 0, 0, 1, 0}, Cand{4, 1, 0, 0, 1, 0}, Cand{4, 2, 0, 0, 1, 0},
+                       Cand{8, 2, 0, 0, 1, 0}, Cand{4, 4, 0, 0, 1, 0}, Cand{4, 1, 0, 0, 1, 3}, Cand{4, 1, 0, 0, 1, 7},
+                       Cand{4, 2, 0, 0, 1, 3}, Cand{4, 2, 0, 0, 1, 7}, Cand{4, 4, 0, 0, 1, 3},
+                       Cand{4, 4, 0, 0, 1, 7}}) {
             if (c.nw && !can_fuse) continue;
             if (!vec_ok(s, c.vec)) continue;
             for (int kz : {4, 5, 6, 8, 12, 16, 32, 64}) {
@@ -1469,14 +1484,15 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
                 s->fused = c.nw != 0;
                 if (c.nw) s->nw = c.nw;
                 s->vec = c.vec;
+                s->rows = c.rows;
                 s->kz = kz;
                 lx_status st = time_steps(4, 3, &c.ms);
                 if (st) return st;
                 c.kz = kz;
                 tried.push_back(c);
                 if (verbose)
-                    std::fprintf(stderr, "lx_sim_autotune: %s nw=%d vec=%d kz=%d  %.4f ms per step\n",
-                                 s->fused ? "fused" : "half-step", c.nw, c.vec, kz, c.ms);
+                    std::fprintf(stderr, "lx_sim_autotune: %s nw=%d vec=%d rows=%d kz=%d  %.4f ms per step\n",
+                                 s->fused ? "fused" : "half-step", c.nw, c.vec, c.rows, kz, c.ms);
             }
         }
         std::sort(tried.begin(), tried.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
@@ -1490,13 +1506,15 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
                 s->fused = c.nw != 0;
                 if (c.nw) s->nw = c.nw;
                 s->vec = c.vec;
+                s->rows = c.rows;
                 s->kz = c.kz;
                 s->kconc = conc;
                 lx_status st = time_steps(12, 3, &c.ms, s->graphs != 0);
                 if (st) return st;
                 if (verbose)
-                    std::fprintf(stderr, "lx_sim_autotune: finalist %s nw=%d vec=%d kz=%d shell %s  %.4f ms per step\n",
-                                 s->fused ? "fused" : "half-step", c.nw, c.vec, c.kz,
+                    std::fprintf(stderr,
+                                 "lx_sim_autotune: finalist %s nw=%d vec=%d rows=%d kz=%d shell %s  %.4f ms per step\n",
+                                 s->fused ? "fused" : "half-step", c.nw, c.vec, c.rows, c.kz,
                                  conc ? "beside" : "behind", c.ms);
                 if (c.ms < best.ms) best = c;
             }
@@ -1504,6 +1522,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         s->fused = best.nw != 0;
         if (best.nw) s->nw = best.nw;
         s->vec = best.vec;
+        s->rows = best.rows;
         s->kz = best.kz;
         s->kconc = best.conc;
         s->kzs = 0;   // the shell takes the interior's depth: a depth of its own was never faster when measured
@@ -1517,7 +1536,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
     if (st) return st;
     {
         std::lock_guard<std::mutex> lk(mu);
-        cache[key] = {s->kz, s->nw, s->fused, s->vec, s->kzs, s->kconc};
+        cache[key] = {s->kz, s->nw, s->fused, s->vec, s->kzs, s->kconc, s->rows};
     }
     if (kz_out) *kz_out = s->kz;
     return LX_OK;

@@ -128,6 +128,8 @@ struct Grid {
     int *counter;             // counter[0] = steps completed before this launch sequence
     T *Fn;                    // fused 3D step: fields written (F is read)
     float *psin;              // fused 3D step: psi written (psi is read)
+    int rows;                 // fused 3D step: rows per block of the interior launch sharing their
+                              // neighbours through LDS (0: one unit per wavefront, no sharing)
 };
 
 struct Launch {

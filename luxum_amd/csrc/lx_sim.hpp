@@ -57,6 +57,8 @@ struct lx_sim {
     hipEvent_t kfork = nullptr, kjoin = nullptr;
     int kconc = 1;              // use ks (when it exists); autotuned
     int tf_merge = 1;           // fused step: one TF/SF launch between steps (LX_TFSF_MERGE=0: two per step)
+    int rows = 0;               // fused step: rows per block of the interior launch sharing neighbours
+                                // through LDS (3 or 7; 0: one unit per wavefront); autotuned
     int shell_pipe = 1;         // fused step: shell launch pipelined past the TF/SF launch (LX_SHELL_PIPE=0: not)
     std::vector<std::pair<void *, size_t>> allocs;   // device blocks and their sizes
     int64_t bytes = 0;

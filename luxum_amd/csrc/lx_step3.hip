@@ -553,6 +553,355 @@ __device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, 
     }
 }
 
+// Interior launch with R rows per block sharing their neighbours through
+// LDS (PM = 0 or 1: no y- or z-slab psi).  A block holds R consecutive rows
+// of one x chunk and one z block, a wavefront per row, plus a halo wavefront
+// for row j0-1 below the block.  Per plane:
+//   1. every row wavefront publishes E(k) of its row (Ex, Ez), which the
+//      row below needs for its Hx and Hz; barrier;
+//   2. each row wavefront computes H_new of its row (row j+1's Ex, Ez from
+//      LDS, from memory for the block's top row) and publishes Hx, Hz; the
+//      halo wavefront computes Hx, Hz of row j0-1 with the same curl2 calls
+//      as the row's own unit and publishes them; barrier;
+//   3. each row wavefront computes E_new of its row (row j-1's new Hx, Hz
+//      from LDS).
+// Rows j-1 and j+1 are then neither read from memory nor recomputed inside a
+// block; every value goes through the same arithmetic as in step3_unit, so
+// the fields are bit-identical.  Waves of rows past the box (the last group)
+// only take part in the barriers.
+template <typename T, int PM, int V, int R>
+__device__ __forceinline__ void step3_rows(const Grid<T> &g, const UnitMap &um, int blk, const float *cbs, float *xe,
+                                           float *xh) {
+    static_assert(PM == 0 || PM == 1, "interior launch only");
+    constexpr bool PML = PM != 0;
+    using F = vf<V>;
+    constexpr int SL = 64 * V;                     // floats per slot array
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // 0..R-1 rows, R halo
+    const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
+    const int nch = (nx + 64 * V - 1) / (64 * V);
+    const int ch = blk % nch;
+    const int ng = (um.jhi - um.jlo + R - 1) / R;
+    const int rest = blk / nch;
+    const int j0 = um.jlo + (rest % ng) * R;
+    const int k0 = (um.tlo + rest / ng) * um.kz;
+    const int k1 = min(k0 + um.kz, nz);
+    const int x0 = ch * 64 * V;
+    const long long plane = g.plane, fs = g.fstride;
+    const int tn = g.tn;
+    const float *tab = g.tab;
+    const bool halo = w == R;
+    const int jr = halo ? j0 - 1 : j0 + w;         // this wavefront's row
+    const bool act = halo ? jr >= 0 : jr < um.jhi;
+    // row j+1 comes from LDS when its wavefront is active, else from memory
+    const bool up_lds = !halo && w + 1 < R && jr + 1 < um.jhi;
+    const int jp = min(jr + 1, ny - 1);
+    const bool rowx = jr < ny - 1;                 // Hx, Hz, Ey rows
+    const bool jin = jr >= 1 && jr <= ny - 2;      // Ex, Ez rows
+    const float khy = tab[(1 * 6 + TK_KH) * tn + min(max(jr, 0), ny - 2)];
+    const float key = tab[(1 * 6 + TK_KE) * tn + min(max(jr, 0), ny - 1)];
+    const F vkhy = vs<V>(khy), vkey = vs<V>(key);
+    const F zero = vs<V>(0.f);
+
+    const int i0 = x0 + lane * V;
+    const bool nxt = x0 + 64 * V < nx;
+    const bool lft = x0 > 0;
+    const unsigned m_hx = vmask<V>(i0, 0, nx - 1), m_in = vmask<V>(i0, 0, nx - 2);
+    const unsigned m_ex = jin ? vmask<V>(i0, 0, nx - 2) : 0u;
+    const unsigned m_ey = rowx ? vmask<V>(i0, 1, nx - 2) : 0u;
+    const unsigned m_ez = jin ? vmask<V>(i0, 1, nx - 2) : 0u;
+    const F khx = ldtab<V>(tab + (0 * 6 + TK_KH) * tn, i0), kex = ldtab<V>(tab + (0 * 6 + TK_KE) * tn, i0);
+    const bool sxw = (PM & 1) && g.pml && (x0 < g.slo[0] || x0 + 64 * V > g.shi[0]);
+    const bool sx = sxw && (i0 < g.slo[0] || i0 >= g.shi[0]);
+    const int xs = sx ? (i0 < g.slo[0] ? i0 : g.slo[0] + (i0 - g.shi[0])) : 0;
+    F bhx = zero, chx = zero, bex = zero, cex = zero;
+    if (sx) {
+        bhx = ldtab<V>(tab + (0 * 6 + TK_BH) * tn, i0);
+        chx = ldtab<V>(tab + (0 * 6 + TK_CH) * tn, i0);
+        bex = ldtab<V>(tab + (0 * 6 + TK_BE) * tn, i0);
+        cex = ldtab<V>(tab + (0 * 6 + TK_CE) * tn, i0);
+    }
+    const int xl = lft ? x0 - 1 : 0;
+    const float khxl = tab[(0 * 6 + TK_KH) * tn + xl];
+    const bool sxl = (PM & 1) && lft && (xl < g.slo[0] || xl >= g.shi[0]);
+    const int xsl = sxl ? (xl < g.slo[0] ? xl : g.slo[0] + (xl - g.shi[0])) : 0;
+    const float bhxl = sxl ? tab[(0 * 6 + TK_BH) * tn + xl] : 0.f, chxl = sxl ? tab[(0 * 6 + TK_CH) * tn + xl] : 0.f;
+    constexpr unsigned S = sizeof(T);
+    const int jrr = max(jr, 0);
+    const unsigned rj = (unsigned)(jrr * px + i0) * S, rjp = (unsigned)(jp * px + i0) * S;
+    const unsigned ri = (unsigned)i0 * 4u, rx = (unsigned)xs * 4u;
+    const unsigned re = (unsigned)(jrr * px + i0);
+    const unsigned bn = rj + (unsigned)(64 * V - lane * V) * S;
+    const unsigned bl = rj - (unsigned)(lane * V + 1) * S, blp = rjp - (unsigned)(lane * V + 1) * S;
+    const unsigned ril = ri - (unsigned)(lane * V + 1) * 4u;
+    const unsigned rxl = (unsigned)xsl * 4u;
+    (void)ril;
+    // LDS slots: xe[w] holds row j0+w's Ex, Ez of the plane; xh[w+1] holds row
+    // j0+w's new Hx, Hz, xh[0] the halo row's
+    auto put = [&](float *base, int slot, const F &a, const F &b) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            base[(slot * 2 + 0) * SL + e * 64 + lane] = a.v[e];
+            base[(slot * 2 + 1) * SL + e * 64 + lane] = b.v[e];
+        }
+    };
+    auto get = [&](const float *base, int slot, F &a, F &b) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            a.v[e] = base[(slot * 2 + 0) * SL + e * 64 + lane];
+            b.v[e] = base[(slot * 2 + 1) * SL + e * 64 + lane];
+        }
+    };
+
+    if (halo) {
+        // row jr = j0-1: Hx and Hz of each plane, the values row j0's unit
+        // recomputes in step3_unit (its hxj, hzj), from Ey of the row (z queue)
+        F eym0 = zero;
+        if (act) eym0 = ldv<T, V>(g.F + (long long)k0 * plane + fs, rj);
+        for (int k = k0; k < k1; ++k) {
+            const bool kin = k < nz - 1;
+            const long long kp = uo(k * plane);
+            const T *Fk = g.F + kp;
+            const long long kn = kin ? plane : 0;
+            F eym1 = zero, hxj = zero, hzj = zero, ezm = zero, exm = zero, qx0 = zero;
+            float eymN = 0.f;
+            if (act) {
+                eym1 = ldv<T, V>(Fk + kn + fs, rj);
+                hxj = ldv<T, V>(Fk + 3 * fs, rj);
+                hzj = ldv<T, V>(Fk + 5 * fs, rj);
+                ezm = ldv<T, V>(Fk + 2 * fs, rj);
+                exm = ldv<T, V>(Fk, rj);
+                if (nxt) eymN = ldb(Fk + fs, bn);
+                if (PM == 1 ? sxw : false)
+                    qx0 = ldv<float, V>(psi_base(g, 0, 0) + uo(((long long)k * ny + jrr) * g.sw[0]), rx);
+            }
+            const float khz = tab[(2 * 6 + TK_KH) * tn + min(k, nz - 2)];
+            const F vkhz = vs<V>(khz);
+            __syncthreads();                       // row j0's E(k) published
+            F ex0, ez0;
+            get(xe, 0, ex0, ez0);
+            if (act) {
+                const F eymi = xnext<V>(eym0, eymN, lane);
+                F qy0 = zero, qy1 = zero, qz1 = zero;
+                if (kin) {   // Hx(j0-1): d1 = Ez(j0) - Ez(j0-1), d2 = dEy(j0-1)/dz
+                    const F acc = curl2<PML, V>(ez0 - ezm, vkhy, false, qy0, zero, zero, eym1 - eym0, vkhz, false, qz1,
+                                                zero, zero);
+                    hxj = vsel<V>(m_hx, hxj - acc, hxj);
+                }
+                {            // Hz(j0-1): d1 = dEy(j0-1)/dx, d2 = Ex(j0) - Ex(j0-1)
+                    const F acc = curl2<PML, V>(eymi - eym0, khx, sx, qx0, bhx, chx, ex0 - exm, vkhy, false, qy1,
+                                                zero, zero);
+                    hzj = vsel<V>(m_in, hzj - acc, hzj);
+                }
+            }
+            put(xh, 0, hxj, hzj);
+            __syncthreads();                       // new H of every row published
+            eym0 = eym1;
+        }
+        return;
+    }
+
+    F ex0 = zero, ey0 = zero, ez0 = zero;
+    float exl0 = 0.f;
+    F hxm = zero, hym = zero;
+    if (act) {
+        if (k0 > 0) {
+            const int kh = k0 - 1;
+            const T *Fk = g.F + kh * plane;
+            const F exh = ldv<T, V>(Fk, rj), eyh = ldv<T, V>(Fk + fs, rj), ezh = ldv<T, V>(Fk + 2 * fs, rj);
+            ex0 = ldv<T, V>(Fk + plane, rj);
+            ey0 = ldv<T, V>(Fk + plane + fs, rj);
+            ez0 = ldv<T, V>(Fk + plane + 2 * fs, rj);
+            if (lft) exl0 = ldb(Fk + plane, bl);
+            const F ezph = ldv<T, V>(Fk + 2 * fs, rjp);
+            const float ezNh = nxt ? ldb(Fk + 2 * fs, bn) : 0.f;
+            const F hxh = ldv<T, V>(Fk + 3 * fs, rj), hyh = ldv<T, V>(Fk + 4 * fs, rj);
+            F py0 = zero, pz1 = zero, pz0 = zero, px1 = zero;
+            if (sx) px1 = ldv<float, V>(psi_base(g, 0, 1) + ((long long)kh * ny + jr) * g.sw[0], rx);
+            const float khz = tab[(2 * 6 + TK_KH) * tn + kh];
+            const F vkhz = vs<V>(khz);
+            const F ezi = xnext<V>(ezh, ezNh, lane);
+            hxm = hxh;
+            if (rowx) {
+                const F acc = curl2<PML, V>(ezph - ezh, vkhy, false, py0, zero, zero, ey0 - eyh, vkhz, false, pz1, zero,
+                                            zero);
+                hxm = vsel<V>(m_hx, hxh - acc, hxh);
+            }
+            {
+                const F acc = curl2<PML, V>(ex0 - exh, vkhz, false, pz0, zero, zero, ezi - ezh, khx, sx, px1, bhx, chx);
+                hym = vsel<V>(m_in, hyh - acc, hyh);
+            }
+        } else {
+            ex0 = ldv<T, V>(g.F, rj);
+            ey0 = ldv<T, V>(g.F + fs, rj);
+            ez0 = ldv<T, V>(g.F + 2 * fs, rj);
+            if (lft) exl0 = ldb(g.F, bl);
+        }
+    }
+
+    for (int k = k0; k < k1; ++k) {
+        const bool kin = k < nz - 1;
+        const bool kine = k >= 1 && kin;
+        const long long kp = uo(k * plane);
+        const T *Fk = g.F + kp;
+        T *Fnk = g.Fn + kp;
+        const long long kn = kin ? plane : 0;
+        F ex1 = zero, ey1 = zero, ez1 = zero, ezp = zero, exq = zero, hx = zero, hy = zero, hz = zero;
+        unsigned id[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) id[e] = 0;
+        float ezN = 0.f, eyN = 0.f, exl1 = 0.f, hyl = 0.f, hzl = 0.f, ezl = 0.f, eyl = 0.f, exlp = 0.f;
+        F px0 = zero, px1 = zero, ex2 = zero, ex3 = zero;
+        float lx0 = 0.f, lx1 = 0.f;
+        const float *PX = nullptr;
+        float *QX = nullptr;
+        const long long cx = g.scells[0];
+        if (act) {
+            ex1 = ldv<T, V>(Fk + kn, rj);
+            ey1 = ldv<T, V>(Fk + kn + fs, rj);
+            ez1 = ldv<T, V>(Fk + kn + 2 * fs, rj);
+            if (!up_lds) {
+                ezp = ldv<T, V>(Fk + 2 * fs, rjp);
+                exq = ldv<T, V>(Fk, rjp);
+            }
+            hx = ldv<T, V>(Fk + 3 * fs, rj);
+            hy = ldv<T, V>(Fk + 4 * fs, rj);
+            hz = ldv<T, V>(Fk + 5 * fs, rj);
+            if constexpr (V == 1) {
+                id[0] = g.mat[kp + re];
+            } else if constexpr (V == 2) {
+                const uchar2 u = *reinterpret_cast<const uchar2 *>(g.mat + kp + re);
+                id[0] = u.x;
+                id[1] = u.y;
+            } else {
+                const unsigned u = *reinterpret_cast<const unsigned *>(g.mat + kp + re);
+#pragma unroll
+                for (int e = 0; e < V; ++e) id[e] = (u >> (8 * e)) & 0xffu;
+            }
+            if (nxt) {
+                ezN = ldb(Fk + 2 * fs, bn);
+                eyN = ldb(Fk + fs, bn);
+            }
+            if (lft) {
+                exl1 = ldb(Fk + kn, bl);
+                hyl = ldb(Fk + 4 * fs, bl);
+                hzl = ldb(Fk + 5 * fs, bl);
+                ezl = ldb(Fk + 2 * fs, bl);
+                eyl = ldb(Fk + fs, bl);
+                exlp = ldb(Fk, blp);
+            }
+            if (sxw || sxl) {
+                const long long xb = uo(((long long)k * ny + jr) * g.sw[0]);
+                PX = psi_base(g, 0, 0) + xb;
+                QX = psi_new(g, 0, 0) + xb;
+                if (sxl) {
+                    lx0 = ldb(PX, rxl);
+                    lx1 = ldb(PX + cx, rxl);
+                }
+            }
+            if (PM == 1 && sxw) {
+                px0 = ldv<float, V>(PX, rx);
+                px1 = ldv<float, V>(PX + cx, rx);
+                ex2 = ldv<float, V>(PX + 2 * cx, rx);
+                ex3 = ldv<float, V>(PX + 3 * cx, rx);
+            }
+        }
+        const float khz = tab[(2 * 6 + TK_KH) * tn + min(k, nz - 2)];
+        const float kez = tab[(2 * 6 + TK_KE) * tn + k];
+        const F vkhz = vs<V>(khz), vkez = vs<V>(kez);
+        put(xe, w, ex0, ez0);
+        __syncthreads();                           // E(k) of every row published
+        if (up_lds) get(xe, w + 1, exq, ezp);
+        const F ezi = xnext<V>(ez0, ezN, lane), eyi = xnext<V>(ey0, eyN, lane);
+        F py0 = zero, py1 = zero, pz0 = zero, pz1 = zero;
+        // ---------------- H half step, row j
+        if (rowx && kin) {   // Hx: d1 = dEz/dy, d2 = dEy/dz
+            const F acc = curl2<PML, V>(ezp - ez0, vkhy, false, py0, zero, zero, ey1 - ey0, vkhz, false, pz1, zero, zero);
+            hx = vsel<V>(m_hx, hx - acc, hx);
+        }
+        if (kin) {           // Hy: d1 = dEx/dz, d2 = dEz/dx
+            const F acc = curl2<PML, V>(ex1 - ex0, vkhz, false, pz0, zero, zero, ezi - ez0, khx, sx, px1, bhx, chx);
+            hy = vsel<V>(m_in, hy - acc, hy);
+        }
+        if (rowx) {          // Hz: d1 = dEy/dx, d2 = dEx/dy
+            const F acc = curl2<PML, V>(eyi - ey0, khx, sx, px0, bhx, chx, exq - ex0, vkhy, false, py1, zero, zero);
+            hz = vsel<V>(m_in, hz - acc, hz);
+        }
+        // ---------------- Hy, Hz of cell x0-1 (only the chunk's first Ey, Ez use them)
+        if (lft) {
+            float lz0 = 0.f, ly1 = 0.f;
+            if (kin) {
+                const float acc = curl2<PML>(exl1 - exl0, khz, false, lz0, 0.f, 0.f, lane0(ez0.v[0]) - ezl, khxl, sxl,
+                                             lx1, bhxl, chxl);
+                hyl = hyl - acc;
+            }
+            if (rowx) {
+                const float acc = curl2<PML>(lane0(ey0.v[0]) - eyl, khxl, sxl, lx0, bhxl, chxl, exlp - exl0, khy,
+                                             false, ly1, 0.f, 0.f);
+                hzl = hzl - acc;
+            }
+        }
+        if (act) {
+            stv<T, V>(Fnk + 3 * fs, rj, hx);
+            stv<T, V>(Fnk + 4 * fs, rj, hy);
+            stv<T, V>(Fnk + 5 * fs, rj, hz);
+            if (PML) {
+                if (sx && kin) stv<float, V>(QX + cx, rx, px1);
+                if (sx && rowx) stv<float, V>(QX, rx, px0);
+            }
+        }
+        put(xh, w + 1, hx, hz);
+        __syncthreads();                           // new H of every row published
+        F hxj, hzj;
+        get(xh, w, hxj, hzj);
+        // ---------------- E half step, row j
+        if (act) {
+            const F hyp = xprev<V>(hy, hyl, lane), hzp = xprev<V>(hz, hzl, lane);
+            F ex = ex0, ey = ey0, ez = ez0;
+            F ey2 = zero, ez3 = zero, ez2 = zero, ey3 = zero;
+            if (kine) {
+                {            // Ex: d1 = dHz/dy, d2 = dHy/dz
+                    const F acc = curl2<PML, V>(hz - hzj, vkey, false, ey2, zero, zero, hy - hym, vkez, false, ez3, zero,
+                                                zero);
+                    ex = vsel<V>(m_ex, ex0 + cbv<V>(cbs, id) * acc, ex0);
+                }
+                {            // Ey: d1 = dHx/dz, d2 = dHz/dx
+                    const F acc = curl2<PML, V>(hx - hxm, vkez, false, ez2, zero, zero, hz - hzp, kex, sx, ex3, bex,
+                                                cex);
+                    ey = vsel<V>(m_ey, ey0 + cbv<V>(cbs + 256, id) * acc, ey0);
+                }
+            }
+            if (kin) {       // Ez: d1 = dHy/dx, d2 = dHx/dy
+                const F acc = curl2<PML, V>(hy - hyp, kex, sx, ex2, bex, cex, hx - hxj, vkey, false, ey3, zero, zero);
+                ez = vsel<V>(m_ez, ez0 + cbv<V>(cbs + 512, id) * acc, ez0);
+            }
+            stv<T, V>(Fnk, rj, ex);
+            stv<T, V>(Fnk + fs, rj, ey);
+            stv<T, V>(Fnk + 2 * fs, rj, ez);
+            if (PML) {
+                if (sx && kine && rowx) stv<float, V>(QX + 3 * cx, rx, ex3);
+                if (sx && kin && jin) stv<float, V>(QX + 2 * cx, rx, ex2);
+            }
+        }
+        hxm = hx;
+        hym = hy;
+        ex0 = ex1;
+        ey0 = ey1;
+        ez0 = ez1;
+        exl0 = exl1;
+    }
+}
+
+template <typename T, int PM, int R, int V>
+__global__ __launch_bounds__(64 * (R + 1)) void k_step3r(Grid<T> g, UnitMap um, int nblocks) {
+    __shared__ float cbs[3 * 256];
+    __shared__ float xe[R * 2 * 64 * V];
+    __shared__ float xh[(R + 1) * 2 * 64 * V];
+    for (int q = threadIdx.x; q < 3 * 256; q += 64 * (R + 1)) cbs[q] = g.cb[q];
+    __syncthreads();
+    step3_rows<T, PM, V, R>(g, um, xcd_remap(blockIdx.x, nblocks), cbs, xe, xh);
+}
+
 template <typename T, int PM, int NW, int V>
 __global__ __launch_bounds__(64 * NW) void k_step3(Grid<T> g, UnitMap um, int nunits) {
     __shared__ float cbs[3 * 256];
@@ -656,11 +1005,29 @@ void step3_unit_maps(int nx, int ny, int nz, int kz, int kzs, int pml, const int
     nunits[1] = nch * (nro * sh.ntz + nr * (sh.nlo + nhi));
 }
 
+// Interior launch with rows sharing neighbours: blocks of R rows (+ the
+// halo wavefront) of one x chunk and one z block.
+template <typename T, int PM, int V>
+static void launch_rows(const Grid<T> &g, const UnitMap &um, hipStream_t s) {
+    const int nch = (g.nx + 64 * V - 1) / (64 * V);
+    const int R = g.rows;
+    const int ng = (um.jhi - um.jlo + R - 1) / R;
+    const int nb = nch * ng * (um.thi - um.tlo);
+    if (nb <= 0) return;
+    if (R == 7) hipLaunchKernelGGL((k_step3r<T, PM, 7, V>), dim3(nb), dim3(512), 0, s, g, um, nb);
+    else hipLaunchKernelGGL((k_step3r<T, PM, 3, V>), dim3(nb), dim3(256), 0, s, g, um, nb);
+}
+
 template <typename T, int V>
 static void launch_parts(const Grid<T> &g, int nw, int part, hipStream_t s) {
     UnitMap um[2];
     int nu[2];
     step3_unit_maps(g.nx, g.ny, g.nz, g.kz, g.kzs, g.pml, g.slo, g.shi, V, um, nu);
+    if (part == 0 && g.rows > 0 && nu[0] > 0) {
+        if (g.pml) launch_rows<T, 1, V>(g, um[0], s);
+        else launch_rows<T, 0, V>(g, um[0], s);
+        return;
+    }
     if (!g.pml) {
         if (part == 0) launch_units<T, 0, V>(g, nw, um[0], nu[0], s);
     } else if (part == 0) {

@@ -595,3 +595,53 @@ def test_fused_shell_pipelined(gpu_ctx, monkeypatch):
             assert np.array_equal(s.probes(), ref.probes())
             s.close()
         ref.close()
+
+
+def test_fused_rows_sharing_neighbours(gpu_ctx, monkeypatch):
+    """The interior launch with 3 or 7 rows per block sharing their
+    neighbour rows through LDS (rows j-1 and j+1 neither read nor recomputed
+    inside a block; a halo wavefront computes row j0-1) writes the same
+    fields and probes as one unit per wavefront, bit for bit: FP32 one and
+    two cells per lane, FP16 four, a CPML grid whose interior rows are not a
+    multiple of the group (a partial last group), a grid without CPML (first
+    group at row 0), odd step counts, graph replay and eager launches."""
+    from luxum_amd.sim import Simulation
+
+    def run(w, mode, rows, graphs):
+        monkeypatch.setenv("LX_FUSED", mode)
+        monkeypatch.setenv("LX_ROWS", str(rows))
+        monkeypatch.setenv("LX_NO_GRAPH", "0" if graphs else "1")
+        s = Simulation(w.p, gpu_ctx)
+        s.set_materials(w.ids, w.eps_r)
+        if w.probes is not None:
+            s.set_probes(w.probes)
+        assert s.info().fused_step == 1
+        assert s.info().interior_rows == rows
+        s.step(37)
+        s.step(8)
+        s.sync()
+        return s
+
+    cases = [("fp32 v1", configs.box3d(200, 70, 60, pml=10, margin=5), "1"),
+             ("fp32 v2", configs.box3d(200, 70, 60, pml=10, margin=5), "2"),
+             ("fp16 v4", configs.box3d(200, 70, 60, pml=10, margin=5, precision=LX_FP16), "4"),
+             ("fp32 v1 cube", configs.config1(n=64, steps=45, pml=8, margin=6), "1")]
+    p = default_params(nx=90, ny=45, nz=50, use_pw=1, incident=LX_INC_ANALYTIC, off_pw=90.0,
+                       pw_sigma=(20 * 5e-8) ** 2, tf_lo=(8, 7, 9), tf_hi=(80, 36, 40), courant=0.5)
+    p.k_pw[:] = (0.6, 0.48, 0.64)
+    p.e_pw[:] = (0.6246950475544243, -0.7808688094430304, 0.0)
+    cases.append(("fp32 v1 no CPML", configs.Workload("plain", p, np.zeros((90, 45, 50), np.uint8), np.array([1.0]),
+                                                       np.array([[LX_EX, 40, 20, 25], [LX_HY, 41, 20, 25]],
+                                                                dtype=np.int32)), "1"))
+    for name, w, mode in cases:
+        ref = run(w, mode, 0, True)
+        for rows in (3, 7):
+            for graphs in ((True, False) if rows == 7 else (True,)):
+                say(f"case {name}: rows {rows}, graphs {graphs}")
+                s = run(w, mode, rows, graphs)
+                for c in w.p.components():
+                    assert np.array_equal(s.field(c), ref.field(c)), f"{name} rows {rows} graphs {graphs} comp {c}"
+                if w.probes is not None:
+                    assert np.array_equal(s.probes(), ref.probes())
+                s.close()
+        ref.close()
