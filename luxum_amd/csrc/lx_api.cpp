@@ -975,8 +975,11 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
                 s->kz = 8;
                 s->nw = 4;
                 s->vec = 1;
-                // LX_ROWS: rows per block of the interior launch sharing neighbours through LDS
-                // (3 or 7; 0: one unit per wavefront)
+                // rows per block of the interior launch sharing neighbours through LDS: 7 for
+                // FP32 storage (faster at every depth measured), 0 (one unit per wavefront) for
+                // FP16, whose four-cells units hold too many registers for 8-wavefront blocks;
+                // LX_ROWS overrides (3, 7 or 0)
+                s->rows = s->fp16 ? 0 : 7;
                 if (const char *rw = std::getenv("LX_ROWS"); rw && *rw) {
                     const int r = std::atoi(rw);
                     s->rows = (r == 3 || r == 7) ? r : 0;
