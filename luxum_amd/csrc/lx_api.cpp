@@ -613,13 +613,12 @@ void enqueue_seq_t(lx_sim *s, int n, std::vector<hipEvent_t> *ev) {
 }
 
 // after an enqueued sequence of n fused steps: the current state is in the
-// other buffer when n is odd
+// other buffer when n is odd (the graphs of both buffer states stay valid)
 void settle_parity(lx_sim *s, int n) {
     if (!s->fused || !(n & 1)) return;
     std::swap(s->F, s->F_alt);
     std::swap(s->psi, s->psi_alt);
     s->gen++;
-    drop_graph(s);
 }
 
 void enqueue_seq(lx_sim *s, int n, std::vector<hipEvent_t> *ev = nullptr) {
@@ -628,34 +627,76 @@ void enqueue_seq(lx_sim *s, int n, std::vector<hipEvent_t> *ev = nullptr) {
     else enqueue_seq_t<float>(s, n, ev);
 }
 
+// Captured step sequences: 32, 16, 8 and 4 steps (even counts keep the
+// buffer state); a call of n steps replays the largest that fit
+constexpr int kGraphLens = 4;
+constexpr int kGraphSteps[kGraphLens] = {32, 16, 8, 4};
+
 void drop_graph(lx_sim *s) {
-    if (s->gexec) {
-        (void)hipGraphExecDestroy(s->gexec);
-        s->gexec = nullptr;
-    }
+    for (int i = 0; i < kGraphLens; ++i)
+        for (int q = 0; q < 2; ++q) {
+            if (s->gexec[i][q]) (void)hipGraphExecDestroy(s->gexec[i][q]);
+            s->gexec[i][q] = nullptr;
+            s->gF[i][q] = nullptr;
+        }
 }
 
-lx_status ensure_graph(lx_sim *s, int K) {
-    if (s->gexec && s->gK == K) return LX_OK;
-    drop_graph(s);
+// The graph of kGraphSteps[i] steps for the current buffer state, captured
+// on first use; nullptr when graphs are off or capture failed.
+hipGraphExec_t ensure_graph(lx_sim *s, int i, lx_status *st) {
+    *st = LX_OK;
+    if (!s->graphs) return nullptr;
+    int slot = -1;
+    for (int q = 0; q < 2; ++q) {
+        if (s->gexec[i][q] && s->gF[i][q] == s->F) return s->gexec[i][q];
+        if (!s->gexec[i][q] && slot < 0) slot = q;
+    }
+    if (slot < 0) {   // both slots hold other buffer states: the buffers changed
+        drop_graph(s);
+        slot = 0;
+    }
     hipGraph_t graph = nullptr;
-    HIP_TRY(hipStreamBeginCapture(s->cs, hipStreamCaptureModeThreadLocal));
-    enqueue_seq(s, K);
-    hipError_t e = hipStreamEndCapture(s->cs, &graph);
+    hipError_t e = hipStreamBeginCapture(s->cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        *st = fail(LX_EDEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    enqueue_seq(s, kGraphSteps[i]);
+    e = hipStreamEndCapture(s->cs, &graph);
     if (e != hipSuccess || !graph) {
         (void)hipGetLastError();
         s->graphs = 0;
-        return LX_OK;
+        return nullptr;
     }
-    e = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+    hipGraphExec_t gx = nullptr;
+    e = hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        s->gexec = nullptr;
         s->graphs = 0;
-        return LX_OK;
+        return nullptr;
     }
-    s->gK = K;
+    s->gexec[i][slot] = gx;
+    s->gF[i][slot] = s->F;
+    return gx;
+}
+
+// Capture every step-sequence graph for both buffer states of the fused
+// step, so that no later lx_sim_step call captures (after autotuning: the
+// configuration is final).
+lx_status prepare_graphs(lx_sim *s) {
+    if (!s->graphs) return LX_OK;
+    for (int side = 0; side < (s->fused ? 2 : 1); ++side) {
+        for (int i = 0; i < kGraphLens; ++i) {
+            lx_status st;
+            ensure_graph(s, i, &st);
+            if (st) return st;
+        }
+        if (s->fused) {   // the other buffer state
+            std::swap(s->F, s->F_alt);
+            std::swap(s->psi, s->psi_alt);
+        }
+    }
     return LX_OK;
 }
 
@@ -1126,13 +1167,14 @@ lx_status lx_sim_step(lx_sim *s, int32_t nsteps) {
     const int chunk = s->p.monitor_every > 0 ? s->p.monitor_every : nsteps;
     while (left > 0) {
         const int n = std::min(left, chunk);
-        const int K = 32;
         int done = 0;
-        if (s->graphs && n >= K) {
-            if ((st = ensure_graph(s, K))) return st;
-            if (s->gexec) {
-                for (; done + K <= n; done += K) HIP_TRY(hipGraphLaunch(s->gexec, s->cs));
-            }
+        for (int i = 0; i < kGraphLens; ++i) {
+            const int K = kGraphSteps[i];
+            if (n - done < K) continue;
+            hipGraphExec_t gx = ensure_graph(s, i, &st);
+            if (st) return st;
+            if (gx)
+                for (; done + K <= n; done += K) HIP_TRY(hipGraphLaunch(gx, s->cs));
         }
         enqueue_seq(s, n - done);
         settle_parity(s, n - done);
@@ -1369,7 +1411,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
             s->gen++;
             drop_graph(s);
             if (kz_out) *kz_out = s->kz;
-            return LX_OK;
+            return prepare_graphs(s);
         }
     }
     // LX_TUNE=fused,nw,vec,kz,shell_concurrent[,rows] pins the result without
@@ -1398,7 +1440,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         s->gen++;
         drop_graph(s);
         if (kz_out) *kz_out = s->kz;
-        return LX_OK;
+        return prepare_graphs(s);
     }
     if (!s->planar) {
         hipEvent_t e0, e1;
@@ -1465,7 +1507,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         // buffer set is kept only if it wins); the CPML shell takes the
         // interior's z-block depth.  A first pass times every candidate over
         // 4 steps, best of 2 (the shell on the second stream where there is one); the
-        // three fastest are timed again over 12 steps (best of 3) replayed
+        // three fastest are timed again over 24 steps (best of 4) replayed
         // from a graph, as the time loop runs them, which separates
         // near-ties the short pass cannot, each with the shell launch beside
         // and behind the interior launch.
@@ -1512,7 +1554,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
                 s->rows = c.rows;
                 s->kz = c.kz;
                 s->kconc = conc;
-                lx_status st = time_steps(12, 3, &c.ms, s->graphs != 0);
+                lx_status st = time_steps(24, 4, &c.ms, s->graphs != 0);
                 if (st) return st;
                 if (verbose)
                     std::fprintf(stderr,
@@ -1537,6 +1579,7 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
     s->gen++;
     lx_status st = reset_state(s);
     if (st) return st;
+    if ((st = prepare_graphs(s))) return st;
     {
         std::lock_guard<std::mutex> lk(mu);
         cache[key] = {s->kz, s->nw, s->fused, s->vec, s->kzs, s->kconc, s->rows};

@@ -97,9 +97,11 @@ struct lx_sim {
     double *rd_stage = nullptr;
     size_t rd_n = 0;
     double *mon = nullptr;
-    // graphs
-    hipGraphExec_t gexec = nullptr;
-    int gK = 0;
+    // graphs of kGraphSteps[i]-step sequences, one per buffer state of the
+    // fused step (keyed by the field buffer read first, gF), kept while the
+    // fused buffer pair swaps after odd step counts
+    hipGraphExec_t gexec[4][2] = {};
+    const void *gF[4][2] = {};
     int graphs = 1;
     int steps = 0;
     float cb_host[3 * 256];   // Ex, Ey, Ez coefficient tables
