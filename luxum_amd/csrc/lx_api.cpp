@@ -1427,7 +1427,8 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         s->kz = kz;
         s->nw = nw;
         s->vec = vec;
-        s->kzs = 0;
+        const char *sk = std::getenv("LX_SHELL_KZ");   // the shell depth stays pinnable beside the pin
+        s->kzs = sk && *sk ? std::max(0, std::atoi(sk)) : 0;
         s->kconc = conc ? 1 : 0;
         if (f && fused_ok(s)) {
             lx_status st = ensure_fused_buffers(s);

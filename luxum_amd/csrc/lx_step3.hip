@@ -192,14 +192,18 @@ __device__ __forceinline__ vf<V> xprev(const vf<V> &a, float edge, int lane) {
 
 // Split curl of one component, d1*k1 - d2*k2, with the CPML recursion of
 // each term on slab cells (p = b*p + c*d, added for term 1, subtracted for
-// term 2).  Every update of the kernel goes through it, so a recomputed
-// neighbour equals the value its owner stores.
+// term 2).  Every update of the kernels goes through it, so a recomputed
+// neighbour equals the value its owner stores.  The fused multiply-adds are
+// written out and contraction is off here, so that every kernel shape (one
+// unit per wavefront, rows sharing neighbours, the batched step) rounds the
+// same way whatever the surrounding code.
 template <bool PML>
 __device__ __forceinline__ float curl2(float d1, float k1, bool s1, float &p1, float b1, float c1, float d2, float k2,
                                        bool s2, float &p2, float b2, float c2) {
-    float acc = d1 * k1 - d2 * k2;
-    if (PML && s1) { p1 = p1 * b1 + d1 * c1; acc = acc + p1; }
-    if (PML && s2) { p2 = p2 * b2 + d2 * c2; acc = acc - p2; }
+#pragma clang fp contract(off)
+    float acc = __builtin_fmaf(d1, k1, -(d2 * k2));
+    if (PML && s1) { p1 = __builtin_fmaf(p1, b1, d1 * c1); acc = acc + p1; }
+    if (PML && s2) { p2 = __builtin_fmaf(p2, b2, d2 * c2); acc = acc - p2; }
     return acc;
 }
 template <bool PML, int V>
@@ -211,6 +215,15 @@ __device__ __forceinline__ vf<V> curl2(const vf<V> &d1, const vf<V> &k1, bool s1
     for (int e = 0; e < V; ++e)
         r.v[e] = curl2<PML>(d1.v[e], k1.v[e], s1, p1.v[e], b1.v[e], c1.v[e], d2.v[e], k2.v[e], s2, p2.v[e], b2.v[e],
                             c2.v[e]);
+    return r;
+}
+
+// E update e + cb*acc of the lane's V cells (one fused multiply-add each)
+template <int V>
+__device__ __forceinline__ vf<V> eupd(const vf<V> &e0, const vf<V> &cb, const vf<V> &acc) {
+    vf<V> r;
+#pragma unroll
+    for (int e = 0; e < V; ++e) r.v[e] = __builtin_fmaf(cb.v[e], acc.v[e], e0.v[e]);
     return r;
 }
 
@@ -521,16 +534,16 @@ __device__ __forceinline__ void step3_unit(const Grid<T> &g, const UnitMap &um, 
         if (kine) {
             {                // Ex: d1 = dHz/dy, d2 = dHy/dz
                 const F acc = curl2<PML, V>(hz - hzj, vkey, sy, ey2, vbey, vcey, hy - hym, vkez, sz, ez3, vbez, vcez);
-                ex = vsel<V>(m_ex, ex0 + cbv<V>(cbs, id) * acc, ex0);
+                ex = vsel<V>(m_ex, eupd<V>(ex0, cbv<V>(cbs, id), acc), ex0);
             }
             {                // Ey: d1 = dHx/dz, d2 = dHz/dx
                 const F acc = curl2<PML, V>(hx - hxm, vkez, sz, ez2, vbez, vcez, hz - hzp, kex, sx, ex3, bex, cex);
-                ey = vsel<V>(m_ey, ey0 + cbv<V>(cbs + 256, id) * acc, ey0);
+                ey = vsel<V>(m_ey, eupd<V>(ey0, cbv<V>(cbs + 256, id), acc), ey0);
             }
         }
         if (kin) {           // Ez: d1 = dHy/dx, d2 = dHx/dy
             const F acc = curl2<PML, V>(hy - hyp, kex, sx, ex2, bex, cex, hx - hxj, vkey, sy, ey3, vbey, vcey);
-            ez = vsel<V>(m_ez, ez0 + cbv<V>(cbs + 512, id) * acc, ez0);
+            ez = vsel<V>(m_ez, eupd<V>(ez0, cbv<V>(cbs + 512, id), acc), ez0);
         }
         stv<T, V>(Fnk, rj, ex);
         stv<T, V>(Fnk + fs, rj, ey);
@@ -863,17 +876,17 @@ __device__ __forceinline__ void step3_rows(const Grid<T> &g, const UnitMap &um, 
                 {            // Ex: d1 = dHz/dy, d2 = dHy/dz
                     const F acc = curl2<PML, V>(hz - hzj, vkey, false, ey2, zero, zero, hy - hym, vkez, false, ez3, zero,
                                                 zero);
-                    ex = vsel<V>(m_ex, ex0 + cbv<V>(cbs, id) * acc, ex0);
+                    ex = vsel<V>(m_ex, eupd<V>(ex0, cbv<V>(cbs, id), acc), ex0);
                 }
                 {            // Ey: d1 = dHx/dz, d2 = dHz/dx
                     const F acc = curl2<PML, V>(hx - hxm, vkez, false, ez2, zero, zero, hz - hzp, kex, sx, ex3, bex,
                                                 cex);
-                    ey = vsel<V>(m_ey, ey0 + cbv<V>(cbs + 256, id) * acc, ey0);
+                    ey = vsel<V>(m_ey, eupd<V>(ey0, cbv<V>(cbs + 256, id), acc), ey0);
                 }
             }
             if (kin) {       // Ez: d1 = dHy/dx, d2 = dHx/dy
                 const F acc = curl2<PML, V>(hy - hyp, kex, sx, ex2, bex, cex, hx - hxj, vkey, false, ey3, zero, zero);
-                ez = vsel<V>(m_ez, ez0 + cbv<V>(cbs + 512, id) * acc, ez0);
+                ez = vsel<V>(m_ez, eupd<V>(ez0, cbv<V>(cbs + 512, id), acc), ez0);
             }
             stv<T, V>(Fnk, rj, ex);
             stv<T, V>(Fnk + fs, rj, ey);
