@@ -449,7 +449,7 @@ def snapshot_section(args, w, ctx, sim):
     return out
 
 
-def _pmc_pass(counter, args, tune, outdir, child_args=None):
+def _pmc_pass(counter, args, tune, outdir, child_args=None, fp16=None):
     """One `rocprofv3 --pmc <counter>` run of a short bench.py child (same
     workload, the tuned configuration pinned through LX_TUNE, plus a device
     copy of known size; or the child arguments given); returns
@@ -468,10 +468,14 @@ def _pmc_pass(counter, args, tune, outdir, child_args=None):
     cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", outdir, "-o", counter.lower(), "--",
            sys.executable, os.path.join(ROOT, "bench.py")]
     if child_args is None:
+        # the main line only, in the storage precision asked for (the FP16
+        # line of the child would run under the same pin)
+        if fp16 is None:
+            fp16 = args.fp16
         cmd += ["--workload", args.workload, "--n", str(args.n), "--steps", "4", "--warmup", "1", "--kernel-steps",
                 "2", "--copy-calib", str(args.traffic_copy_mib), "--no-batch", "--no-serve", "--no-cpu", "--no-snap",
-                "--no-traffic"]
-        if args.fp16:
+                "--no-traffic", "--no-fp16"]
+        if fp16:
             cmd.append("--fp16")
     else:
         cmd += child_args
@@ -531,15 +535,17 @@ def fp16_section(args, ctx):
     step_gbs = info.bytes_per_step / (ms_step * 1e-3) / 1e9
     core_gbs = info.core_bytes / (ms_h * 1e-3) / 1e9
     print(f"bench: FP16 storage {ms_step:.4f} ms/step", file=sys.stderr, flush=True)
-    return {"workload": "the main line's workload, E and H stored in FP16 (FP32 arithmetic, automatic "
+    kern = (f"k_step3r interior box (fused H+E step, {info.interior_rows} rows per block sharing neighbour rows "
+            f"through LDS), FP16 storage" if info.interior_rows else "k_step3 interior box (fused H+E step), FP16 storage")
+    return info, {"workload": "the main line's workload, E and H stored in FP16 (FP32 arithmetic, automatic "
                         "power-of-two storage scale)",
             "value": round(w.cells * args.steps / elapsed / 1e9, 3), "unit": "Gcell-updates/s",
             "steps": args.steps, "ms_per_step": round(ms_step, 5), "bytes_per_step": info.bytes_per_step,
             "step_GBps": round(step_gbs, 1), "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
-            "roofline": {"bound": "hbm", "kernel": "k_step3 interior box (fused H+E step), FP16 storage",
+            "roofline": {"bound": "hbm", "kernel": kern,
                          "achieved": round(core_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(core_gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": info.core_bytes,
-                         "ms_per_launch": round(ms_h, 5)},
+                         "ms_per_launch": round(ms_h, 5), "traffic": None},
             "ms_shell": round(ms_e, 5), "kz": info.kz, "cells_per_lane": info.cells_per_lane,
             "shell_concurrent": bool(info.shell_concurrent), "shell_pipelined": bool(info.shell_pipelined),
             "interior_rows": info.interior_rows}
@@ -610,7 +616,7 @@ def batch_traffic(args, cells, gcell_per_s):
             "hbm_frac": round((rd + wr) * gcell_per_s / HBM_PEAK_GBS, 4)}
 
 
-def traffic_section(args, info, dom_pattern):
+def traffic_section(args, info, dom_pattern, fp16=None):
     """HBM bytes per launch of the dominant kernel: FETCH_SIZE and WRITE_SIZE
     (KiB per dispatch) from two rocprofv3 --pmc runs, each scaled by what the
     same counter reads for a plain device copy of known size in the same run."""
@@ -627,7 +633,7 @@ def traffic_section(args, info, dom_pattern):
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         outdir = tempfile.mkdtemp(prefix="lx_pmc_", dir="/tmp")
         try:
-            rows = _pmc_pass(counter, args, tune, outdir)
+            rows = _pmc_pass(counter, args, tune, outdir, fp16=fp16)
         finally:
             shutil.rmtree(outdir, ignore_errors=True)
         dom = [v for name, v in rows.values() if pat.search(name)]
@@ -747,7 +753,7 @@ def main():
         # fused H+E step: the interior-box launch (ms_h) and, with CPML, the
         # shell launch around it (ms_e); the box launch is the dominant kernel
         dom, dbytes, dms = "k_step3 interior box (fused H+E step)", info.core_bytes, ms_h
-        dom_pat = r"k_step3r?<[^<>,]+, [01], \d+, \d+>"
+        dom_pat = rf"k_step3r?<{'__half' if args.fp16 else 'float'}, [01], \d+, \d+>"
         if info.interior_rows:
             dom = (f"k_step3r interior box (fused H+E step, {info.interior_rows} rows per block sharing neighbour "
                    f"rows through LDS)")
@@ -762,6 +768,19 @@ def main():
     achieved = dbytes / (dms * 1e-3) / 1e9
     step_gbs = info.bytes_per_step / (ms_step * 1e-3) / 1e9
     mx, _ = sim.monitor()
+    # the reference's time loop prints max|E|^2 every step
+    # (maxwell_three_dim_pml.cc:167-173, 266-276); the C++ facade runs it so
+    # by default: one step, then the FP64 monitor, per iteration
+    loop_n = 40
+    t1 = time.perf_counter()
+    for _ in range(loop_n):
+        sim.monitor()
+    ms_mon = (time.perf_counter() - t1) * 1e3 / loop_n
+    t1 = time.perf_counter()
+    for _ in range(loop_n):
+        sim.step(1)
+        sim.monitor()
+    ms_loop = (time.perf_counter() - t1) * 1e3 / loop_n
 
     out = {
         "metric": "Gcell-updates/s, %HBM roofline; batched solves/s, p50/p99 @ fixed L2 err",
@@ -789,19 +808,31 @@ def main():
                      "bytes_per_launch": dbytes, "ms_per_launch": round(dms, 5)},
     }
     out["config"]["shell_pipelined"] = bool(info.shell_pipelined)
+    out["reference_loop"] = {"iterations": loop_n, "ms_per_step": round(ms_loop, 5), "ms_monitor": round(ms_mon, 5),
+                             "note": "lx_sim_step(1) then lx_sim_monitor (FP64 max|E|^2, energy, NaN guard) per "
+                                     "iteration, the facade's default loop; the main line above steps in graphs "
+                                     "without a per-step monitor"}
     if not args.no_snap and args.workload == "cube":
         out["snapshots"] = snapshot_section(args, w, ctx, sim)
     sim.close()
+    info16 = None
     if not args.no_fp16 and args.workload == "cube" and not args.fp16:
-        out["fp16"] = fp16_section(args, ctx)
+        info16, out["fp16"] = fp16_section(args, ctx)
     if not args.no_traffic:
-        try:
-            t = traffic_section(args, info, dom_pat)
-            out["roofline"]["traffic"] = t["read"] + t["write"]
-            out["roofline"]["traffic_detail"] = t
-            print(f"bench: dominant kernel HBM traffic {t}", file=sys.stderr, flush=True)
-        except Exception as exc:   # traffic stays null; the reason goes to stderr
-            print(f"bench: traffic not measured: {exc}", file=sys.stderr, flush=True)
+        # HBM traffic of the dominant kernel (and of the FP16 line's interior
+        # launch), each from children running only that line, pinned to the
+        # configuration the timed run used
+        lines = [(out["roofline"], info, dom_pat, args.fp16)]
+        if info16 is not None and info16.fused_step:
+            lines.append((out["fp16"]["roofline"], info16, r"k_step3r?<__half, [01], \d+, \d+>", True))
+        for roof, inf, pat, f16 in lines:
+            try:
+                t = traffic_section(args, inf, pat, fp16=f16)
+                roof["traffic"] = t["read"] + t["write"]
+                roof["traffic_detail"] = t
+                print(f"bench: {'FP16' if f16 else 'main line'} interior HBM traffic {t}", file=sys.stderr, flush=True)
+            except Exception as exc:   # traffic stays null; the reason goes to stderr
+                print(f"bench: traffic not measured: {exc}", file=sys.stderr, flush=True)
     if not args.no_batch:
         out["batched"] = batch_section(args)
         if args.batch3d_sims > 0:

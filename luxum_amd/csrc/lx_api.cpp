@@ -994,7 +994,7 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
     if ((st = dalloc(s, &q, 64))) return bail(st);
     s->counter = (int *)q;
     launch_zero_bytes(s->counter, 64, s->cs);
-    if ((st = dalloc(s, &q, 64))) return bail(st);
+    if ((st = dalloc(s, &q, sizeof(double) * (4 + 3 * (size_t)kMonBlocks)))) return bail(st);
     s->mon = (double *)q;
     if ((st = build_tables(s))) return bail(st);
     if ((st = build_tfsf(s))) return bail(st);
@@ -1307,7 +1307,6 @@ lx_status lx_sim_monitor(lx_sim *s, double *max_e2, double *energy) {
     if (!s) return fail(LX_EINVAL, "null simulation");
     HIP_TRY(hipSetDevice(s->ctx->device));
     member_enter(s);
-    launch_zero_bytes(s->mon, 32, s->cs);
     const double cb0 = s->c0 * s->dt / s->p.dx;   // vacuum coefficient: eps_r = cb0/cb
     if (s->fp16) launch_monitor(make_grid<__half>(s), s->planar, cb0, s->mon, s->cs);
     else launch_monitor(make_grid<float>(s), s->planar, cb0, s->mon, s->cs);

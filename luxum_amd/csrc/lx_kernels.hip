@@ -771,61 +771,103 @@ __global__ __launch_bounds__(256) void k_gather(Grid<T> g, const T *F, int ex, i
     }
 }
 
-__device__ __forceinline__ void atomic_max_pos(double *addr, double v) {
-    // non-negative doubles order like their bit patterns as unsigned integers
-    atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
+// Monitor, pass 1: max|E|^2 at equal indices (over the reference's index
+// range), the discrete energy sum of eps_r*E^2 + (eta0*H)^2 and the count of
+// non-finite values, one partial triple per block.  eps_r of each E
+// component and material is cb0/cb (cb0 = c*dt/dx), computed once per block
+// into LDS.  Block b takes rows b, b + gridDim.x, ... (a row: px contiguous
+// cells of one (j, k)), two rows per iteration so that each thread has two
+// rows' loads in flight; the threads of a block stride along x.  The cell
+// partition and every reduction order are fixed, so the result does not
+// change from run to run.
+template <typename T>
+__device__ __forceinline__ void monitor_cell(const Grid<T> &g, int planar, const double (*seps)[256], long long o,
+                                             bool in_range, double &mx, double &sum, int &bad) {
+    const long long fs = g.fstride;
+    if (planar) {
+        const double e = ld1(g.F + fs + o);
+        const double hx = ld1(g.F + 3 * fs + o), hz = ld1(g.F + 5 * fs + o);
+        mx = fmax(mx, e * e);
+        sum += e * e * seps[1][g.mat[o]] + hx * hx + hz * hz;
+        bad += !isfinite(e) || !isfinite(hx) || !isfinite(hz);
+    } else {
+        const double ex = ld1(g.F + o), ey = ld1(g.F + fs + o), ez = ld1(g.F + 2 * fs + o);
+        const double hx = ld1(g.F + 3 * fs + o), hy = ld1(g.F + 4 * fs + o), hz = ld1(g.F + 5 * fs + o);
+        const int id = g.mat[o];
+        if (in_range) mx = fmax(mx, ex * ex + ey * ey + ez * ez);
+        sum += ex * ex * seps[0][id] + ey * ey * seps[1][id] + ez * ez * seps[2][id] + hx * hx + hy * hy + hz * hz;
+        bad += !isfinite(ex) || !isfinite(ey) || !isfinite(ez) || !isfinite(hx) || !isfinite(hy) || !isfinite(hz);
+    }
 }
 
-// max|E|^2 at equal indices, the discrete energy sum of eps_r*E^2 + (eta0*H)^2
-// (eps_r of each E component from its coefficient table: cb0/cb = eps_r with
-// cb0 = c*dt/dx) and the count of non-finite values.
+// block-wide (256 threads) max, sum and count; thread 0 holds the result
+__device__ __forceinline__ void block_reduce3(double &mx, double &sum, int &bad) {
+    __shared__ double smax[4], ssum[4];
+    __shared__ int sbad[4];
+    for (int d = 32; d >= 1; d >>= 1) {
+        mx = fmax(mx, __shfl_down(mx, d, 64));
+        sum += __shfl_down(sum, d, 64);
+        bad += __shfl_down(bad, d, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        smax[w] = mx;
+        ssum[w] = sum;
+        sbad[w] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mx = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+        sum = (ssum[0] + ssum[1]) + (ssum[2] + ssum[3]);
+        bad = (sbad[0] + sbad[1]) + (sbad[2] + sbad[3]);
+    }
+}
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_monitor(Grid<T> g, int planar, double cb0, double *out) {
-    __shared__ double smax[256];
-    __shared__ double ssum[256];
-    __shared__ int sbad[256];
-    const int tid = threadIdx.x;
-    const int i = blockIdx.x * 256 + tid;
-    const int j = blockIdx.y;
-    const int k = blockIdx.z;
+__global__ __launch_bounds__(256) void k_monitor(Grid<T> g, int planar, double cb0, double *part) {
+    __shared__ double seps[3][256];
+    for (int q = threadIdx.x; q < 3 * 256; q += 256) seps[q >> 8][q & 255] = cb0 / (double)g.cb[q];
+    __syncthreads();
+    const int ny = planar ? 1 : g.ny, nz = g.nz, nx = g.nx;
+    const long long rows = (long long)ny * nz;
     double mx = 0.0, sum = 0.0;
     int bad = 0;
-    const long long fs = g.fstride;
-    if (i < g.nx) {
-        const long long o = (long long)k * g.plane + (long long)j * g.px + i;
-        if (planar) {
-            const double e = ld1(g.F + fs + o);
-            mx = e * e;
-            sum = e * e * (cb0 / (double)g.cb[256 + g.mat[o]]);
-            const double hx = ld1(g.F + 3 * fs + o), hz = ld1(g.F + 5 * fs + o);
-            sum += hx * hx + hz * hz;
-            bad = !isfinite(e) || !isfinite(hx) || !isfinite(hz);
-        } else {
-            const double ex = ld1(g.F + o), ey = ld1(g.F + fs + o), ez = ld1(g.F + 2 * fs + o);
-            const double hx = ld1(g.F + 3 * fs + o), hy = ld1(g.F + 4 * fs + o), hz = ld1(g.F + 5 * fs + o);
-            if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) mx = ex * ex + ey * ey + ez * ez;
-            const int id = g.mat[o];
-            sum = ex * ex * (cb0 / (double)g.cb[id]) + ey * ey * (cb0 / (double)g.cb[256 + id]) +
-                  ez * ez * (cb0 / (double)g.cb[512 + id]) + hx * hx + hy * hy + hz * hz;
-            bad = !isfinite(ex) || !isfinite(ey) || !isfinite(ez) || !isfinite(hx) || !isfinite(hy) || !isfinite(hz);
+    for (long long r = blockIdx.x; r < rows; r += 2 * (long long)gridDim.x) {
+        const long long r2 = r + gridDim.x;
+        const bool two = r2 < rows;
+        const int j = (int)(r % ny), k = (int)(r / ny);
+        const int j2 = two ? (int)(r2 % ny) : 0, k2 = two ? (int)(r2 / ny) : 0;
+        const long long base = (long long)k * g.plane + (long long)j * g.px;
+        const long long base2 = (long long)k2 * g.plane + (long long)j2 * g.px;
+        const bool rin = j < ny - 1 && k < nz - 1, rin2 = j2 < ny - 1 && k2 < nz - 1;
+        for (int i = threadIdx.x; i < nx; i += 256) {
+            monitor_cell(g, planar, seps, base + i, rin && i < nx - 1, mx, sum, bad);
+            if (two) monitor_cell(g, planar, seps, base2 + i, rin2 && i < nx - 1, mx, sum, bad);
         }
     }
-    smax[tid] = mx;
-    ssum[tid] = sum;
-    sbad[tid] = bad;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) {
-            smax[tid] = fmax(smax[tid], smax[tid + s]);
-            ssum[tid] += ssum[tid + s];
-            sbad[tid] += sbad[tid + s];
-        }
-        __syncthreads();
+    block_reduce3(mx, sum, bad);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x + 0] = mx;
+        part[3 * blockIdx.x + 1] = sum;
+        part[3 * blockIdx.x + 2] = (double)bad;
     }
-    if (tid == 0) {
-        atomic_max_pos(&out[0], smax[0]);
-        atomicAdd(&out[1], ssum[0]);
-        if (sbad[0]) atomicAdd(&out[2], (double)sbad[0]);
+}
+
+// Monitor, pass 2 (one block): the partial triples of pass 1 into out[0..2],
+// each thread over a fixed subset in a fixed order.
+__global__ __launch_bounds__(256) void k_monitor_final(const double *__restrict__ part, int nb, double *out) {
+    double mx = 0.0, sum = 0.0;
+    int bad = 0;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        mx = fmax(mx, part[3 * b]);
+        sum += part[3 * b + 1];
+        bad += (int)part[3 * b + 2];
+    }
+    block_reduce3(mx, sum, bad);
+    if (threadIdx.x == 0) {
+        out[0] = mx;
+        out[1] = sum;
+        out[2] = (double)bad;
     }
 }
 
@@ -944,8 +986,11 @@ void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double scale, do
 }
 template <typename T>
 void launch_monitor(const Grid<T> &g, int planar, double cb0, double *out, hipStream_t s) {
-    dim3 grd((g.nx + 255) / 256, g.ny, g.nz);
-    hipLaunchKernelGGL(k_monitor<T>, grd, dim3(256), 0, s, g, planar, cb0, out);
+    const long long rows = (long long)(planar ? 1 : g.ny) * g.nz;
+    const int nb = (int)std::min<long long>(kMonBlocks, std::max<long long>(rows, 1));
+    double *part = out + 4;
+    hipLaunchKernelGGL(k_monitor<T>, dim3(nb), dim3(256), 0, s, g, planar, cb0, part);
+    hipLaunchKernelGGL(k_monitor_final, dim3(1), dim3(256), 0, s, part, nb, out);
 }
 void launch_ids_to_device(const uint8_t *src, uint8_t *dst, int nx, int ny, int nz, int px, long long plane,
                           hipStream_t s) {

@@ -157,3 +157,55 @@ def test_monitor_energy_vs_oracle(gpu_ctx, planar):
     assert abs(en - oen) <= TOL32 * oen
     assert abs(mx - omx) <= 2 * TOL32 * omx
     g.close()
+
+
+@pytest.mark.parametrize("planar", [False, True], ids=["3d", "2d"])
+def test_monitor_reduction_exact(gpu_ctx, planar):
+    """The monitor's reduction against numpy on the same run's read-back
+    fields (no oracle tolerance in between): max|E|^2 over i < nx-1, j < ny-1,
+    k < nz-1 at equal indices (3D; every Ey cell in 2D) equal to 1e-12, the
+    energy sum(eps_r*E^2 + (eta0*H)^2)/2 * eps0 * cell volume to 1e-12, and
+    a second call bit-identical to the first (fixed partition and reduction
+    order)."""
+    from luxum_amd.params import LX_EX, LX_EZ, LX_HX, LX_HY, LX_HZ
+    w = configs.config0(n=160, steps=300) if planar else aniso(n=40, steps=200)
+    steps = 300 if planar else 200
+    say(f"monitor reduction: {'2D' if planar else '3D'}, {steps} steps")
+    g = make(w, gpu_ctx)
+    info = g.info()
+    g.step(steps)
+    g.sync()
+    mx, en = g.monitor()
+    mx2, en2 = g.monitor()
+    assert (mx, en) == (mx2, en2)
+    p = w.p
+    nx, ny, nz = p.dims
+    eps = np.asarray(w.eps_r, dtype=np.float64)
+    eps3 = eps if eps.ndim == 2 else np.repeat(eps[:, None], 3, axis=1)
+    # the device weighs E by cb0/cb of its FP32 coefficient tables
+    cb = (info.dt / (info.eps0 * eps3) / (info.eta0 * p.dx)).astype(np.float32).astype(np.float64)
+    eps3 = (info.c0 * info.dt / p.dx) / cb
+    ids = w.ids
+    if planar:
+        ey = g.field(LX_EY)
+        hx, hz = g.field(LX_HX), g.field(LX_HZ)
+        ref_mx = float(np.max(ey ** 2))
+        e_sum = float(np.sum(ey ** 2 * eps3[ids.reshape(ey.shape), 1]))
+        h_sum = float(np.sum((info.eta0 * hx) ** 2) + np.sum((info.eta0 * hz) ** 2))
+        vol = p.dx * p.dz
+    else:
+        es = [g.field(c) for c in (LX_EX, LX_EY, LX_EZ)]
+        hs = [g.field(c) for c in (LX_HX, LX_HY, LX_HZ)]
+        ref_mx = float(np.max(sum(e[:nx - 1, :ny - 1, :nz - 1] ** 2 for e in es)))
+        e_sum = 0.0
+        for c, e in enumerate(es):
+            sx, sy, sz = e.shape
+            e_sum += float(np.sum(e ** 2 * eps3[ids[:sx, :sy, :sz], c]))
+        h_sum = float(sum(np.sum((info.eta0 * h) ** 2) for h in hs))
+        vol = p.dx * p.dy * p.dz
+    ref_en = 0.5 * info.eps0 * (e_sum + h_sum) * vol
+    say(f"max|E|^2 {mx:.15e} vs {ref_mx:.15e}; energy {en:.15e} vs {ref_en:.15e}")
+    assert ref_mx > 0 and ref_en > 0
+    assert abs(mx - ref_mx) <= 1e-12 * ref_mx
+    assert abs(en - ref_en) <= 1e-12 * ref_en
+    g.close()
