@@ -627,10 +627,12 @@ void enqueue_seq(lx_sim *s, int n, std::vector<hipEvent_t> *ev = nullptr) {
     else enqueue_seq_t<float>(s, n, ev);
 }
 
-// Captured step sequences: 32, 16, 8 and 4 steps (even counts keep the
-// buffer state); a call of n steps replays the largest that fit
-constexpr int kGraphLens = 4;
-constexpr int kGraphSteps[kGraphLens] = {32, 16, 8, 4};
+// Captured step sequences of 32, 16, 8, 4, 2 and 1 steps; a call of n steps
+// replays the largest that fit (a one-step call, as a per-step time loop
+// makes, replays the one-step graph; an odd count leaves the fused step's
+// state in the other buffer set, as the eager sequence does)
+constexpr int kGraphLens = 6;
+constexpr int kGraphSteps[kGraphLens] = {32, 16, 8, 4, 2, 1};
 
 void drop_graph(lx_sim *s) {
     for (int i = 0; i < kGraphLens; ++i)
@@ -1170,11 +1172,14 @@ lx_status lx_sim_step(lx_sim *s, int32_t nsteps) {
         int done = 0;
         for (int i = 0; i < kGraphLens; ++i) {
             const int K = kGraphSteps[i];
-            if (n - done < K) continue;
-            hipGraphExec_t gx = ensure_graph(s, i, &st);
-            if (st) return st;
-            if (gx)
-                for (; done + K <= n; done += K) HIP_TRY(hipGraphLaunch(gx, s->cs));
+            while (n - done >= K) {
+                hipGraphExec_t gx = ensure_graph(s, i, &st);   // per buffer state
+                if (st) return st;
+                if (!gx) break;
+                HIP_TRY(hipGraphLaunch(gx, s->cs));
+                done += K;
+                settle_parity(s, K);
+            }
         }
         enqueue_seq(s, n - done);
         settle_parity(s, n - done);

@@ -100,8 +100,8 @@ struct lx_sim {
     // graphs of kGraphSteps[i]-step sequences, one per buffer state of the
     // fused step (keyed by the field buffer read first, gF), kept while the
     // fused buffer pair swaps after odd step counts
-    hipGraphExec_t gexec[4][2] = {};
-    const void *gF[4][2] = {};
+    hipGraphExec_t gexec[6][2] = {};
+    const void *gF[6][2] = {};
     int graphs = 1;
     int steps = 0;
     float cb_host[3 * 256];   // Ex, Ey, Ez coefficient tables
