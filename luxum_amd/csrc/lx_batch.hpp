@@ -8,9 +8,16 @@
 
 namespace lx {
 
+// Tile geometry of the batched 2D kernel.  Measured on 256 PhC members
+// (Gcell-updates/s): kTT 4/6/8/10/12/16 -> 526/690/802/686/617/507; kRW
+// 4/8/16/32 rows per wavefront (7-8, 7, 4, 2 waves per SIMD) -> 519/728/
+// 802/598.  Fewer rows per wavefront raise occupancy but add boundary-row
+// exchanges, barriers and per-wavefront work per cell; a shallower halo adds
+// region loads per step, a deeper one recomputes more halo cells.
 constexpr int kTT = 8;               // timesteps per chunk = halo width
-constexpr int kNW = 4;               // wavefronts per block, 16 region rows each
-constexpr int kRH = 16 * kNW;        // region rows (z)
+constexpr int kRW = 16;              // region rows (z) per wavefront
+constexpr int kNW = 64 / kRW;        // wavefronts per block
+constexpr int kRH = kRW * kNW;       // region rows (z): 64
 constexpr int kTW = 64 - 2 * kTT;    // tile interior columns (x): region of 64 columns, one per lane
 constexpr int kTH = kRH - 2 * kTT;   // tile interior rows (z)
 

@@ -95,10 +95,10 @@ __device__ __forceinline__ float rdlane(float v, int l) {
 }
 
 // Region of a job: 64 columns (one per lane) x kRH rows; wave w owns rows
-// 16w .. 16w+15.  Fields Ey, Hx, Hz, the cell's cb and (general path) the
+// kRW*w .. kRW*w+kRW-1.  Fields Ey, Hx, Hz, the cell's cb and (general path) the
 // CPML psi stay in VGPRs for the whole chunk; x coefficients are per lane,
 // z coefficients per row.  Per half step the waves exchange one boundary row
-// through LDS (Ey row 16w before the H update, Hx row 16w+15 before the E
+// through LDS (Ey row kRW*w before the H update, Hx row kRW*w+kRW-1 before the E
 // update); x neighbours come from DPP wavefront shifts.
 //
 // No cell update is masked and no psi update is conditional: the CPML terms
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
     const int nx = S.nx, nz = S.nz;
     const int gi0 = job.i0 - kTT, gk0 = job.k0 - kTT;   // region origin
     const int i = gi0 + lane;
-    const int kb = gk0 + 16 * w;                        // first row of this wave
+    const int kb = gk0 + kRW * w;                        // first row of this wave
     const long long px = S.px, fs = S.fstride;
     const float *tab = S.tab;
     const int tn = S.tn;
@@ -204,14 +204,14 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
     const int kc0 = min(max(kb, 0), tn - 1);
     const float khz0 = tab[(2 * 6 + TK_KH) * tn + kc0], kez0 = tab[(2 * 6 + TK_KE) * tn + kc0];
 
-    // fields, cb and psi of this lane's 16 cells
+    // fields, cb and psi of this lane's kRW cells
     const int slo0 = S.slo[0], shi0 = S.shi[0], slo2 = S.slo[2], shi2 = S.shi[2];
     const bool xslab = pml && colin && (i < slo0 || i >= shi0);
     const float *px0 = S.psi[par], *pz0 = S.psi[par] + 4 * (S.scells[0] + S.scells[1]);
-    float ey[16], hx[16], hz[16], cb[16];
-    float pxh[16], pxe[16], pzh[16], pze[16];   // x psi (Hz, Ey), z psi (Hx, Ey); unused when INNER
+    float ey[kRW], hx[kRW], hz[kRW], cb[kRW];
+    float pxh[kRW], pxe[kRW], pzh[kRW], pze[kRW];   // x psi (Hz, Ey), z psi (Hx, Ey); unused when INNER
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
+    for (int m = 0; m < kRW; ++m) {
         const int k = kb + m;
         ey[m] = hx[m] = hz[m] = cb[m] = 0.f;
         pxh[m] = pxe[m] = pzh[m] = pze[m] = 0.f;
@@ -238,15 +238,15 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
     }
     __syncthreads();   // probe list complete
     const int nprb = snprb;
-    // TF/SF boxes that touch this wave's 16 x 64 cells (wave-uniform bit masks)
+    // TF/SF boxes that touch this wave's kRW x 64 cells (wave-uniform bit masks)
     unsigned hmask = 0, emask = 0;
     for (int bq = 0; bq < S.ntfH; ++bq) {
         const TfBox &bx = S.tfH[bq];
-        if (bx.hi[0] >= gi0 && bx.lo[0] < gi0 + 64 && bx.hi[2] >= kb && bx.lo[2] < kb + 16) hmask |= 1u << bq;
+        if (bx.hi[0] >= gi0 && bx.lo[0] < gi0 + 64 && bx.hi[2] >= kb && bx.lo[2] < kb + kRW) hmask |= 1u << bq;
     }
     for (int bq = 0; bq < S.ntfE; ++bq) {
         const TfBox &bx = S.tfE[bq];
-        if (bx.hi[0] >= gi0 && bx.lo[0] < gi0 + 64 && bx.hi[2] >= kb && bx.lo[2] < kb + 16) emask |= 1u << bq;
+        if (bx.hi[0] >= gi0 && bx.lo[0] < gi0 + 64 && bx.hi[2] >= kb && bx.lo[2] < kb + kRW) emask |= 1u << bq;
     }
     hmask = __builtin_amdgcn_readfirstlane(hmask);
     emask = __builtin_amdgcn_readfirstlane(emask);
@@ -261,16 +261,16 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
         __syncthreads();
         const float eyn = (w + 1 < kNW) ? sEtop[w + 1][lane] : 0.f;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
+        for (int m = 0; m < kRW; ++m) {
             const float e0 = ey[m];
-            const float eu = m < 15 ? ey[m + 1] : eyn;
+            const float eu = m < kRW - 1 ? ey[m + 1] : eyn;
             const float er = wave_next(e0);
             const float d2 = eu - e0, d1 = er - e0;
             if (INNER) {
                 hx[m] -= -d2 * khz0;
                 hz[m] -= d1 * khx;
             } else {
-                const float4 z = szh[16 * w + m];
+                const float4 z = szh[kRW * w + m];
                 float acc = -d2 * z.x;
                 if (ZP) {
                     pzh[m] = z.y * pzh[m] + z.z * d2;
@@ -294,13 +294,13 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
             const int a = (bx.comp + 1 + bx.curl) % 3;
             const float sgn = bx.curl == 0 ? 1.f : -1.f;
             const bool xf = bx.fa == 0;
-            const int kl = kb + (lane & 15);
-            const bool ld = xf ? (lane < 16 && kl >= bx.lo[2] && kl <= bx.hi[2])
+            const int kl = kb + (lane & (kRW - 1));
+            const bool ld = xf ? (lane < kRW && kl >= bx.lo[2] && kl <= bx.hi[2])
                                : (i >= bx.lo[0] && i <= bx.hi[0]);
             const float dl = ld ? (float)(bx.dir * snap_value(S, bx, xf ? bx.lo[0] : i, xf ? kl : bx.lo[2], snap, mbase)) : 0.f;
             const bool face_lane = xf ? i == bx.lo[0] : ld;
 #pragma unroll
-            for (int m = 0; m < 16; ++m) {
+            for (int m = 0; m < kRW; ++m) {
                 const int k = kb + m;
                 if (!(xf ? (k >= bx.lo[2] && k <= bx.hi[2]) : k == bx.lo[2])) continue;   // wave-uniform
                 const float delta = xf ? rdlane(dl, m) : dl;
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
                 } else if (INNER) {
                     coef = khz0;
                 } else {
-                    const float4 z = szh[16 * w + m];
+                    const float4 z = szh[kRW * w + m];
                     coef = z.x;
                     if (ZP) {
                         if (face_lane) pzh[m] += z.z * delta;
@@ -331,20 +331,20 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
         // H probes of step t
         for (int q = 0; q < nprb; ++q) {
             const int d = sprd[q], c = d >> 16, rK = (d >> 8) & 255, rI = d & 255;
-            if (c >= 3 && t < pcap && lane == rI && (rK >> 4) == w) {
+            if (c >= 3 && t < pcap && lane == rI && rK / kRW == w) {
                 float v = 0.f;
 #pragma unroll
-                for (int m = 0; m < 16; ++m)
-                    if (m == (rK & 15)) v = c == 3 ? hx[m] : (c == 5 ? hz[m] : 0.f);
+                for (int m = 0; m < kRW; ++m)
+                    if (m == (rK & (kRW - 1))) v = c == 3 ? hx[m] : (c == 5 ? hz[m] : 0.f);
                 prec[(long long)t * np + sprb[q]] = (double)v * S.hscale;
             }
         }
         // ---------------- E half step
-        sHbot[w][lane] = hx[15];
+        sHbot[w][lane] = hx[kRW - 1];
         __syncthreads();
         const float hxp = w > 0 ? sHbot[w - 1][lane] : 0.f;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
+        for (int m = 0; m < kRW; ++m) {
             const float hd = m > 0 ? hx[m - 1] : hxp;
             const float hl = wave_prev(hz[m]);
             const float d1 = hx[m] - hd;
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
             if (INNER) {
                 ey[m] += cb[m] * (d1 * kez0 - d2 * kex);
             } else {
-                const float4 z = sze[16 * w + m];
+                const float4 z = sze[kRW * w + m];
                 float acc = d1 * z.x - d2 * kex;
                 if (ZP) {
                     pze[m] = z.y * pze[m] + z.z * d1;
@@ -372,13 +372,13 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
             const int a = (bx.comp + 1 + bx.curl) % 3;
             const float sgn = bx.curl == 0 ? 1.f : -1.f;
             const bool xf = bx.fa == 0;
-            const int kl = kb + (lane & 15);
-            const bool ld = xf ? (lane < 16 && kl >= bx.lo[2] && kl <= bx.hi[2])
+            const int kl = kb + (lane & (kRW - 1));
+            const bool ld = xf ? (lane < kRW && kl >= bx.lo[2] && kl <= bx.hi[2])
                                : (i >= bx.lo[0] && i <= bx.hi[0]);
             const float dl = ld ? (float)(bx.dir * snap_value(S, bx, xf ? bx.lo[0] : i, xf ? kl : bx.lo[2], snap, mbase)) : 0.f;
             const bool face_lane = xf ? i == bx.lo[0] : ld;
 #pragma unroll
-            for (int m = 0; m < 16; ++m) {
+            for (int m = 0; m < kRW; ++m) {
                 const int k = kb + m;
                 if (!(xf ? (k >= bx.lo[2] && k <= bx.hi[2]) : k == bx.lo[2])) continue;   // wave-uniform
                 const float delta = xf ? rdlane(dl, m) : dl;
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
                 } else if (INNER) {
                     coef = kez0;
                 } else {
-                    const float4 z = sze[16 * w + m];
+                    const float4 z = sze[kRW * w + m];
                     coef = z.x;
                     if (ZP) {
                         if (face_lane) pze[m] += z.z * delta;
@@ -405,11 +405,11 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
         // E probes of step t
         for (int q = 0; q < nprb; ++q) {
             const int d = sprd[q], c = d >> 16, rK = (d >> 8) & 255, rI = d & 255;
-            if (c < 3 && t < pcap && lane == rI && (rK >> 4) == w) {
+            if (c < 3 && t < pcap && lane == rI && rK / kRW == w) {
                 float v = 0.f;
 #pragma unroll
-                for (int m = 0; m < 16; ++m)
-                    if (m == (rK & 15) && c == 1) v = ey[m];
+                for (int m = 0; m < kRW; ++m)
+                    if (m == (rK & (kRW - 1)) && c == 1) v = ey[m];
                 prec[(long long)t * np + sprb[q]] = (double)v;
             }
         }
@@ -418,8 +418,8 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
     // ---------------- write the interior back into the other buffers
     float *qx0 = S.psi[par ^ 1], *qz0 = S.psi[par ^ 1] + 4 * (S.scells[0] + S.scells[1]);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const int k = kb + m, rk = 16 * w + m;
+    for (int m = 0; m < kRW; ++m) {
+        const int k = kb + m, rk = kRW * w + m;
         if (rk >= kTT && rk < kRH - kTT && lane >= kTT && lane < 64 - kTT && (INNER || (i < nx && k < nz))) {
             const long long o = (long long)k * px + i;
             bst(Fout + fs + o, ey[m]);
