@@ -232,6 +232,13 @@ lx_status lx_sim_read_probes(lx_sim *sim, double *dst, size_t n, int32_t *n_step
  * (eps_r of each E component from its material table).  Synchronous. */
 lx_status lx_sim_monitor(lx_sim *sim, double *max_e2, double *energy);
 
+/* The reference monitor alone: FP64 max over cells of Ex^2+Ey^2+Ez^2 at
+ * equal indices (maxwell_three_dim_pml.cc:266-276; 2D: max Ey^2,
+ * maxwell_two_dim_pml.cc:181), the same value as lx_sim_monitor's max_e2,
+ * from a pass over the E components only (3*s bytes per cell instead of
+ * 6*s+1).  LX_EUNSTABLE when an E value is not finite.  Synchronous. */
+lx_status lx_sim_max_e2(lx_sim *sim, double *max_e2);
+
 lx_status lx_sim_get_info(lx_sim *sim, lx_sim_info *info);
 
 /* The simulation's compute stream (hipStream_t) for event timing. */

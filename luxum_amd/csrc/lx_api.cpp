@@ -1186,9 +1186,9 @@ lx_status lx_sim_step(lx_sim *s, int32_t nsteps) {
         if ((st = check_launch())) return st;
         s->steps += n;
         left -= n;
-        if (s->p.monitor_every > 0) {
-            double mx, en;
-            if ((st = lx_sim_monitor(s, &mx, &en))) return st;
+        if (s->p.monitor_every > 0) {   // the NaN guard: the E-only pass (H feeds E within a step)
+            double mx;
+            if ((st = lx_sim_max_e2(s, &mx))) return st;
         }
     }
     return LX_OK;
@@ -1308,23 +1308,46 @@ lx_status lx_sim_read_probes(lx_sim *s, double *dst, size_t n, int32_t *n_steps)
     return LX_OK;
 }
 
+// The monitor's two passes on the compute stream and the result to the host
+// (max|E|^2, the energy sum and the non-finite count, unscaled); e_only
+// reads the E components alone (max and their non-finite count).
+static lx_status run_monitor(lx_sim *s, bool e_only, double out[3]) {
+    const double cb0 = s->c0 * s->dt / s->p.dx;   // vacuum coefficient: eps_r = cb0/cb
+    if (s->fp16) launch_monitor(make_grid<__half>(s), s->planar, cb0, s->mon, e_only, s->cs);
+    else launch_monitor(make_grid<float>(s), s->planar, cb0, s->mon, e_only, s->cs);
+    lx_status st = check_launch();
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(out, s->mon, sizeof(double) * 3, hipMemcpyDeviceToHost, s->cs));
+    HIP_TRY(hipStreamSynchronize(s->cs));
+    const double s2 = s->sscale * s->sscale;
+    out[0] /= s2;
+    out[1] /= s2;
+    return LX_OK;
+}
+
 lx_status lx_sim_monitor(lx_sim *s, double *max_e2, double *energy) {
     if (!s) return fail(LX_EINVAL, "null simulation");
     HIP_TRY(hipSetDevice(s->ctx->device));
     member_enter(s);
-    const double cb0 = s->c0 * s->dt / s->p.dx;   // vacuum coefficient: eps_r = cb0/cb
-    if (s->fp16) launch_monitor(make_grid<__half>(s), s->planar, cb0, s->mon, s->cs);
-    else launch_monitor(make_grid<float>(s), s->planar, cb0, s->mon, s->cs);
-    lx_status st = check_launch();
+    double out[3] = {0, 0, 0};
+    lx_status st = run_monitor(s, false, out);
     if (st) return st;
-    double out[4] = {0, 0, 0, 0};
-    HIP_TRY(hipMemcpyAsync(out, s->mon, sizeof(double) * 3, hipMemcpyDeviceToHost, s->cs));
-    HIP_TRY(hipStreamSynchronize(s->cs));
     const double vol = s->p.dx * s->p.dz * (s->planar ? 1.0 : s->p.dy);
-    const double s2 = s->sscale * s->sscale;
-    if (max_e2) *max_e2 = out[0] / s2;
-    if (energy) *energy = 0.5 * s->eps0 * out[1] * vol / s2;
+    if (max_e2) *max_e2 = out[0];
+    if (energy) *energy = 0.5 * s->eps0 * out[1] * vol;
     if (out[2] > 0) return fail(LX_EUNSTABLE, "non-finite field values after %d steps", s->steps);
+    return LX_OK;
+}
+
+lx_status lx_sim_max_e2(lx_sim *s, double *max_e2) {
+    if (!s || !max_e2) return fail(LX_EINVAL, "lx_sim_max_e2: null pointer");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    member_enter(s);
+    double out[3] = {0, 0, 0};
+    lx_status st = run_monitor(s, true, out);
+    if (st) return st;
+    *max_e2 = out[0];
+    if (out[2] > 0) return fail(LX_EUNSTABLE, "non-finite E values after %d steps", s->steps);
     return LX_OK;
 }
 

@@ -257,11 +257,12 @@ template <typename T>
 void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double scale, double *dst, hipStream_t s);
 
 // FP64 monitor: out[0] = max |E|^2 (reference index range), out[1] = sum of
-// eps_r*E^2 + Hs^2, out[2] = non-finite count; out holds 4 + 3*kMonBlocks
+// eps_r*E^2 + Hs^2 (0 with e_only), out[2] = non-finite count (of the E
+// components only with e_only); out holds 4 + 3*kMonBlocks
 // doubles (the per-block partial triples follow the result).
 constexpr int kMonBlocks = 2048;
 template <typename T>
-void launch_monitor(const Grid<T> &g, int planar, double cb0, double *out, hipStream_t s);
+void launch_monitor(const Grid<T> &g, int planar, double cb0, double *out, bool e_only, hipStream_t s);
 
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 // material ids, host C order (i, j, k) -> device layout (k, j, x)

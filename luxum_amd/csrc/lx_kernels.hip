@@ -780,10 +780,22 @@ __global__ __launch_bounds__(256) void k_gather(Grid<T> g, const T *F, int ex, i
 // rows' loads in flight; the threads of a block stride along x.  The cell
 // partition and every reduction order are fixed, so the result does not
 // change from run to run.
-template <typename T>
+template <typename T, bool EONLY>
 __device__ __forceinline__ void monitor_cell(const Grid<T> &g, int planar, const double (*seps)[256], long long o,
                                              bool in_range, double &mx, double &sum, int &bad) {
     const long long fs = g.fstride;
+    if (EONLY) {   // max|E|^2 and the E components' non-finite count only
+        if (planar) {
+            const double e = ld1(g.F + fs + o);
+            mx = fmax(mx, e * e);
+            bad += !isfinite(e);
+        } else {
+            const double ex = ld1(g.F + o), ey = ld1(g.F + fs + o), ez = ld1(g.F + 2 * fs + o);
+            if (in_range) mx = fmax(mx, ex * ex + ey * ey + ez * ez);
+            bad += !isfinite(ex) || !isfinite(ey) || !isfinite(ez);
+        }
+        return;
+    }
     if (planar) {
         const double e = ld1(g.F + fs + o);
         const double hx = ld1(g.F + 3 * fs + o), hz = ld1(g.F + 5 * fs + o);
@@ -823,11 +835,15 @@ __device__ __forceinline__ void block_reduce3(double &mx, double &sum, int &bad)
     }
 }
 
-template <typename T>
+// EONLY: the reference's monitor alone (max|E|^2 and the E components'
+// non-finite count; 3*s B per cell instead of 6*s+1), energy left 0.
+template <typename T, bool EONLY>
 __global__ __launch_bounds__(256) void k_monitor(Grid<T> g, int planar, double cb0, double *part) {
     __shared__ double seps[3][256];
-    for (int q = threadIdx.x; q < 3 * 256; q += 256) seps[q >> 8][q & 255] = cb0 / (double)g.cb[q];
-    __syncthreads();
+    if (!EONLY) {
+        for (int q = threadIdx.x; q < 3 * 256; q += 256) seps[q >> 8][q & 255] = cb0 / (double)g.cb[q];
+        __syncthreads();
+    }
     const int ny = planar ? 1 : g.ny, nz = g.nz, nx = g.nx;
     const long long rows = (long long)ny * nz;
     double mx = 0.0, sum = 0.0;
@@ -841,8 +857,8 @@ __global__ __launch_bounds__(256) void k_monitor(Grid<T> g, int planar, double c
         const long long base2 = (long long)k2 * g.plane + (long long)j2 * g.px;
         const bool rin = j < ny - 1 && k < nz - 1, rin2 = j2 < ny - 1 && k2 < nz - 1;
         for (int i = threadIdx.x; i < nx; i += 256) {
-            monitor_cell(g, planar, seps, base + i, rin && i < nx - 1, mx, sum, bad);
-            if (two) monitor_cell(g, planar, seps, base2 + i, rin2 && i < nx - 1, mx, sum, bad);
+            monitor_cell<T, EONLY>(g, planar, seps, base + i, rin && i < nx - 1, mx, sum, bad);
+            if (two) monitor_cell<T, EONLY>(g, planar, seps, base2 + i, rin2 && i < nx - 1, mx, sum, bad);
         }
     }
     block_reduce3(mx, sum, bad);
@@ -985,11 +1001,12 @@ void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double scale, do
     hipLaunchKernelGGL((k_gather<T, true>), grd, blk, 0, s, g, (const T *)g.F, ex, ey, ez, scale, dst);
 }
 template <typename T>
-void launch_monitor(const Grid<T> &g, int planar, double cb0, double *out, hipStream_t s) {
+void launch_monitor(const Grid<T> &g, int planar, double cb0, double *out, bool e_only, hipStream_t s) {
     const long long rows = (long long)(planar ? 1 : g.ny) * g.nz;
     const int nb = (int)std::min<long long>(kMonBlocks, std::max<long long>(rows, 1));
     double *part = out + 4;
-    hipLaunchKernelGGL(k_monitor<T>, dim3(nb), dim3(256), 0, s, g, planar, cb0, part);
+    if (e_only) hipLaunchKernelGGL((k_monitor<T, true>), dim3(nb), dim3(256), 0, s, g, planar, cb0, part);
+    else hipLaunchKernelGGL((k_monitor<T, false>), dim3(nb), dim3(256), 0, s, g, planar, cb0, part);
     hipLaunchKernelGGL(k_monitor_final, dim3(1), dim3(256), 0, s, part, nb, out);
 }
 void launch_ids_to_device(const uint8_t *src, uint8_t *dst, int nx, int ny, int nz, int px, long long plane,
@@ -1022,7 +1039,7 @@ void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
                                       const ProbeArgs &, Launch, hipStream_t);                              \
     template void launch_point_source<T>(const Grid<T> &, const PointSrc &, const ProbeArgs &, int, hipStream_t); \
     template void launch_gather<T>(const Grid<T> &, int, int, int, int, double, double *, hipStream_t);      \
-    template void launch_monitor<T>(const Grid<T> &, int, double, double *, hipStream_t);                            \
+    template void launch_monitor<T>(const Grid<T> &, int, double, double *, bool, hipStream_t);                            \
     template void launch_gather_e2<T>(const Grid<T> &, int, int, int, double, double *, hipStream_t);              \
     template void launch_tfsf_batch<T>(const B3Sim<T> *, const int2 *, int, int, int, int, hipStream_t);   \
     template void launch_finish_batch<T>(const B3Sim<T> *, int, int, int, hipStream_t);                   \

@@ -121,6 +121,12 @@ class Simulation:
         check(lib().lx_sim_monitor(self._h, C.byref(mx), C.byref(en)))
         return mx.value, en.value
 
+    def max_e2(self) -> float:
+        """The reference monitor alone (max|E|^2), from the E components."""
+        mx = C.c_double()
+        check(lib().lx_sim_max_e2(self._h, C.byref(mx)))
+        return mx.value
+
     def info(self) -> SimInfo:
         inf = SimInfo()
         check(lib().lx_sim_get_info(self._h, C.byref(inf)))

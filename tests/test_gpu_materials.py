@@ -166,7 +166,7 @@ def test_monitor_reduction_exact(gpu_ctx, planar):
     k < nz-1 at equal indices (3D; every Ey cell in 2D) equal to 1e-12, the
     energy sum(eps_r*E^2 + (eta0*H)^2)/2 * eps0 * cell volume to 1e-12, and
     a second call bit-identical to the first (fixed partition and reduction
-    order)."""
+    order); lx_sim_max_e2 (the E components alone) returns the same max."""
     from luxum_amd.params import LX_EX, LX_EZ, LX_HX, LX_HY, LX_HZ
     w = configs.config0(n=160, steps=300) if planar else aniso(n=40, steps=200)
     steps = 300 if planar else 200
@@ -178,6 +178,7 @@ def test_monitor_reduction_exact(gpu_ctx, planar):
     mx, en = g.monitor()
     mx2, en2 = g.monitor()
     assert (mx, en) == (mx2, en2)
+    assert g.max_e2() == mx   # the E-only pass: the same maximum
     p = w.p
     nx, ny, nz = p.dims
     eps = np.asarray(w.eps_r, dtype=np.float64)
