@@ -210,3 +210,38 @@ def test_monitor_reduction_exact(gpu_ctx, planar):
     assert abs(mx - ref_mx) <= 1e-12 * ref_mx
     assert abs(en - ref_en) <= 1e-12 * ref_en
     g.close()
+
+
+def test_monitor_unstable_guard(gpu_ctx):
+    """The non-finite guard: a 3D grid stepped above its Courant limit (0.9
+    against 1/sqrt(3)) grows until its fields overflow (FP32 inf, then NaN;
+    no trap).  With monitor_every = 25, lx_sim_step returns LX_EUNSTABLE at
+    the first check after that, and lx_sim_max_e2 and lx_sim_monitor report
+    it too; a stable run with the same guard steps through."""
+    from luxum_amd._lib import LX_EUNSTABLE, LuxumError
+    from luxum_amd.params import LX_SRC_SOFT, default_params
+    from luxum_amd.sim import Simulation
+    p = default_params(nx=40, ny=36, nz=32, use_pml=1, pml_thickness=6 * 5e-8, use_ps=1, src_kind=LX_SRC_SOFT,
+                       src_amp=1.0, src_lambda0=20 * 5e-8, src_fwhm=3.0, src_delay=4.0, src_pos=(20, 18, 16),
+                       courant=0.9)
+    p.monitor_every = 25
+    s = Simulation(p, gpu_ctx)
+    steps = 0
+    with pytest.raises(LuxumError) as e:
+        for _ in range(200):       # 5000 steps at most; growth overflows FP32 long before
+            s.step(25)
+            steps += 25
+    say(f"unstable run stopped after {steps + 25} steps: {e.value}")
+    assert e.value.status == LX_EUNSTABLE
+    for call in (s.max_e2, s.monitor):
+        with pytest.raises(LuxumError) as e2:
+            call()
+        assert e2.value.status == LX_EUNSTABLE
+    s.close()
+    p.courant = 0.5
+    s = Simulation(p, gpu_ctx)
+    s.step(500)
+    s.sync()
+    mx = s.max_e2()
+    assert np.isfinite(mx) and mx > 0
+    s.close()
