@@ -716,7 +716,11 @@ def main():
         del a, b
 
     prec = LX_FP16 if args.fp16 else LX_FP32
-    total = max(args.steps + args.warmup + args.kernel_steps, 2000)
+    # every step this simulation takes (the timed run, the kernel timing, the
+    # reference-shaped loop, the snapshot section's three runs and its |E|^2
+    # run), so that its probe buffer never grows mid-run (a growth drops the
+    # captured step graphs, and the next calls would time their capture)
+    total = max(args.steps + args.warmup + args.kernel_steps + 40 + 3 * args.snap_steps + 30 + 64, 2000)
     tag = "FP16 storage" if args.fp16 else "FP32"
     if args.workload == "phc_slab":
         w = configs.config2(steps=total, precision=prec)
