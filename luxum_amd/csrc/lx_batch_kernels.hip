@@ -30,12 +30,15 @@ template <>
 __device__ __forceinline__ float bld<float>(const float *p) { return *p; }
 template <>
 __device__ __forceinline__ float bld<__half>(const __half *p) { return __half2float(*p); }
+// the interior a job writes back is read only by the next chunk: non-temporal stores
 template <typename T>
 __device__ __forceinline__ void bst(T *p, float v);
 template <>
-__device__ __forceinline__ void bst<float>(float *p, float v) { *p = v; }
+__device__ __forceinline__ void bst<float>(float *p, float v) { __builtin_nontemporal_store(v, p); }
 template <>
-__device__ __forceinline__ void bst<__half>(__half *p, float v) { *p = __float2half(v); }
+__device__ __forceinline__ void bst<__half>(__half *p, float v) {
+    __builtin_nontemporal_store(__half_as_ushort(__float2half(v)), reinterpret_cast<unsigned short *>(p));
+}
 
 __device__ __forceinline__ int bslab(int x, int lo, int hi) { return x < lo ? x : lo + (x - hi); }
 
@@ -428,13 +431,13 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
             if (!INNER && pml) {
                 if (XP && xslab) {
                     const long long off = (long long)k * S.sw[0] + bslab(i, slo0, shi0);
-                    qx0[off] = pxh[m];
-                    qx0[3 * S.scells[0] + off] = pxe[m];
+                    __builtin_nontemporal_store(pxh[m], qx0 + off);
+                    __builtin_nontemporal_store(pxe[m], qx0 + 3 * S.scells[0] + off);
                 }
                 if (ZP && (k < slo2 || k >= shi2)) {
                     const long long off = (long long)bslab(k, slo2, shi2) * px + i;
-                    qz0[S.scells[2] + off] = pzh[m];
-                    qz0[2 * S.scells[2] + off] = pze[m];
+                    __builtin_nontemporal_store(pzh[m], qz0 + S.scells[2] + off);
+                    __builtin_nontemporal_store(pze[m], qz0 + 2 * S.scells[2] + off);
                 }
             }
         }

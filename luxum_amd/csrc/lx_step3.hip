@@ -139,29 +139,38 @@ __device__ __forceinline__ vf<4> ldv<__half, 4>(const __half *b, unsigned o) {
 }
 template <typename T, int V>
 __device__ __forceinline__ void stv(T *base, unsigned boff, const vf<V> &v);
+// stores of the written buffer are non-temporal: nothing reads them in the same step
+typedef float f2nt __attribute__((ext_vector_type(2)));
+typedef float f4nt __attribute__((ext_vector_type(4)));
+typedef unsigned u2nt __attribute__((ext_vector_type(2)));
 template <>
-__device__ __forceinline__ void stv<float, 1>(float *b, unsigned o, const vf<1> &v) { *at(b, o) = v.v[0]; }
+__device__ __forceinline__ void stv<float, 1>(float *b, unsigned o, const vf<1> &v) { __builtin_nontemporal_store(v.v[0], at(b, o)); }
 template <>
 __device__ __forceinline__ void stv<float, 2>(float *b, unsigned o, const vf<2> &v) {
-    *reinterpret_cast<float2 *>(at(b, o)) = make_float2(v.v[0], v.v[1]);
+    const f2nt x = {v.v[0], v.v[1]};
+    __builtin_nontemporal_store(x, reinterpret_cast<f2nt *>(at(b, o)));
 }
 template <>
-__device__ __forceinline__ void stv<__half, 1>(__half *b, unsigned o, const vf<1> &v) { *at(b, o) = __float2half(v.v[0]); }
+__device__ __forceinline__ void stv<__half, 1>(__half *b, unsigned o, const vf<1> &v) {
+    __builtin_nontemporal_store(__half_as_ushort(__float2half(v.v[0])), reinterpret_cast<unsigned short *>(at(b, o)));
+}
 template <>
 __device__ __forceinline__ void stv<__half, 2>(__half *b, unsigned o, const vf<2> &v) {
-    *reinterpret_cast<__half2 *>(at(b, o)) = __floats2half2_rn(v.v[0], v.v[1]);
+    const __half2 h = __floats2half2_rn(v.v[0], v.v[1]);
+    __builtin_nontemporal_store(*reinterpret_cast<const unsigned *>(&h), reinterpret_cast<unsigned *>(at(b, o)));
 }
 template <>
 __device__ __forceinline__ void stv<float, 4>(float *b, unsigned o, const vf<4> &v) {
-    *reinterpret_cast<float4 *>(at(b, o)) = make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
+    const f4nt x = {v.v[0], v.v[1], v.v[2], v.v[3]};
+    __builtin_nontemporal_store(x, reinterpret_cast<f4nt *>(at(b, o)));
 }
 template <>
 __device__ __forceinline__ void stv<__half, 4>(__half *b, unsigned o, const vf<4> &v) {
     const __half2 lo = __floats2half2_rn(v.v[0], v.v[1]), hi = __floats2half2_rn(v.v[2], v.v[3]);
-    uint2 u;
+    u2nt u;
     u.x = *reinterpret_cast<const unsigned *>(&lo);
     u.y = *reinterpret_cast<const unsigned *>(&hi);
-    *reinterpret_cast<uint2 *>(at(b, o)) = u;
+    __builtin_nontemporal_store(u, reinterpret_cast<u2nt *>(at(b, o)));
 }
 // coefficient row of the x axis (float table) at x index i0
 template <int V>
