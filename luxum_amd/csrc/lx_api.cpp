@@ -1474,33 +1474,11 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
-        // n steps of the current settings on cs (H and E of the half-step
-        // pair, or the fused step's launches)
-        auto enq = [&](int n) {
-            for (int q = 0; q < n; ++q) {
-                if (s->fp16) {
-                    Grid<__half> g = make_grid<__half>(s);
-                    if (s->fused) {
-                        g.Fn = (__half *)s->F_alt;
-                        g.psin = s->psi_alt;
-                        launch_fused_step(s, g);
-                    } else {
-                        launch_bulk(g, 0, 0, s->cs);
-                        launch_bulk(g, 0, 1, s->cs);
-                    }
-                } else {
-                    Grid<float> g = make_grid<float>(s);
-                    if (s->fused) {
-                        g.Fn = (float *)s->F_alt;
-                        g.psin = s->psi_alt;
-                        launch_fused_step(s, g);
-                    } else {
-                        launch_bulk(g, 0, 0, s->cs);
-                        launch_bulk(g, 0, 1, s->cs);
-                    }
-                }
-            }
-        };
+        // n steps of the current settings: the sequence lx_sim_step enqueues
+        // (TF/SF launches, the shell pipelined where it may be, the
+        // sequence-end kernel), so that every candidate is timed as the time
+        // loop will run it; the state it leaves is reset below
+        auto enq = [&](int n) { enqueue_seq(s, n); };
         // milliseconds per step of the current settings: the fastest of reps
         // repetitions of n steps, launched directly or, with graph, replayed
         // from a captured graph as the time loop runs them
@@ -1535,10 +1513,9 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         // buffer set is kept only if it wins); the CPML shell takes the
         // interior's z-block depth.  A first pass times every candidate over
         // 4 steps, best of 2 (the shell on the second stream where there is one); the
-        // three fastest are timed again over 24 steps (best of 4) replayed
-        // from a graph, as the time loop runs them, which separates
-        // near-ties the short pass cannot, each with the shell launch beside
-        // and behind the interior launch.
+        // five fastest are timed again over longer runs replayed from graphs,
+        // as the time loop runs them, which separates near-ties the short
+        // pass cannot (below).
         const int can_fuse = fused_ok(s) && ensure_fused_buffers(s) == LX_OK;
         s->kzs = 0;
         s->kconc = 1;
@@ -1569,29 +1546,43 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
             }
         }
         std::sort(tried.begin(), tried.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
-        Cand best = tried.front();
-        best.ms = 1e30f;
-        for (size_t q = 0; q < std::min<size_t>(3, tried.size()); ++q) {
+        // finalists: the five fastest, each with the shell beside and behind
+        // the interior launch; timed in three interleaved rounds (so that a
+        // drift of the device's clock falls on every candidate alike) of
+        // about 25 ms of steps each, replayed from graphs, best round kept
+        std::vector<Cand> fin;
+        for (size_t q = 0; q < std::min<size_t>(5, tried.size()); ++q)
             for (int conc = 1; conc >= 0; --conc) {
                 Cand c = tried[q];
                 if (conc == 0 && !(c.nw && s->ks)) continue;   // only the fused step has a shell launch
                 c.conc = conc;
+                c.ms = 1e30f;
+                fin.push_back(c);
+            }
+        const int nfin = (int)std::min(256.0, std::max(8.0, std::ceil(25.0 / std::max(tried.front().ms, 1e-3f))));
+        for (int round = 0; round < 3; ++round) {
+            for (Cand &c : fin) {
                 s->fused = c.nw != 0;
                 if (c.nw) s->nw = c.nw;
                 s->vec = c.vec;
                 s->rows = c.rows;
                 s->kz = c.kz;
-                s->kconc = conc;
-                lx_status st = time_steps(24, 4, &c.ms, s->graphs != 0);
+                s->kconc = c.conc;
+                float ms = 0;
+                lx_status st = time_steps(nfin, 2, &ms, s->graphs != 0);
                 if (st) return st;
+                c.ms = std::min(c.ms, ms);
                 if (verbose)
                     std::fprintf(stderr,
-                                 "lx_sim_autotune: finalist %s nw=%d vec=%d rows=%d kz=%d shell %s  %.4f ms per step\n",
-                                 s->fused ? "fused" : "half-step", c.nw, c.vec, c.rows, c.kz,
-                                 conc ? "beside" : "behind", c.ms);
-                if (c.ms < best.ms) best = c;
+                                 "lx_sim_autotune: finalist round %d %s nw=%d vec=%d rows=%d kz=%d shell %s  %.4f ms per "
+                                 "step\n",
+                                 round, s->fused ? "fused" : "half-step", c.nw, c.vec, c.rows, c.kz,
+                                 c.conc ? "beside" : "behind", ms);
             }
         }
+        Cand best = fin.front();
+        for (const Cand &c : fin)
+            if (c.ms < best.ms) best = c;
         s->fused = best.nw != 0;
         if (best.nw) s->nw = best.nw;
         s->vec = best.vec;
