@@ -207,8 +207,8 @@ lx_status lx_sim_sync(lx_sim *sim);
 lx_status lx_sim_read_field(lx_sim *sim, int32_t comp, double *dst, size_t n);
 
 /* No counterpart in the reference: lx_sim_read_field for callers that cannot
- * allocate device memory (code under HIP graph capture, tight serving
- * loops).  Same arguments and the same values; calls no hipMalloc/hipFree:
+ * allocate device memory (tight serving loops with a fixed footprint,
+ * harnesses that forbid allocation).  Same arguments and the same values; calls no hipMalloc/hipFree:
  * the component is gathered in slices of whole x indices through a fixed
  * 32 MiB device array per GPU (held from library load until the process
  * ends) and copied slice by slice into dst.  LX_EINVAL when one x slice

@@ -3,7 +3,9 @@
 // lx_sim_read_field gathers a component into an FP64 staging buffer of the
 // component's size (allocated on first use) and copies that to the host.
 // lx_sim_read_field_noalloc gives the same values to callers that cannot
-// allocate device memory (code under HIP graph capture, tight serving loops):
+// allocate device memory (tight serving loops with a fixed footprint,
+// harnesses that forbid allocation; it synchronizes the simulation's stream,
+// so it is not itself recorded into a graph):
 // it gathers the component in slices of whole x indices through `scratch`, a
 // fixed device array that every GPU the library runs on holds from module
 // load until the process ends, and copies each slice to its place in the
