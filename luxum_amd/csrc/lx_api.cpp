@@ -244,6 +244,8 @@ lx_status build_tables(lx_sim *s) {
         }
         s->ulo[a] = std::max(ul, s->slo[a]);
         s->uhi[a] = std::max(s->ulo[a], std::min(uh, s->shi[a]));
+        if (a == 0) s->unit_ke = ke[mid] == 1.f;
+        if (a == 2) s->unit_ke = s->unit_ke && ke[mid] == 1.f;
     }
     void *p = nullptr;
     lx_status st = dalloc(s, &p, host.size() * sizeof(float));

@@ -42,7 +42,7 @@ struct Group {
     BSim *d_sims = nullptr;
     BJob *d_jobs = nullptr;
     BAux *d_aux = nullptr;
-    int njobs[5] = {0, 0, 0, 0, 0};           // jobs per class, stored in class order (general 0..3, inner)
+    int njobs[kJobClasses] = {};              // jobs per class, stored in class order (general 0..3, inner, inner unit)
     int naux = 0;
     int par = 0;                              // buffer index holding the current state
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // two chunks starting at parity 0 / 1
@@ -106,7 +106,7 @@ int job_class(const lx_sim *s, int i0, int k0) {
     const int x0 = i0 - kTT, x1 = x0 + 64, z0 = k0 - kTT, z1 = z0 + kRH;
     if (x0 >= std::max(1, s->ulo[0]) && x1 <= std::min(s->N[0] - 1, s->uhi[0]) && z0 >= std::max(1, s->ulo[2]) &&
         z1 <= std::min(s->N[2] - 1, s->uhi[2]))
-        return kInnerJobs;
+        return s->unit_ke ? kInnerUnitJobs : kInnerJobs;
     if (!s->pml) return 0;
     return (x0 < s->slo[0] || x1 > s->shi[0] ? 1 : 0) | (z0 < s->slo[2] || z1 > s->shi[2] ? 2 : 0);
 }
@@ -294,7 +294,7 @@ lx_status rebuild(lx_batch *b) {
     for (Group &g : b->grp) {
         if (g.sims.empty()) continue;
         std::vector<BSim> hs(g.sims.size());
-        std::vector<BJob> cjobs[5];
+        std::vector<BJob> cjobs[kJobClasses];
         std::vector<BAux> aux;
         for (size_t q = 0; q < g.sims.size(); ++q) {
             lx_sim *s = g.sims[q];
@@ -383,7 +383,7 @@ lx_status rebuild(lx_batch *b) {
                 }
         }
         std::vector<BJob> jobs;
-        for (int c = 0; c < 5; ++c) {
+        for (int c = 0; c < kJobClasses; ++c) {
             g.njobs[c] = (int)cjobs[c].size();
             jobs.insert(jobs.end(), cjobs[c].begin(), cjobs[c].end());
         }
@@ -417,17 +417,18 @@ lx_status rebuild(lx_batch *b) {
 void enqueue_chunk(const lx_batch *b, const Group &g, int nt, int par) {
     hipStream_t s = b->stream;
     if (g.naux) launch_aux_chunk(g.d_aux, g.naux, nt, s);
-    const bool fork = (g.njobs[0] || g.njobs[1] || g.njobs[2]) && (g.njobs[3] || g.njobs[kInnerJobs]);
+    const bool fork =
+        (g.njobs[0] || g.njobs[1] || g.njobs[2]) && (g.njobs[3] || g.njobs[kInnerJobs] || g.njobs[kInnerUnitJobs]);
     if (fork) {
         (void)hipEventRecord(b->fork, s);
         (void)hipStreamWaitEvent(b->side, b->fork, 0);
     }
     hipStream_t so = fork ? b->side : s;
-    int off[6] = {0};
-    for (int c = 0; c < 5; ++c) off[c + 1] = off[c] + g.njobs[c];
-    static const int order[5] = {kInnerJobs, 3, 0, 1, 2};
+    int off[kJobClasses + 1] = {0};
+    for (int c = 0; c < kJobClasses; ++c) off[c + 1] = off[c] + g.njobs[c];
+    static const int order[kJobClasses] = {kInnerUnitJobs, kInnerJobs, 3, 0, 1, 2};
     for (int c : order) {
-        hipStream_t sc = (c == kInnerJobs || c == 3) ? s : so;
+        hipStream_t sc = (c == kInnerJobs || c == kInnerUnitJobs || c == 3) ? s : so;
         if (g.fp16) launch_tile2d<__half>(g.d_sims, g.d_jobs + off[c], g.njobs[c], c, nt, par, sc);
         else launch_tile2d<float>(g.d_sims, g.d_jobs + off[c], g.njobs[c], c, nt, par, sc);
     }

@@ -101,9 +101,13 @@ template <typename T>
 void launch_finish_batch(const B3Sim<T> *sims, int nsims, int n, int par_end, hipStream_t s);
 
 // Job class: kInnerJobs = every job's region lies inside the domain, outside
-// the CPML slabs, on rows with one z coefficient; 0..3 = the general path for
-// regions that meet an x slab (bit 0) and/or a z slab (bit 1).
+// the CPML slabs, on rows with one z coefficient; kInnerUnitJobs the same for
+// members whose E coefficients there are exactly 1 (dz = dx: the E update's
+// curl terms need no multiplication); 0..3 = the general path for regions
+// that meet an x slab (bit 0) and/or a z slab (bit 1).
 constexpr int kInnerJobs = 4;
+constexpr int kInnerUnitJobs = 5;
+constexpr int kJobClasses = 6;
 template <typename T>
 void launch_tile2d(const BSim *sims, const BJob *jobs, int njobs, int cls, int nt, int par, hipStream_t s);
 void launch_aux_chunk(const BAux *aux, int naux, int nt, hipStream_t s);

@@ -116,7 +116,10 @@ __device__ __forceinline__ float rdlane(float v, int l) {
 //
 // INNER jobs have their region inside [1, N-2] on both axes, outside the
 // CPML slabs and on rows with one z coefficient (the host checks): no psi,
-// one z coefficient per wave.  The other jobs come in four classes XZ by the
+// one z coefficient per wave.  For INNER jobs, XZ = 1 marks members whose E
+// coefficients there are exactly 1 (dz = dx, kappa 1): their E update forms
+// d1 - d2 without the two unit products, which rounds exactly as
+// d1*kez0 - d2*kex does with kez0 = kex = 1.  The other jobs come in four classes XZ by the
 // slabs their region meets (bit 0: an x slab, bit 1: a z slab); the psi of a
 // slab the region does not meet is zero throughout (c = 0 there), so those
 // classes carry only the psi they need.
@@ -125,6 +128,7 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
                                                      int nt, int par) {
     constexpr int NT = 64 * kNW;
     constexpr bool XP = !INNER && (XZ & 1), ZP = !INNER && (XZ & 2);   // x / z psi carried
+    constexpr bool UE = INNER && (XZ & 1);                              // unit E coefficients
     __shared__ float scb[256];
     __shared__ float sEtop[kNW][64], sHbot[kNW][64];
     __shared__ int sprb[256];                 // probe index
@@ -352,7 +356,9 @@ __global__ __launch_bounds__(64 * kNW) void k_tile2d(const BSim *__restrict__ si
             const float hl = wave_prev(hz[m]);
             const float d1 = hx[m] - hd;
             const float d2 = hz[m] - hl;
-            if (INNER) {
+            if (UE) {
+                ey[m] += cb[m] * (d1 - d2);
+            } else if (INNER) {
                 ey[m] += cb[m] * (d1 * kez0 - d2 * kex);
             } else {
                 const float4 z = sze[kRW * w + m];
@@ -450,6 +456,7 @@ void launch_tile2d(const BSim *sims, const BJob *jobs, int njobs, int cls, int n
     const dim3 grd((unsigned)njobs), blk(64 * kNW);
     switch (cls) {
     case kInnerJobs: hipLaunchKernelGGL((k_tile2d<T, true, 0>), grd, blk, 0, s, sims, jobs, nt, par); break;
+    case kInnerUnitJobs: hipLaunchKernelGGL((k_tile2d<T, true, 1>), grd, blk, 0, s, sims, jobs, nt, par); break;
     case 0: hipLaunchKernelGGL((k_tile2d<T, false, 0>), grd, blk, 0, s, sims, jobs, nt, par); break;
     case 1: hipLaunchKernelGGL((k_tile2d<T, false, 1>), grd, blk, 0, s, sims, jobs, nt, par); break;
     case 2: hipLaunchKernelGGL((k_tile2d<T, false, 2>), grd, blk, 0, s, sims, jobs, nt, par); break;

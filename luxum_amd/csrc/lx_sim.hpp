@@ -72,6 +72,7 @@ struct lx_sim {
     int tn = 0;
     int slo[3] = {0, 0, 0}, shi[3] = {0, 0, 0}, sw[3] = {0, 0, 0};
     int ulo[3] = {0, 0, 0}, uhi[3] = {0, 0, 0};   // uniform-coefficient range per axis, inside [slo, shi)
+    int unit_ke = 0;            // the E coefficients of x and z in that range are exactly 1 (batched 2D tiles)
     int64_t slab_cells[3] = {0, 0, 0};
     lx::TfList tfH{}, tfE{};
     int kz = 32;
