@@ -912,6 +912,7 @@ void lx_sim_destroy(lx_sim *s) {
     if (s->cs) (void)hipStreamSynchronize(s->cs);
     if (s->ss) (void)hipStreamSynchronize(s->ss);
     if (s->ks) (void)hipStreamSynchronize(s->ks);
+    for (lx::DmaSlot &d : s->snap_dma) lx::dma_release(&d);
     drop_graph(s);
     for (const auto &a : s->allocs) release_block(s->ctx, a.first, a.second);
     if (s->prec) (void)hipFree(s->prec);
@@ -1216,6 +1217,8 @@ lx_status lx_sim_sync(lx_sim *s) {
     lx_status st = drain(s->cs);
     if (st) return st;
     s->cs_dirty = false;
+    for (lx::DmaSlot &d : s->snap_dma)
+        if ((st = lx::dma_wait(&d))) return st;
     return s->ss ? drain(s->ss) : LX_OK;
 }
 
@@ -1272,10 +1275,12 @@ lx_status lx_sim_snapshot_async(lx_sim *s, int32_t comp, double *dst, size_t n) 
     if (n != m) return fail(LX_EINVAL, "buffer holds %zu values, component has %zu", n, m);
     HIP_TRY(hipSetDevice(s->ctx->device));
     member_enter(s);
-    lx_status sst = ensure_side_stream(s);
-    if (sst) return sst;
     const int slot = s->snap_next;
     s->snap_next ^= 1;
+    // the previous copy out of this staging slot must be finished (a DMA
+    // copy on the host, a stream copy on the device)
+    lx_status st = dma_wait(&s->snap_dma[slot]);
+    if (st) return st;
     if (s->stage_n[slot] < m) {
         HIP_TRY(hipEventSynchronize(s->done[slot]));
         if (s->stage[slot]) (void)hipFree(s->stage[slot]);
@@ -1283,13 +1288,18 @@ lx_status lx_sim_snapshot_async(lx_sim *s, int32_t comp, double *dst, size_t n) 
         HIP_TRY(hipMalloc(&s->stage[slot], sizeof(double) * m));
         s->stage_n[slot] = m;
     }
-    // the previous copy out of this staging slot must be finished
     HIP_TRY(hipStreamWaitEvent(s->cs, s->done[slot], 0));
-    lx_status st = gather(s, comp, s->stage[slot], ext);
-    if (st) return st;
+    if ((st = gather(s, comp, s->stage[slot], ext))) return st;
+    const size_t bytes = sizeof(double) * m;
+    if (dma_eligible(dst, s->stage[slot], bytes)) {   // runtime-allocated (pinned) host memory: the DMA engine
+        if ((st = dma_mark(&s->snap_dma[slot], s->cs))) return st;
+        return dma_enqueue(&s->snap_dma[slot], dst, s->stage[slot], bytes);
+    }
+    // pageable or registered memory: a copy on the side stream
+    if ((st = ensure_side_stream(s))) return st;
     HIP_TRY(hipEventRecord(s->ready[slot], s->cs));
     HIP_TRY(hipStreamWaitEvent(s->ss, s->ready[slot], 0));
-    HIP_TRY(hipMemcpyAsync(dst, s->stage[slot], sizeof(double) * m, hipMemcpyDeviceToHost, s->ss));
+    HIP_TRY(hipMemcpyAsync(dst, s->stage[slot], bytes, hipMemcpyDeviceToHost, s->ss));
     HIP_TRY(hipEventRecord(s->done[slot], s->ss));
     return LX_OK;
 }

@@ -705,7 +705,7 @@ lx_status lx_batch_sync(lx_batch *b) {
     HIP_TRY(hipStreamSynchronize(b->stream));
     for (lx_sim *s : b->sims) {
         s->batch_wait = nullptr;   // that step is complete
-        if (!s->cs_dirty && !s->ss) continue;
+        if (!s->cs_dirty && !s->ss && dma_idle(&s->snap_dma[0]) && dma_idle(&s->snap_dma[1])) continue;
         lx_status st = lx_sim_sync(s);
         if (st) return st;
     }

@@ -13,10 +13,11 @@
 // LX_EIO error, not a load-time dependency of libluxum.
 //
 // lx_writer: write() gathers the field on the compute stream into a device
-// staging buffer (the step loop waits only for that kernel), copies it to a
-// pinned host slot on the side stream, and a writer thread puts it into the
-// file once the copy has landed, so the time loop keeps running while files
-// are written.
+// staging buffer (the step loop waits only for that kernel); a writer thread
+// copies it to a pinned host slot on the DMA engine (lx_dma.hpp) once the
+// gather is done and puts it into the file, so the time loop keeps running
+// while files are written.  Without the HSA runtime's copy the slot is
+// copied on the side stream instead.
 #include <hip/hip_runtime.h>
 #include <dlfcn.h>
 
@@ -149,7 +150,10 @@ struct lx_writer {
     hid_t file = -1;
     double *dstage[2] = {nullptr, nullptr};    // device staging per slot
     double *host[2] = {nullptr, nullptr};      // pinned host slot
-    hipEvent_t copied[2] = {nullptr, nullptr}; // D2H of the slot done
+    hipEvent_t copied[2] = {nullptr, nullptr}; // D2H of the slot done (side-stream copies)
+    hipEvent_t gathered[2] = {nullptr, nullptr};   // gather into dstage done (side-stream copies)
+    lx::DmaSlot dma[2];
+    bool use_dma = false;
     double slot_t[2] = {0, 0};
     bool busy[2] = {false, false};
     int next = 0;
@@ -177,10 +181,14 @@ static void writer_loop(lx_writer *w) {
         }
         lx_status st = LX_OK;
         std::string msg;
-        if (hipEventSynchronize(w->copied[slot]) != hipSuccess) {
+        if (w->use_dma) {
+            if ((st = lx::dma_copy_marked(&w->dma[slot], w->host[slot], w->dstage[slot], sizeof(double) * w->n)))
+                msg = lx_last_error();
+        } else if (hipEventSynchronize(w->copied[slot]) != hipSuccess) {
             st = LX_EDEVICE;
             msg = "snapshot copy failed";
-        } else {
+        }
+        if (!st) {
             const hsize_t dims[3] = {(hsize_t)w->ext[0], (hsize_t)w->ext[1], (hsize_t)w->ext[2]};
             std::lock_guard<std::mutex> lk(g_h5_mu);
             st = put(a, w->file, dataset_name(w->slot_t[slot]), w->host[slot], dims);
@@ -293,8 +301,11 @@ lx_status lx_writer_open(lx_sim *s, const char *path, int32_t what, int32_t comp
     for (int q = 0; q < 2 && good; ++q) {
         good = hipMalloc(&w->dstage[q], sizeof(double) * w->n) == hipSuccess &&
                hipHostMalloc(&w->host[q], sizeof(double) * w->n, hipHostMallocDefault) == hipSuccess &&
-               hipEventCreateWithFlags(&w->copied[q], hipEventDisableTiming) == hipSuccess;
+               hipEventCreateWithFlags(&w->copied[q], hipEventDisableTiming) == hipSuccess &&
+               hipEventCreateWithFlags(&w->gathered[q], hipEventDisableTiming) == hipSuccess;
     }
+    if (good) w->use_dma = lx::dma_eligible(w->host[0], w->dstage[0], sizeof(double) * w->n) &&
+                           lx::dma_eligible(w->host[1], w->dstage[1], sizeof(double) * w->n);
     if (!good) {
         (void)hipGetLastError();
         w->th = std::thread();
@@ -319,13 +330,16 @@ lx_status lx_writer_write(lx_writer *w, double t) {
         w->next ^= 1;
     }
     lx_status st = hipSetDevice(s->ctx->device) == hipSuccess ? LX_OK : fail(LX_EDEVICE, "hipSetDevice failed");
-    if (!st) st = lx::ensure_side_stream(s);
+    if (!st && !w->use_dma) st = lx::ensure_side_stream(s);
     if (!st) lx::member_enter(s);
-    // gather on the compute stream (ordered with the steps), copy out on the side stream
+    // gather on the compute stream (ordered with the steps); the writer
+    // thread copies the slot out once the gather is done, or the side stream does
     if (!st) st = lx::gather_output(s, w->what, w->comp, w->dstage[slot], w->ext);
-    if (!st && (hipEventRecord(s->ready[0], s->cs) != hipSuccess || hipStreamWaitEvent(s->ss, s->ready[0], 0) != hipSuccess ||
-                hipMemcpyAsync(w->host[slot], w->dstage[slot], sizeof(double) * w->n, hipMemcpyDeviceToHost, s->ss) != hipSuccess ||
-                hipEventRecord(w->copied[slot], s->ss) != hipSuccess))
+    if (!st && w->use_dma) st = lx::dma_mark(&w->dma[slot], s->cs);
+    if (!st && !w->use_dma &&
+        (hipEventRecord(w->gathered[slot], s->cs) != hipSuccess || hipStreamWaitEvent(s->ss, w->gathered[slot], 0) != hipSuccess ||
+         hipMemcpyAsync(w->host[slot], w->dstage[slot], sizeof(double) * w->n, hipMemcpyDeviceToHost, s->ss) != hipSuccess ||
+         hipEventRecord(w->copied[slot], s->ss) != hipSuccess))
         st = fail(LX_EDEVICE, "snapshot copy could not be enqueued");
     std::lock_guard<std::mutex> lk(w->mu);
     if (st) {
@@ -358,8 +372,11 @@ lx_status lx_writer_close(lx_writer *w) {
         }
     }
     for (int q = 0; q < 2; ++q) {
+        lx::dma_release(&w->dma[q]);
         if (w->copied[q]) (void)hipEventSynchronize(w->copied[q]);
         if (w->copied[q]) (void)hipEventDestroy(w->copied[q]);
+        if (w->gathered[q]) (void)hipEventSynchronize(w->gathered[q]);
+        if (w->gathered[q]) (void)hipEventDestroy(w->gathered[q]);
         if (w->dstage[q]) (void)hipFree(w->dstage[q]);
         if (w->host[q]) (void)hipHostFree(w->host[q]);
     }

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/luxum.h"
+#include "lx_dma.hpp"
 #include "lx_internal.hpp"
 
 struct lx_ctx {
@@ -94,6 +95,7 @@ struct lx_sim {
     double *stage[2] = {nullptr, nullptr};
     size_t stage_n[2] = {0, 0};
     hipEvent_t ready[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
+    lx::DmaSlot snap_dma[2];    // copies into host memory allocated by the runtime (lx_dma.hpp)
     int snap_next = 0;
     double *rd_stage = nullptr;
     size_t rd_n = 0;

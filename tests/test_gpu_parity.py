@@ -172,6 +172,47 @@ def test_monitor_and_snapshot(gpu_ctx):
     assert np.array_equal(snap, g.field(LX_EZ))
 
 
+def test_snapshot_pinned_dma(gpu_ctx):
+    """Snapshots into pinned host memory (hipHostMalloc) go to the DMA engine
+    outside the streams: four in flight over two staging slots, with step
+    graphs (CPML shell beside the interior) between them, equal a second
+    run's synchronous readbacks at the same steps bit for bit."""
+    import ctypes
+
+    from luxum_amd.sim import Simulation
+    w = configs.config1(n=48, steps=200, pml=6, margin=4)
+    shape = w.p.field_shape(LX_EZ)
+    m = int(np.prod(shape))
+    hip = ctypes.CDLL("libamdhip64.so")
+    ptrs, bufs = [], []
+    try:
+        for _ in range(4):
+            q = ctypes.c_void_p()
+            assert hip.hipHostMalloc(ctypes.byref(q), ctypes.c_size_t(8 * m), 0) == 0
+            ptrs.append(q)
+            bufs.append(np.ctypeslib.as_array(ctypes.cast(q, ctypes.POINTER(ctypes.c_double)), shape=shape))
+            bufs[-1][...] = -1.0
+        g = Simulation(w.p, gpu_ctx)
+        g.set_materials(w.ids, w.eps_r)
+        for q in range(4):
+            g.step(37)
+            g.snapshot_async(LX_EZ, bufs[q])
+        g.sync()
+        r = Simulation(w.p, gpu_ctx)
+        r.set_materials(w.ids, w.eps_r)
+        for q in range(4):
+            r.step(37)
+            ref = r.field(LX_EZ)
+            assert np.array_equal(bufs[q], ref), f"snapshot {q}"
+        assert float(np.abs(bufs[3]).max()) > 0
+        say(f"pinned snapshots: 4 x {m} values equal the readbacks")
+        g.close()
+        r.close()
+    finally:
+        for q in ptrs:
+            hip.hipHostFree(q)
+
+
 def test_3d_cpml_aux_2000_steps(gpu_ctx):
     """Config 1's set-up at 128^3 over config 1's 2000 steps, FP32 and FP16
     storage side by side against one FP64 oracle run.  The error grows as the
