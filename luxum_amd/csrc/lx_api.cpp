@@ -1430,31 +1430,9 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
     static std::mutex mu;
     static std::map<Key, std::array<int, 7>> cache;   // (kz, nw, fused, vec, shell kz, shell stream, rows)
     const Key key{s->N[0], s->N[1], s->N[2], s->fp16, s->pml, s->planar, 0};
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        auto it = cache.find(key);
-        if (it != cache.end()) {
-            s->kz = it->second[0];
-            s->nw = it->second[1];
-            s->vec = it->second[3];
-            s->kzs = it->second[4];
-            s->kconc = it->second[5];
-            s->rows = it->second[6];
-            if (it->second[2] && fused_ok(s)) {
-                lx_status st = ensure_fused_buffers(s);
-                if (st) return st;
-                s->fused = 1;
-            } else if (s->F_alt && !s->planar) {
-                release_fused_buffers(s);
-            }
-            s->gen++;
-            drop_graph(s);
-            if (kz_out) *kz_out = s->kz;
-            return prepare_graphs(s);
-        }
-    }
     // LX_TUNE=fused,nw,vec,kz,shell_concurrent[,rows] pins the result without
-    // timing (a profiling process repeats another process's choice)
+    // timing, ahead of any cached result (a profiling process repeats another
+    // process's choice)
     if (const char *pin = std::getenv("LX_TUNE"); pin && !s->planar) {
         int f = 0, nw = 0, vec = 0, kz = 0, conc = 0, rows = 0;
         const int nf = std::sscanf(pin, "%d,%d,%d,%d,%d,%d", &f, &nw, &vec, &kz, &conc, &rows);
@@ -1481,6 +1459,29 @@ lx_status lx_sim_autotune(lx_sim *s, int32_t *kz_out) {
         drop_graph(s);
         if (kz_out) *kz_out = s->kz;
         return prepare_graphs(s);
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            s->kz = it->second[0];
+            s->nw = it->second[1];
+            s->vec = it->second[3];
+            s->kzs = it->second[4];
+            s->kconc = it->second[5];
+            s->rows = it->second[6];
+            if (it->second[2] && fused_ok(s)) {
+                lx_status st = ensure_fused_buffers(s);
+                if (st) return st;
+                s->fused = 1;
+            } else if (s->F_alt && !s->planar) {
+                release_fused_buffers(s);
+            }
+            s->gen++;
+            drop_graph(s);
+            if (kz_out) *kz_out = s->kz;
+            return prepare_graphs(s);
+        }
     }
     if (!s->planar) {
         hipEvent_t e0, e1;
