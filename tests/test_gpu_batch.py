@@ -346,3 +346,48 @@ def test_batch_members_with_storage_scale(gpu_ctx):
     assert e <= 1e-6 and pe <= 1e-6
     b0.close()
     b1.close()
+
+
+def test_batch_member_pinned_snapshots(gpu_ctx):
+    """Snapshots of batch members into pinned host memory (the DMA-engine
+    path) between batch steps, one 2D tiled and one 3D member: complete after
+    the batch's sync alone, and equal to each member's synchronous readback."""
+    import ctypes
+
+    from luxum_amd.params import LX_EZ
+    from luxum_amd.sim import Batch
+    ws = [phc(160, 5), configs.box3d(64, 40, 36, steps=80)]
+    comps = [LX_EY, LX_EZ]
+    members = [make(w, gpu_ctx) for w in ws]
+    hip = ctypes.CDLL("libamdhip64.so")
+    ptrs, bufs = [], []
+    b = Batch(gpu_ctx)
+    try:
+        for w, c in zip(ws, comps):
+            shape = w.p.field_shape(c)
+            m = int(np.prod(shape))
+            q = ctypes.c_void_p()
+            assert hip.hipHostMalloc(ctypes.byref(q), ctypes.c_size_t(8 * m), 0) == 0
+            ptrs.append(q)
+            bufs.append(np.ctypeslib.as_array(ctypes.cast(q, ctypes.POINTER(ctypes.c_double)), shape=shape))
+            bufs[-1][...] = -1.0
+        for s in members:
+            b.add(s)
+        say("batch of a 2D and a 3D member: step 24, pinned snapshots and readbacks, step 16, batch sync")
+        b.step(24)
+        refs = []
+        for s, c, buf in zip(members, comps, bufs):
+            s.snapshot_async(c, buf)
+            refs.append(s.field(c))      # the same state, read synchronously
+        b.step(16)
+        b.sync()
+        for w, g, ref in zip(ws, bufs, refs):
+            say(f"{w.name}: snapshot vs readback max diff {float(np.abs(g - ref).max()):.2e}")
+            assert np.array_equal(g, ref)
+            assert float(np.abs(ref).max()) > 0
+    finally:
+        b.close()
+        for s in members:
+            s.sync()
+        for q in ptrs:
+            hip.hipHostFree(q)
