@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <hsa/hsa.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 
@@ -30,7 +31,7 @@ struct DmaSlot {
     hsa_signal_t sig{0};
     uint64_t *flag = nullptr;
     uint64_t seq = 0;
-    bool failed = false;
+    std::atomic<bool> failed{false};   // set by the copying thread before the signal drops
 };
 
 // dst is host memory allocated by the HSA runtime and src device memory, and
