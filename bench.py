@@ -720,7 +720,7 @@ def main():
     # reference-shaped loop, the snapshot section's three runs and its |E|^2
     # run), so that its probe buffer never grows mid-run (a growth drops the
     # captured step graphs, and the next calls would time their capture)
-    total = max(args.steps + args.warmup + args.kernel_steps + 40 + 3 * args.snap_steps + 30 + 64, 2000)
+    total = max(args.steps + args.warmup + args.kernel_steps + 80 + 3 * args.snap_steps + 30 + 64, 2000)
     tag = "FP16 storage" if args.fp16 else "FP32"
     if args.workload == "phc_slab":
         w = configs.config2(steps=total, precision=prec)
@@ -773,8 +773,11 @@ def main():
     step_gbs = info.bytes_per_step / (ms_step * 1e-3) / 1e9
     mx, _ = sim.monitor()
     # the reference's time loop prints max|E|^2 every step
-    # (maxwell_three_dim_pml.cc:167-173, 266-276); the C++ facade runs it so
-    # by default: one step, then lx_sim_max_e2, per iteration
+    # (maxwell_three_dim_pml.cc:167-173, 266-276): at the top of iteration t
+    # the max that the end of iteration t-1 computed.  The C++ facade runs it
+    # so by default: step t, then the max of step t-1 (its monitor, started
+    # after step t-1, ran beside step t), then start the monitor of step t.
+    # Timed beside it: one step, then the synchronous lx_sim_max_e2.
     loop_n = 40
     t1 = time.perf_counter()
     for _ in range(loop_n):
@@ -788,6 +791,15 @@ def main():
     for _ in range(loop_n):
         sim.step(1)
         sim.max_e2()
+    ms_loop_sync = (time.perf_counter() - t1) * 1e3 / loop_n
+    sim.sync()
+    t1 = time.perf_counter()
+    for q in range(loop_n):
+        sim.step(1)
+        if q:
+            sim.max_e2_finish()
+        sim.max_e2_start()
+    sim.max_e2_finish()
     ms_loop = (time.perf_counter() - t1) * 1e3 / loop_n
 
     out = {
@@ -816,12 +828,15 @@ def main():
                      "bytes_per_launch": dbytes, "ms_per_launch": round(dms, 5)},
     }
     out["config"]["shell_pipelined"] = bool(info.shell_pipelined)
-    out["reference_loop"] = {"iterations": loop_n, "ms_per_step": round(ms_loop, 5), "ms_max_e2": round(ms_max, 5),
+    out["reference_loop"] = {"iterations": loop_n, "ms_per_step": round(ms_loop, 5),
+                             "ms_per_step_synchronous": round(ms_loop_sync, 5), "ms_max_e2": round(ms_max, 5),
                              "ms_monitor": round(ms_mon, 5),
-                             "note": "lx_sim_step(1) then lx_sim_max_e2 (FP64 max|E|^2 over the E components, "
-                                     "non-finite guard) per iteration, the facade's default loop; ms_monitor: "
-                                     "lx_sim_monitor (max, energy over all six components); the main line above "
-                                     "steps in graphs without a per-step monitor"}
+                             "note": "the facade's default loop: lx_sim_step(1), then lx_sim_max_e2_finish (FP64 "
+                                     "max|E|^2 over the E components of the previous step, non-finite guard; its "
+                                     "monitor ran beside this step), then lx_sim_max_e2_start, per iteration; "
+                                     "synchronous: lx_sim_step(1) then lx_sim_max_e2; ms_monitor: lx_sim_monitor "
+                                     "(max, energy over all six components); the main line above steps in graphs "
+                                     "without a per-step monitor"}
     if not args.no_snap and args.workload == "cube":
         out["snapshots"] = snapshot_section(args, w, ctx, sim)
     sim.close()

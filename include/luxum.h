@@ -239,6 +239,19 @@ lx_status lx_sim_monitor(lx_sim *sim, double *max_e2, double *energy);
  * 6*s+1).  LX_EUNSTABLE when an E value is not finite.  Synchronous. */
 lx_status lx_sim_max_e2(lx_sim *sim, double *max_e2);
 
+/* No counterpart in the reference: lx_sim_max_e2 in two halves, for time
+ * loops that print the previous step's max (maxwell_three_dim_pml.cc:166-173
+ * prints at the top of iteration t what :266-276 computed at the end of
+ * t-1).  _start enqueues the E-only monitor of the state after the steps
+ * enqueued so far on a second stream and returns; the next lx_sim_step of a
+ * fused 3D simulation (one step, no point source) runs beside it, and any
+ * later work that writes the buffer it reads waits for it.  _finish waits
+ * and returns the value lx_sim_max_e2 would have returned at _start
+ * (LX_EUNSTABLE likewise; LX_EINVAL without a started monitor).  A second
+ * _start before _finish drops the first result. */
+lx_status lx_sim_max_e2_start(lx_sim *sim);
+lx_status lx_sim_max_e2_finish(lx_sim *sim, double *max_e2);
+
 lx_status lx_sim_get_info(lx_sim *sim, lx_sim_info *info);
 
 /* The simulation's compute stream (hipStream_t) for event timing. */

@@ -245,14 +245,24 @@ class maxwell {
     void run(const char *file, int what, int comp, int every, int from, bool print_after, bool timed) {
         fileio writer(file, sim_, what, comp);
         double mx = 0.0;
+        bool started = false;
         const auto begin = std::chrono::steady_clock::now();
         for (int t = 0; t < num_timesteps; ++t) {
             const bool print = print_every > 0 && mpi.rank == 0 && t % print_every == 0;
-            if (print && !print_after) *out << "t: " << t << ", max: " << mx << std::endl;
             check(lx_sim_step(sim_, 1));
-            // The max printed at the top of iteration t+1 is that of step t.
-            const bool need = print_after ? print : print_every > 0 && (t + 1) % print_every == 0;
-            if (need) check(lx_sim_max_e2(sim_, &mx));
+            // The max printed at the top of iteration t is that of step t-1:
+            // its monitor, started after step t-1, ran beside step t.  The
+            // lines printed are the reference's; only the step is enqueued
+            // before the print.
+            if (started) check(lx_sim_max_e2_finish(sim_, &mx));
+            started = false;
+            if (print && !print_after) *out << "t: " << t << ", max: " << mx << std::endl;
+            if (print_after) {
+                if (print) check(lx_sim_max_e2(sim_, &mx));
+            } else if (print_every > 0 && (t + 1) % print_every == 0 && t + 1 < num_timesteps) {
+                check(lx_sim_max_e2_start(sim_));
+                started = true;
+            }
             if (print && print_after) *out << "t: " << t << ", max: " << mx << std::endl;
             if (t % every == 0 && t >= from) writer.writeHDF5(t);
         }

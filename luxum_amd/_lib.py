@@ -24,7 +24,8 @@ EXPORTS = (
     "lx_ctx_create", "lx_ctx_destroy", "lx_ctx_mem_info",
     "lx_sim_create", "lx_sim_destroy", "lx_sim_set_materials", "lx_sim_set_materials3", "lx_sim_set_probes",
     "lx_sim_step", "lx_sim_sync", "lx_sim_read_field", "lx_sim_read_field_noalloc", "lx_sim_snapshot_async",
-    "lx_sim_read_probes", "lx_sim_monitor", "lx_sim_max_e2", "lx_sim_get_info", "lx_sim_stream",
+    "lx_sim_read_probes", "lx_sim_monitor", "lx_sim_max_e2", "lx_sim_max_e2_start", "lx_sim_max_e2_finish",
+    "lx_sim_get_info", "lx_sim_stream",
     "lx_sim_time_kernels", "lx_sim_autotune",
     "lx_batch_create", "lx_batch_destroy", "lx_batch_add", "lx_batch_remove",
     "lx_batch_step", "lx_batch_sync",
@@ -138,6 +139,8 @@ def lib():
         "lx_sim_read_probes": ([vp, vp, C.c_size_t, C.POINTER(C.c_int32)], st),
         "lx_sim_monitor": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], st),
         "lx_sim_max_e2": ([vp, C.POINTER(C.c_double)], st),
+        "lx_sim_max_e2_start": ([vp], st),
+        "lx_sim_max_e2_finish": ([vp, C.POINTER(C.c_double)], st),
         "lx_sim_get_info": ([vp, C.POINTER(SimInfo)], st),
         "lx_sim_stream": ([vp], vp),
         "lx_sim_time_kernels": ([vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),

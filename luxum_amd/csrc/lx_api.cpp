@@ -836,6 +836,10 @@ void member_enter(lx_sim *s) {
         (void)hipStreamWaitEvent(s->cs, s->batch_wait, 0);
         s->batch_wait = nullptr;
     }
+    if (s->mguard) {   // a started monitor still reads the fields
+        (void)hipStreamWaitEvent(s->cs, s->mdone, 0);
+        s->mguard = nullptr;
+    }
     s->cs_dirty = true;
 }
 
@@ -913,6 +917,7 @@ void lx_sim_destroy(lx_sim *s) {
     if (s->ss) (void)hipStreamSynchronize(s->ss);
     if (s->ks) (void)hipStreamSynchronize(s->ks);
     for (lx::DmaSlot &d : s->snap_dma) lx::dma_release(&d);
+    if (s->ms) (void)hipStreamSynchronize(s->ms);
     drop_graph(s);
     for (const auto &a : s->allocs) release_block(s->ctx, a.first, a.second);
     if (s->prec) (void)hipFree(s->prec);
@@ -925,6 +930,10 @@ void lx_sim_destroy(lx_sim *s) {
     give_stream(s->ctx, s->cs);
     give_stream(s->ctx, s->ss);
     give_stream(s->ctx, s->ks);
+    give_stream(s->ctx, s->ms);
+    if (s->mstart) (void)hipEventDestroy(s->mstart);
+    if (s->mdone) (void)hipEventDestroy(s->mdone);
+    if (s->mres) (void)hipHostFree(s->mres);
     if (s->kfork) (void)hipEventDestroy(s->kfork);
     if (s->kjoin) (void)hipEventDestroy(s->kjoin);
     delete s;
@@ -999,7 +1008,7 @@ lx_status lx_sim_create(lx_ctx *ctx, const lx_params *p, lx_sim **out) {
     if ((st = dalloc(s, &q, 64))) return bail(st);
     s->counter = (int *)q;
     launch_zero_bytes(s->counter, 64, s->cs);
-    if ((st = dalloc(s, &q, sizeof(double) * (4 + 3 * (size_t)kMonBlocks)))) return bail(st);
+    if ((st = dalloc(s, &q, 2 * sizeof(double) * (4 + 3 * (size_t)kMonBlocks)))) return bail(st);
     s->mon = (double *)q;
     if ((st = build_tables(s))) return bail(st);
     if ((st = build_tfsf(s))) return bail(st);
@@ -1165,6 +1174,16 @@ lx_status lx_sim_step(lx_sim *s, int32_t nsteps) {
     if (!s || nsteps < 0) return fail(LX_EINVAL, "lx_sim_step: bad arguments");
     if (nsteps == 0) return LX_OK;
     HIP_TRY(hipSetDevice(s->ctx->device));
+    // One fused step from the buffer a started monitor reads writes the other
+    // buffer (its H corrections touch H, which the E-only monitor does not
+    // read), so it may run beside the monitor; the next writer of that buffer
+    // waits.  A point source writes E into the buffer read, and the other
+    // paths update in place: they wait at once.
+    const void *keep = nullptr;
+    if (s->mguard && s->fused && nsteps == 1 && !s->p.use_ps && s->p.monitor_every <= 0 && s->F == s->mguard) {
+        keep = s->mguard;
+        s->mguard = nullptr;
+    }
     member_enter(s);
     lx_status st = ensure_probe_cap(s, s->steps + nsteps);
     if (st) return st;
@@ -1189,6 +1208,7 @@ lx_status lx_sim_step(lx_sim *s, int32_t nsteps) {
         if ((st = check_launch())) return st;
         s->steps += n;
         left -= n;
+        if (keep) s->mguard = keep;
         if (s->p.monitor_every > 0) {   // the NaN guard: the E-only pass (H feeds E within a step)
             double mx;
             if ((st = lx_sim_max_e2(s, &mx))) return st;
@@ -1219,6 +1239,7 @@ lx_status lx_sim_sync(lx_sim *s) {
     s->cs_dirty = false;
     for (lx::DmaSlot &d : s->snap_dma)
         if ((st = lx::dma_wait(&d))) return st;
+    if (s->ms && (st = drain(s->ms))) return st;
     return s->ss ? drain(s->ss) : LX_OK;
 }
 
@@ -1360,6 +1381,50 @@ lx_status lx_sim_max_e2(lx_sim *s, double *max_e2) {
     if (st) return st;
     *max_e2 = out[0];
     if (out[2] > 0) return fail(LX_EUNSTABLE, "non-finite E values after %d steps", s->steps);
+    return LX_OK;
+}
+
+lx_status lx_sim_max_e2_start(lx_sim *s) {
+    if (!s) return fail(LX_EINVAL, "null simulation");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    if (s->mpend) HIP_TRY(hipEventSynchronize(s->mdone));   // its result is dropped
+    s->mpend = 0;
+    if (!s->ms) {
+        lx_status st = take_stream(s->ctx, &s->ms);
+        if (st) return st;
+        HIP_TRY(hipEventCreateWithFlags(&s->mstart, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&s->mdone, hipEventDisableTiming));
+        void *p = nullptr;
+        HIP_TRY(hipHostMalloc(&p, 4 * sizeof(double), hipHostMallocCoherent | hipHostMallocMapped));
+        s->mres = (double *)p;
+    }
+    member_enter(s);
+    const double cb0 = s->c0 * s->dt / s->p.dx;
+    double *scratch = s->mon + 4 + 3 * (size_t)kMonBlocks;   // the second half: not the synchronous monitor's
+    double *res = nullptr;
+    HIP_TRY(hipHostGetDevicePointer((void **)&res, s->mres, 0));
+    HIP_TRY(hipEventRecord(s->mstart, s->cs));
+    HIP_TRY(hipStreamWaitEvent(s->ms, s->mstart, 0));
+    if (s->fp16) launch_monitor(make_grid<__half>(s), s->planar, cb0, scratch, true, s->ms, res);
+    else launch_monitor(make_grid<float>(s), s->planar, cb0, scratch, true, s->ms, res);
+    lx_status st = check_launch();
+    if (st) return st;
+    HIP_TRY(hipEventRecord(s->mdone, s->ms));
+    s->mpend = 1;
+    s->mguard = s->F;
+    return LX_OK;
+}
+
+lx_status lx_sim_max_e2_finish(lx_sim *s, double *max_e2) {
+    if (!s || !max_e2) return fail(LX_EINVAL, "lx_sim_max_e2_finish: null pointer");
+    if (!s->mpend) return fail(LX_EINVAL, "lx_sim_max_e2_finish: no started monitor");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(hipEventSynchronize(s->mdone));
+    s->mpend = 0;
+    s->mguard = nullptr;   // complete: nothing on cs needs to wait for it
+    const double mx = s->mres[0], bad = s->mres[2];
+    *max_e2 = mx / (s->sscale * s->sscale);
+    if (bad > 0) return fail(LX_EUNSTABLE, "non-finite E values");
     return LX_OK;
 }
 

@@ -616,6 +616,10 @@ lx_status lx_batch_step(lx_batch *b, int32_t nsteps) {
     for (Group &g : b->grp) grouped.insert(grouped.end(), g.sims.begin(), g.sims.end());
     for (Group3 &g : b->grp3) grouped.insert(grouped.end(), g.sims.begin(), g.sims.end());
     for (lx_sim *s : grouped) {
+        if (s->mguard) {   // a started monitor still reads the member's fields
+            HIP_TRY(hipStreamWaitEvent(b->stream, s->mdone, 0));
+            s->mguard = nullptr;
+        }
         if (!s->cs_dirty) continue;
         s->cs_dirty = false;
         const hipError_t q = hipStreamQuery(s->cs);

@@ -259,10 +259,12 @@ void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double scale, do
 // FP64 monitor: out[0] = max |E|^2 (reference index range), out[1] = sum of
 // eps_r*E^2 + Hs^2 (0 with e_only), out[2] = non-finite count (of the E
 // components only with e_only); out holds 4 + 3*kMonBlocks
-// doubles (the per-block partial triples follow the result).
+// doubles (the per-block partial triples follow the result).  With res the
+// result triple goes there instead (host-mapped memory: no copy follows).
 constexpr int kMonBlocks = 2048;
 template <typename T>
-void launch_monitor(const Grid<T> &g, int planar, double cb0, double *out, bool e_only, hipStream_t s);
+void launch_monitor(const Grid<T> &g, int planar, double cb0, double *out, bool e_only, hipStream_t s,
+                    double *res = nullptr);
 
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 // material ids, host C order (i, j, k) -> device layout (k, j, x)

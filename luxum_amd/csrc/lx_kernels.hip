@@ -1001,13 +1001,13 @@ void launch_gather_e2(const Grid<T> &g, int ex, int ey, int ez, double scale, do
     hipLaunchKernelGGL((k_gather<T, true>), grd, blk, 0, s, g, (const T *)g.F, ex, ey, ez, scale, dst);
 }
 template <typename T>
-void launch_monitor(const Grid<T> &g, int planar, double cb0, double *out, bool e_only, hipStream_t s) {
+void launch_monitor(const Grid<T> &g, int planar, double cb0, double *out, bool e_only, hipStream_t s, double *res) {
     const long long rows = (long long)(planar ? 1 : g.ny) * g.nz;
     const int nb = (int)std::min<long long>(kMonBlocks, std::max<long long>(rows, 1));
     double *part = out + 4;
     if (e_only) hipLaunchKernelGGL((k_monitor<T, true>), dim3(nb), dim3(256), 0, s, g, planar, cb0, part);
     else hipLaunchKernelGGL((k_monitor<T, false>), dim3(nb), dim3(256), 0, s, g, planar, cb0, part);
-    hipLaunchKernelGGL(k_monitor_final, dim3(1), dim3(256), 0, s, part, nb, out);
+    hipLaunchKernelGGL(k_monitor_final, dim3(1), dim3(256), 0, s, part, nb, res ? res : out);
 }
 void launch_ids_to_device(const uint8_t *src, uint8_t *dst, int nx, int ny, int nz, int px, long long plane,
                           hipStream_t s) {
@@ -1039,7 +1039,7 @@ void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
                                       const ProbeArgs &, Launch, hipStream_t);                              \
     template void launch_point_source<T>(const Grid<T> &, const PointSrc &, const ProbeArgs &, int, hipStream_t); \
     template void launch_gather<T>(const Grid<T> &, int, int, int, int, double, double *, hipStream_t);      \
-    template void launch_monitor<T>(const Grid<T> &, int, double, double *, bool, hipStream_t);                            \
+    template void launch_monitor<T>(const Grid<T> &, int, double, double *, bool, hipStream_t, double *);                  \
     template void launch_gather_e2<T>(const Grid<T> &, int, int, int, double, double *, hipStream_t);              \
     template void launch_tfsf_batch<T>(const B3Sim<T> *, const int2 *, int, int, int, int, hipStream_t);   \
     template void launch_finish_batch<T>(const B3Sim<T> *, int, int, int, hipStream_t);                   \

@@ -99,7 +99,17 @@ struct lx_sim {
     int snap_next = 0;
     double *rd_stage = nullptr;
     size_t rd_n = 0;
-    double *mon = nullptr;
+    double *mon = nullptr;      // monitor scratch: the synchronous monitor's, then the started one's
+    // the E-only monitor started by lx_sim_max_e2_start: on stream ms after
+    // event mstart (recorded on cs), complete at event mdone; its result
+    // triple lands in pinned host memory mres.  mguard is the field buffer it
+    // reads, while work on cs that may write that buffer must still wait for
+    // mdone (nullptr: nothing to wait for).
+    hipStream_t ms = nullptr;
+    hipEvent_t mstart = nullptr, mdone = nullptr;
+    double *mres = nullptr;
+    int mpend = 0;
+    const void *mguard = nullptr;
     // graphs of kGraphSteps[i]-step sequences, one per buffer state of the
     // fused step (keyed by the field buffer read first, gF), kept while the
     // fused buffer pair swaps after odd step counts

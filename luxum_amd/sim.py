@@ -127,6 +127,16 @@ class Simulation:
         check(lib().lx_sim_max_e2(self._h, C.byref(mx)))
         return mx.value
 
+    def max_e2_start(self) -> None:
+        """Start the reference monitor of the current state beside the next step."""
+        check(lib().lx_sim_max_e2_start(self._h))
+
+    def max_e2_finish(self) -> float:
+        """The started monitor's max|E|^2 (waits for it)."""
+        mx = C.c_double()
+        check(lib().lx_sim_max_e2_finish(self._h, C.byref(mx)))
+        return mx.value
+
     def info(self) -> SimInfo:
         inf = SimInfo()
         check(lib().lx_sim_get_info(self._h, C.byref(inf)))
