@@ -4,7 +4,8 @@ the next iteration, :166-173) computed beside the next step.  Every value
 must equal the synchronous lx_sim_max_e2 of the same state bit for bit, and
 the fields must be those of a run without it, on the paths that overlap (one
 fused 3D step, FP32 and FP16) and on those that wait (point source, 2D,
-several steps, a batch step)."""
+several steps, a batch step).  Also: an LX_TUNE pin applies over the
+autotuner's cached result for the same grid.
 import sys
 
 import numpy as np
@@ -104,5 +105,30 @@ def test_started_monitor_across_longer_steps_and_a_batch(gpu_ctx):
     with pytest.raises(Exception):
         a.max_e2_finish()         # nothing started
     bt.close()
+    a.close()
+    b.close()
+
+
+def test_autotune_pin_over_cached_result(gpu_ctx, monkeypatch):
+    """LX_TUNE pins the settings even after the same grid was autotuned in
+    this process (the cached result does not override the pin), and the
+    pinned run's fields equal the tuned run's (every variant is bit-identical)."""
+    w = configs.config1(n=48, steps=80, pml=6, margin=4)
+    monkeypatch.delenv("LX_TUNE", raising=False)
+    a = make(w, gpu_ctx)
+    a.autotune()
+    ia = a.info()
+    pin = (1, 8, 1, 5, 0, 3) if (ia.kz, ia.interior_rows) != (5, 3) else (1, 4, 1, 6, 1, 0)
+    monkeypatch.setenv("LX_TUNE", ",".join(map(str, pin)))
+    b = make(w, gpu_ctx)
+    b.autotune()
+    ib = b.info()
+    say(f"tuned kz {ia.kz} rows {ia.interior_rows} vec {ia.cells_per_lane}; pinned kz {ib.kz} rows {ib.interior_rows} "
+        f"vec {ib.cells_per_lane}")
+    assert (ib.kz, ib.waves, ib.cells_per_lane, ib.interior_rows) == (pin[3], pin[1], pin[2], pin[5])
+    a.step(40)
+    b.step(40)
+    for c in w.p.components():
+        assert np.array_equal(a.field(c), b.field(c)), c
     a.close()
     b.close()
