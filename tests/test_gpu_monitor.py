@@ -5,7 +5,7 @@ must equal the synchronous lx_sim_max_e2 of the same state bit for bit, and
 the fields must be those of a run without it, on the paths that overlap (one
 fused 3D step, FP32 and FP16) and on those that wait (point source, 2D,
 several steps, a batch step).  Also: an LX_TUNE pin applies over the
-autotuner's cached result for the same grid.
+autotuner's cached result for the same grid."""
 import sys
 
 import numpy as np
